@@ -1,0 +1,224 @@
+// pda_kernels.hip -- gfx950 kernels of the Monte-Carlo PDA hot path.
+//
+// walk_kernel replaces gSMFRETda's mc_kernel (src/mc.cu:83-197) together
+// with the host histogram and chi2 of src/streamWorker.cpp:184-206:
+//   * one 64-lane workgroup (one wavefront) per tile of 64 consecutive
+//     (burst, resample) units, burst-major, so the lanes of a wave are
+//     resamples of the same one or two bursts;
+//   * the tile's photons (burst-relative uint32 ticks + channel-run links)
+//     are staged in LDS with coalesced loads, then each lane runs its own
+//     dwell walk, locating every window end by a binary search in LDS
+//     instead of rescanning photons from global memory;
+//   * random numbers are stateless Philox4x32-10 blocks keyed by
+//     (seed, set, window, sub-stream, resample, burst) -- no RNG state in
+//     memory, nothing to initialise (replaces setup_kernel,
+//     src/gen_rand.cuh:7-25);
+//   * each workgroup keeps an LDS int32 histogram, adds it to the run's
+//     accumulator with device-scope atomics, and the last workgroup to
+//     finish computes chi2 and resets the accumulator and ticket, so one
+//     launch is the whole evaluation and needs no memset.
+#include <hip/hip_runtime.h>
+
+#include "pda_math.h"
+#include "pda_types.h"
+#include "pda_walk.h"
+
+namespace pda {
+
+__global__ __launch_bounds__(kWave) void walk_kernel(
+    const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
+    const uint2* __restrict__ links, const Model* __restrict__ model, const RunArgs a,
+    const float* __restrict__ obs, int32_t* __restrict__ hist_acc, uint32_t* __restrict__ ticket,
+    ResultHeader* __restrict__ result, float* __restrict__ outE, int32_t* __restrict__ outAcc,
+    float* __restrict__ outF) {
+    extern __shared__ uint32_t smem[];
+    __shared__ Model sM;
+    __shared__ uint32_t s_last;
+    const uint32_t lane = threadIdx.x;
+    int32_t* s_hist = reinterpret_cast<int32_t*>(smem);
+    const uint32_t hist_words = (uint32_t(a.B) + 1u) & ~1u;
+    uint32_t* s_tick = smem + hist_words;
+    uint2* s_link = reinterpret_cast<uint2*>(s_tick + ((a.cap + 1u) & ~1u));
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(model);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&sM);
+        for (uint32_t i = lane; i < sizeof(Model) / 4; i += kWave) dst[i] = src[i];
+        for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) s_hist[i] = 0;
+    }
+    __syncthreads();
+
+    // XCD-aware order: workgroup g runs on XCD g % 8 and walks the contiguous
+    // tile chunk of that XCD, so neighbouring tiles (which share bursts)
+    // meet in the same L2.
+    const uint32_t xcd = blockIdx.x & 7u;
+    const uint32_t per_xcd = gridDim.x >> 3;
+    const uint32_t t_begin = xcd * a.tiles_per_xcd;
+    uint32_t t_end = t_begin + a.tiles_per_xcd;
+    if (t_end > a.ntiles) t_end = a.ntiles;
+    for (uint32_t tile = t_begin + (blockIdx.x >> 3); tile < t_end; tile += per_xcd) {
+        const uint32_t u0 = tile * kWave;
+        const uint32_t ulast = (u0 + kWave < a.nunits ? u0 + kWave : a.nunits) - 1u;
+        const uint32_t bfirst = a.begin + u0 / a.R;
+        const uint32_t blast = a.begin + ulast / a.R;
+        const uint32_t p0 = bursts[bfirst].off;
+        const BurstDesc dlast = bursts[blast];
+        const uint32_t span = dlast.off + dlast.P - p0;
+        const bool staged = span <= a.cap;
+        if (staged) {
+            for (uint32_t i = lane; i < span; i += kWave) {
+                s_tick[i] = ticks[p0 + i];
+                s_link[i] = links[p0 + i];
+            }
+        }
+        __syncthreads();
+        const uint32_t u = u0 + lane;
+        if (u < a.nunits) {
+            const uint32_t bl = u / a.R;
+            const uint32_t r = u - bl * a.R;
+            const uint32_t b = a.begin + bl;
+            const BurstDesc d = bursts[b];
+            const uint32_t* tk = staged ? s_tick + (d.off - p0) : ticks + d.off;
+            const uint2* lk = staged ? s_link + (d.off - p0) : links + d.off;
+            const UnitKey key{a.seed, a.set, r, b};
+            const WalkOut o = walk_unit(sM, key, d.P, d.stop_rel, d.T, tk, lk, a);
+            const float E = o.F > 0.0f ? o.acc / o.F : 0.0f;
+            if (E >= 0.0f && E < 1.0f) {
+                const int bin = int(floor(double(E) * double(a.B)));
+                if (bin < a.B) atomicAdd(&s_hist[bin], 1);
+            }
+            if (outE) outE[u] = E;
+            if (outAcc) outAcc[u] = int32_t(o.acc);
+            if (outF) outF[u] = o.F;
+        }
+        __syncthreads();
+    }
+
+    for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
+        const int32_t v = s_hist[i];
+        if (v) atomicAdd(&hist_acc[i], v);
+    }
+    __threadfence();
+    if (lane == 0) s_last = atomicAdd(ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
+    __syncthreads();
+    if (!s_last) return;
+
+    // Last workgroup: collect the histogram (and reset it for the next run).
+    __threadfence();
+    int32_t* rhist = reinterpret_cast<int32_t*>(result + 1);
+    for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
+        const int32_t v = atomicExch(&hist_acc[i], 0);
+        s_hist[i] = v;
+        rhist[i] = v;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        // reduced chi2 of src/streamWorker.cpp:194-206, in the reference's
+        // float accumulation and in FP64
+        int effN = a.B - a.left_pad;
+        float c32 = 0.0f;
+        double c64 = 0.0;
+        const float Rf = float(a.R);
+        for (int b = a.left_pad; b < a.B; ++b) {
+            const int32_t Mb = s_hist[b];
+            if (Mb > 0) {
+                const float Mf = float(Mb);
+                const float dlt = obs[b] - Mf / Rf;
+                c32 = float(double(c32) + (double(dlt) * double(dlt)) / double(Mf / Rf));
+                const double md = double(Mb) / double(a.R);
+                const double dd = double(obs[b]) - md;
+                c64 += dd * dd / md;
+            } else {
+                --effN;
+            }
+        }
+        const int dof = effN - a.nstates * (a.nstates + 1);
+        result->chi2_ref = c32 / float(dof);
+        result->chi2_64 = c64 / double(dof);
+        result->effN = effN;
+        result->units = int64_t(a.nunits);
+        result->B = a.B;
+        result->pad = 0;
+        atomicExch(ticket, 0u);
+    }
+}
+
+// Evaluates shared-library primitives for the bit-exactness tests.
+struct BinomItem {
+    int32_t n;
+    uint32_t pad0;
+    double p;
+    uint32_t seed, set, r, b, w, hi, lo, pad1;
+};
+struct BgItem {
+    float lam, cap, u1, u2;
+    uint32_t seed, set, r, b, w, sub;
+};
+
+__global__ void primitive_kernel(int op, const void* __restrict__ in, void* __restrict__ out,
+                                 int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    switch (op) {
+        case 0: {
+            const uint32_t* q = static_cast<const uint32_t*>(in) + 6 * i;
+            const Block x = philox10(q[0], q[1], q[2], q[3], q[4], q[5]);
+            uint32_t* o = static_cast<uint32_t*>(out) + 4 * i;
+            o[0] = x.w[0]; o[1] = x.w[1]; o[2] = x.w[2]; o[3] = x.w[3];
+            break;
+        }
+        case 1: static_cast<float*>(out)[i] = flog(static_cast<const float*>(in)[i]); break;
+        case 2: static_cast<float*>(out)[i] = fexp(static_cast<const float*>(in)[i]); break;
+        case 3: static_cast<double*>(out)[i] = dlog(static_cast<const double*>(in)[i]); break;
+        case 4: static_cast<double*>(out)[i] = dexp(static_cast<const double*>(in)[i]); break;
+        case 5: static_cast<float*>(out)[i] = cos_turns(static_cast<const float*>(in)[i]); break;
+        case 6: {
+            const uint32_t* q = static_cast<const uint32_t*>(in) + 2 * i;
+            static_cast<float*>(out)[i] = gauss(q[0], q[1]);
+            break;
+        }
+        case 7: {
+            const BinomItem it = static_cast<const BinomItem*>(in)[i];
+            const UnitKey key{it.seed, it.set, it.r, it.b};
+            static_cast<int64_t*>(out)[i] = binomial(it.n, it.p, it.hi, it.lo, key, it.w);
+            break;
+        }
+        case 8: {
+            const BgItem it = static_cast<const BgItem*>(in)[i];
+            const UnitKey key{it.seed, it.set, it.r, it.b};
+            static_cast<float*>(out)[i] =
+                background_count(it.lam, it.cap, it.u1, it.u2, key, it.w, it.sub);
+            break;
+        }
+        default: break;
+    }
+}
+
+// ---- launchers (called by the C-ABI layer) ---------------------------------
+
+size_t walk_smem_bytes(int B, uint32_t cap) {
+    return (size_t((B + 1) & ~1) + size_t((cap + 1u) & ~1u) + 2u * size_t(cap)) * 4u;
+}
+
+hipError_t walk_occupancy(int B, uint32_t cap, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel, kWave,
+                                                        walk_smem_bytes(B, cap));
+}
+
+hipError_t launch_walk(uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
+                       const uint2* links, const Model* model, const RunArgs& a, const float* obs,
+                       int32_t* hist_acc, uint32_t* ticket, ResultHeader* result, float* outE,
+                       int32_t* outAcc, float* outF, hipStream_t stream) {
+    hipLaunchKernelGGL(walk_kernel, dim3(grid), dim3(kWave), walk_smem_bytes(a.B, a.cap), stream,
+                       bursts, ticks, links, model, a, obs, hist_acc, ticket, result, outE,
+                       outAcc, outF);
+    return hipGetLastError();
+}
+
+hipError_t launch_primitive(int op, const void* in, void* out, int64_t n, hipStream_t stream) {
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(primitive_kernel, dim3(uint32_t(blocks)), dim3(256), 0, stream, op, in, out,
+                       n);
+    return hipGetLastError();
+}
+
+}  // namespace pda

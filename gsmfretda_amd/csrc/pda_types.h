@@ -1,0 +1,75 @@
+// pda_types.h -- device data layout of the Monte-Carlo PDA path.
+#pragma once
+#include <cstdint>
+
+namespace pda {
+
+constexpr int kMaxStates = 8;
+constexpr int kMaxBins = 1024;
+constexpr int kWave = 64;            // lanes per workgroup of the walk kernel
+constexpr uint32_t kTickSentinel = 0x80000000u;  // "never closes a window"
+constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
+
+// One burst, original order (the range [begin, end) of mc::setBurstBd
+// addresses these directly).  16 bytes, one dwordx4 load.
+struct BurstDesc {
+    uint32_t off;       // first photon in the photon arrays
+    uint32_t P;         // photon count (istop - istart + 1)
+    uint32_t stop_rel;  // stop - start, ticks
+    float T;            // T_burst_duration, seconds
+};
+
+// Per photon i of a burst (local indices):
+//   tick[i]   burst-relative tick; photons in neither channel carry the tick
+//             of the next channel photon (kTickSentinel if none), so the
+//             ticks are sorted and "first channel photon at or after w1"
+//             is a lower_bound.
+//   link[i].x first channel photon f >= i (P if none) | colour(f) << 30
+//             (colour: 1 donor, 2 acceptor, 3 both)
+//   link[i].y first photon after f carrying the other colour (P if none;
+//             f itself when f carries both)
+// The sticky-flag count of a window [c, close) is then
+// min(close, link[c].y) - f for f = link[c].x < close (src/mc.cu:136-181).
+
+// Model tables of one parameter set, built on the host from the args of
+// mc::set_params (src/mc.cu:790-800, src/eigenhelper.cpp:4-54).
+struct Model {
+    int32_t n;
+    int32_t any_v;                // some v[s] != 0: broadening draws needed
+    float gmx;                    // gamma - Dch2Ach
+    float omdd;                   // 1 - DexDirAem
+    float r0;
+    float pad0[3];
+    float p0cum[kMaxStates];                      // running sums of genMatP's inflow
+    float nscum[kMaxStates][kMaxStates];          // [s][jj] running sums of P_i2j column s
+    float krate[kMaxStates][kMaxStates];          // [s][j] = K(j, s), rate s -> j
+    float v[kMaxStates];
+    float base[kMaxStates];                       // r0 (1/E - 1)^(1/6)
+    float ep0[kMaxStates];                        // broadened E when v == 0
+};
+
+// Arguments of one launch of the walk kernel (passed by value).
+struct RunArgs {
+    uint32_t begin;      // first burst
+    uint32_t nunits;     // bursts * R
+    uint32_t R;
+    uint32_t seed, set;
+    uint32_t ntiles;     // ceil(nunits / kWave)
+    uint32_t tiles_per_xcd;
+    int32_t B, left_pad, bg_on;
+    float clk_p, bg_ad, bg_dd;
+    uint32_t cap;        // LDS photon capacity of a tile
+    int32_t nstates;     // for the chi2 degrees of freedom
+};
+
+// Device result of one run, followed by int32 hist[B].
+struct ResultHeader {
+    double chi2_64;
+    float chi2_ref;
+    int32_t effN;
+    int64_t units;
+    int32_t B;
+    int32_t pad;
+};
+
+}  // namespace pda

@@ -1,0 +1,313 @@
+"""Python host interface of the MI355X Monte-Carlo PDA path (ctypes over the C ABI).
+
+`MC` mirrors gSMFRETda's `class mc` (src/mc.hpp:32-180) method for method, so
+tests and callers read like the reference's gpuWorker loop
+(src/gpuWorker.cpp:69-83):
+
+    m = MC(0, 1)
+    m.init_data_gpu(...)             # bursts resident in HBM, once
+    m.set_params(n, sid, args)       # E, K off-diagonals, v
+    m.setBurstBd(begin, end, sid)
+    m.run_kernel(N, sid)             # async on the GPU
+    while not m.streamQuery(sid): ...
+    m.get_hist_chi2(sid)             # int histogram + chi2, computed on device
+
+The shared library must exist (gsmfretda_amd/libgsmfretda_pda.so, built by
+gsmfretda_amd.build); there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgsmfretda_pda.so")
+
+PDA_RUN_SAMPLES = 1
+MAX_STATES = 8
+MAX_BINS = 1024
+
+# Symbols declared in include/gsmfretda_pda.h.
+EXPORTS = (
+    "pda_last_error", "pda_version", "pda_create", "pda_destroy", "pda_num_streams",
+    "pda_stream", "pda_load_bursts", "pda_load_bursts_tagged", "pda_num_bursts",
+    "pda_set_observed", "pda_set_observed_hist", "pda_set_params", "pda_set_photophysics",
+    "pda_set_burst_range", "pda_run", "pda_query", "pda_synchronize", "pda_get_result",
+    "pda_get_samples", "pda_samples_host", "pda_last_run_ms", "pda_debug_primitive",
+)
+
+
+class PdaError(RuntimeError):
+    pass
+
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads the HIP library; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PdaError(f"{path} is missing: build it with gsmfretda_amd.build.build()")
+    L = ctypes.CDLL(path)
+    L.pda_last_error.restype = ctypes.c_char_p
+    L.pda_version.restype = ctypes.c_char_p
+    L.pda_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+    L.pda_destroy.argtypes = [_vp]
+    L.pda_num_streams.argtypes = [_vp]
+    L.pda_stream.argtypes = [_vp, ctypes.c_int]
+    L.pda_stream.restype = _vp
+    L.pda_load_bursts.argtypes = [_vp, _i64p, _i64p, _i64p, _i32p, ctypes.c_int64, _i64p, _u8p,
+                                  _u8p, ctypes.c_int64, _f32p, ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_float]
+    L.pda_load_bursts_tagged.argtypes = [_vp, _i64p, _i64p, _i64p, _i32p, ctypes.c_int64, _i64p,
+                                         _i64p, ctypes.c_int64, _f32p, ctypes.c_float,
+                                         ctypes.c_float, ctypes.c_float]
+    L.pda_num_bursts.argtypes = [_vp]
+    L.pda_num_bursts.restype = ctypes.c_int64
+    L.pda_set_observed.argtypes = [_vp, _f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+    L.pda_set_observed_hist.argtypes = [_vp, _f32p, ctypes.c_int, ctypes.c_int]
+    L.pda_set_params.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int]
+    L.pda_set_photophysics.argtypes = [_vp, ctypes.c_int] + [ctypes.c_float] * 4
+    L.pda_set_burst_range.argtypes = [_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _i64p]
+    L.pda_run.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                          ctypes.c_int]
+    L.pda_query.argtypes = [_vp, ctypes.c_int]
+    L.pda_synchronize.argtypes = [_vp, ctypes.c_int]
+    L.pda_get_result.argtypes = [_vp, ctypes.c_int, _i32p, ctypes.c_int, _f64p, _f32p,
+                                 ctypes.POINTER(ctypes.c_int)]
+    L.pda_get_samples.argtypes = [_vp, ctypes.c_int, _f32p, _i32p, _f32p, ctypes.c_int64]
+    L.pda_samples_host.argtypes = [_vp, ctypes.c_int]
+    L.pda_samples_host.restype = _f32p
+    L.pda_last_run_ms.argtypes = [_vp, ctypes.c_int, _f32p]
+    L.pda_debug_primitive.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int64]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc < 0:
+        raise PdaError(f"pda error {rc}: {_lib.pda_last_error().decode()}")
+    return rc
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class MC:
+    """The reference's `class mc` (src/mc.hpp:32-180) over libgsmfretda_pda.so."""
+
+    def __init__(self, devid=0, streamNum=1):
+        L = load_library()
+        h = _vp()
+        _check(L.pda_create(devid, streamNum, ctypes.byref(h)))
+        self._h = h
+        self._L = L
+        self.devid = devid
+        self.streamNum = L.pda_num_streams(h)
+        self.reSampleTimes = 4          # src/mc.cu:308
+        self.s_n = [0] * self.streamNum
+        self.sz_burst = 0
+        self.seed = 1234
+        self._N = [0] * self.streamNum
+        self._hist_bins = 200           # fret_hist_num default (cmdline.ggo)
+        self._left_pad = 2               # src/main.cpp:155
+
+    # -- lifetime ------------------------------------------------------------
+    def close(self):
+        if self._h:
+            self._L.pda_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def getStreams(self):
+        return [self._L.pda_stream(self._h, s) for s in range(self.streamNum)]
+
+    def set_gpuid(self):
+        pass
+
+    # -- data ------------------------------------------------------------------
+    def init_data_gpu(self, istart, start, stop, phCount, times, mask_ad, mask_dd, T, SgDivSr,
+                      clk_p, bg_ad, bg_dd, nbins=None, left_pad=None):
+        """mc::init_data_gpu (src/mc.cu:456-521) from the HDF5 datasets
+        (times + masks instead of the burst_ad/burst_dd encoding)."""
+        a = dict(istart=np.ascontiguousarray(istart, np.int64),
+                 start=np.ascontiguousarray(start, np.int64),
+                 stop=np.ascontiguousarray(stop, np.int64),
+                 ph=np.ascontiguousarray(phCount, np.int32),
+                 times=np.ascontiguousarray(times, np.int64),
+                 mad=np.ascontiguousarray(mask_ad, np.uint8),
+                 mdd=np.ascontiguousarray(mask_dd, np.uint8),
+                 T=np.ascontiguousarray(T, np.float32))
+        _check(self._L.pda_load_bursts(
+            self._h, _p(a["start"], _i64p), _p(a["stop"], _i64p), _p(a["istart"], _i64p),
+            _p(a["ph"], _i32p), len(a["start"]), _p(a["times"], _i64p), _p(a["mad"], _u8p),
+            _p(a["mdd"], _u8p), len(a["times"]), _p(a["T"], _f32p), clk_p, bg_ad, bg_dd))
+        self.sz_burst = len(a["start"])
+        if SgDivSr is not None:
+            self.set_observed(SgDivSr, nbins or self._hist_bins,
+                              self._left_pad if left_pad is None else left_pad)
+
+    def init_data_gpu_tagged(self, istart, start, stop, phCount, burst_ad, burst_dd, T,
+                             SgDivSr, clk_p, bg_ad, bg_dd):
+        """mc::init_data_gpu with the reference's exact photon encoding."""
+        a = [np.ascontiguousarray(x, t) for x, t in (
+            (istart, np.int64), (start, np.int64), (stop, np.int64), (phCount, np.int32),
+            (burst_ad, np.int64), (burst_dd, np.int64), (T, np.float32))]
+        _check(self._L.pda_load_bursts_tagged(
+            self._h, _p(a[0], _i64p), _p(a[1], _i64p), _p(a[2], _i64p), _p(a[3], _i32p),
+            len(a[1]), _p(a[4], _i64p), _p(a[5], _i64p), len(a[4]), _p(a[6], _f32p),
+            clk_p, bg_ad, bg_dd))
+        self.sz_burst = len(a[1])
+        if SgDivSr is not None:
+            self.set_observed(SgDivSr, self._hist_bins, self._left_pad)
+
+    def load(self, bursts, SgDivSr=None, nbins=50, left_pad=2):
+        """Convenience: load a gsmfretda_amd.synth.BurstData."""
+        self.init_data_gpu(bursts.istart, bursts.start, bursts.stop, bursts.phcount,
+                           bursts.times, bursts.mask_ad, bursts.mask_dd, bursts.T, SgDivSr,
+                           bursts.clk_p, bursts.bg_ad, bursts.bg_dd, nbins, left_pad)
+
+    def set_observed(self, SgDivSr, nbins=50, left_pad=2):
+        x = np.ascontiguousarray(SgDivSr, np.float32)
+        _check(self._L.pda_set_observed(self._h, _p(x, _f32p), len(x), nbins, left_pad))
+        self._hist_bins = nbins
+        self._left_pad = left_pad
+
+    def set_observed_hist(self, ohist, left_pad=2):
+        h = np.ascontiguousarray(ohist, np.float32)
+        _check(self._L.pda_set_observed_hist(self._h, _p(h, _f32p), len(h), left_pad))
+        self._hist_bins = len(h)
+        self._left_pad = left_pad
+
+    def get_max_conversion_capacity(self, max_stateNum=8):
+        return self.sz_burst > 0
+
+    # -- per parameter set -------------------------------------------------------
+    def set_nstates(self, n, sid):
+        prev = self.s_n[sid]
+        self.s_n[sid] = n
+        return prev if prev != n else n
+
+    def set_params(self, n, sid, args):
+        """mc::set_params: args = [E(n), K off-diagonals (n(n-1)), v(n)].
+        Returns False on a bad argument count, like the reference."""
+        a = np.ascontiguousarray(args, np.float32)
+        rc = self._L.pda_set_params(self._h, sid, n, _p(a, _f32p), len(a))
+        if rc == -1:
+            return False
+        _check(rc)
+        self.s_n[sid] = n
+        return True
+
+    def set_photophysics(self, sid, gamma=0.34, dexdir=0.08, crosstalk=0.07, r0=52.0):
+        _check(self._L.pda_set_photophysics(self._h, sid, gamma, dexdir, crosstalk, r0))
+
+    def setBurstBd(self, cstart, cstop, sid):
+        """mc::setBurstBd (src/mc.cu:523-535), including its range rules."""
+        rcstart, rcstop = cstart, cstop
+        if cstop >= self.sz_burst or cstop < 0:
+            rcstop = self.sz_burst
+        if cstart >= rcstop:
+            rcstart = 0
+        N = rcstop - rcstart
+        if cstart == -1:
+            return -1
+        _check(self._L.pda_set_burst_range(self._h, sid, rcstart, rcstop, None))
+        self._N[sid] = N
+        return N
+
+    def set_burst_range(self, begin, end, sid):
+        n = ctypes.c_int64()
+        _check(self._L.pda_set_burst_range(self._h, sid, begin, end, ctypes.byref(n)))
+        self._N[sid] = n.value
+        return n.value
+
+    def init_randstate(self, N, sid):
+        pass  # stateless Philox streams: nothing to initialise
+
+    def set_params_buff(self, oldS_n, N, sid):
+        pass
+
+    def run_kernel(self, N=None, sid=0, seed=None, set_id=None, samples=False):
+        """mc::run_kernel: async launch of reSampleTimes resamples per burst."""
+        flags = PDA_RUN_SAMPLES if samples else 0
+        s = self.seed if seed is None else seed
+        k = sid if set_id is None else set_id
+        _check(self._L.pda_run(self._h, sid, self.reSampleTimes, s & 0xFFFFFFFF,
+                               k & 0xFFFFFFFF, flags))
+
+    def streamQuery(self, sid):
+        return _check(self._L.pda_query(self._h, sid)) == 1
+
+    def synchronize(self, sid):
+        _check(self._L.pda_synchronize(self._h, sid))
+
+    # -- results ---------------------------------------------------------------
+    def get_hist_chi2(self, sid, nbins=None):
+        nb = nbins or self._hist_bins
+        h = np.zeros(nb, np.int32)
+        c64 = ctypes.c_double()
+        c32 = ctypes.c_float()
+        eff = ctypes.c_int()
+        _check(self._L.pda_get_result(self._h, sid, _p(h, _i32p), nb, ctypes.byref(c64),
+                                      ctypes.byref(c32), ctypes.byref(eff)))
+        return h, c64.value, c32.value, eff.value
+
+    def get_samples(self, sid, n):
+        E = np.zeros(n, np.float32)
+        acc = np.zeros(n, np.int32)
+        F = np.zeros(n, np.float32)
+        _check(self._L.pda_get_samples(self._h, sid, _p(E, _f32p), _p(acc, _i32p),
+                                       _p(F, _f32p), n))
+        return E, acc, F
+
+    def last_run_ms(self, sid):
+        ms = ctypes.c_float()
+        _check(self._L.pda_last_run_ms(self._h, sid, ctypes.byref(ms)))
+        return ms.value
+
+    def debug_primitive(self, op, inp, out_dtype, n):
+        inp = np.ascontiguousarray(inp)
+        out = np.zeros(n * {0: 4}.get(op, 1), out_dtype)
+        _check(self._L.pda_debug_primitive(self._h, op, inp.ctypes.data, out.ctypes.data, n))
+        return out
+
+
+def evaluate(m, n, args, R, seed, set_id=0, sid=0, begin=0, end=-1, samples=False,
+             photophysics=None):
+    """One parameter-set evaluation, blocking: returns (hist, chi2_64, chi2_ref, effN)."""
+    if photophysics is not None:
+        m.set_photophysics(sid, *photophysics)
+    if not m.set_params(n, sid, args):
+        raise PdaError("bad parameter set")
+    m.set_burst_range(begin, end, sid)
+    m.reSampleTimes = R
+    m.run_kernel(sid=sid, seed=seed, set_id=set_id, samples=samples)
+    m.synchronize(sid)
+    return m.get_hist_chi2(sid)

@@ -1,0 +1,150 @@
+/*
+ * gsmfretda_pda.h -- C ABI of the MI355X (gfx950) Monte-Carlo PDA hot path.
+ *
+ * Drop-in boundary for gSMFRETda's `class mc` device API (src/mc.hpp:32-180,
+ * host half of src/mc.cu:204-829).  Plain pointers and sizes only.  Every
+ * entry point returns 0 on success and a negative PDA_E* code on failure;
+ * pda_last_error() then describes the failure (thread-local).  Host arrays
+ * are copied before the call returns unless stated otherwise.
+ *
+ * Threading (as the reference, src/mc.cu:199-203, src/gpuWorker.cpp:39):
+ * calls that mutate a stream slot must come from one thread at a time.
+ */
+#ifndef GSMFRETDA_PDA_H
+#define GSMFRETDA_PDA_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDA_OK 0
+#define PDA_E_ARG -1     /* bad argument (size, range, null pointer)        */
+#define PDA_E_STATE -2   /* call out of order (no data, no params, ...)     */
+#define PDA_E_DATA -3    /* burst data violates a precondition              */
+#define PDA_E_HIP -4     /* HIP runtime error                               */
+#define PDA_E_NOMEM -5   /* allocation failed                               */
+
+#define PDA_MAX_STATES 8
+#define PDA_MAX_BINS 1024
+
+/* pda_run flags */
+#define PDA_RUN_SAMPLES 1 /* also keep per-sample E (and acceptor count, F)  */
+
+typedef struct pda_ctx pda_ctx;
+
+/* Description of the last failure on the calling thread ("" if none). */
+const char *pda_last_error(void);
+
+/* Library version string. */
+const char *pda_version(void);
+
+/* Replaces mc::mc(devid, streamNum, ...) (src/mc.cu:212-315): selects the GPU
+ * and creates `nstreams` stream slots (0 = 4).  Each slot evaluates its own
+ * parameter set, as the reference's per-stream buffers (src/mc.cu:235-242). */
+int pda_create(int device, int nstreams, pda_ctx **out);
+
+/* Replaces mc::~mc (src/mc.cu:618-663): waits for every slot, frees all. */
+int pda_destroy(pda_ctx *ctx);
+
+/* Number of stream slots and their hipStream_t handles (as void*), replaces
+ * mc::getStreams (src/mc.cu:317-322). */
+int pda_num_streams(const pda_ctx *ctx);
+void *pda_stream(pda_ctx *ctx, int sid);
+
+/* Replaces mc::init_data_gpu (src/mc.cu:456-521) fed by burst_data
+ * (src/loadHdf5.cpp:101-128): the HDF5 datasets of Appendix-B layout
+ * (/sub_bursts_l/start, stop, istart; photon count istop-istart+1; times,
+ * mask_ad, mask_dd; T_burst_duration; clk_p; bg_ad_rate; bg_dd_rate).
+ * The bursts are re-encoded once into burst-relative uint32 ticks plus
+ * per-photon channel-run links and kept resident.  Preconditions (else
+ * PDA_E_DATA): 0 <= istart, istart+phcount <= nphotons; every photon of a
+ * channel lies in [start, start + 2^31) ticks; channel photons of a burst
+ * are in non-decreasing time order; 0 <= stop-start < 2^31; T finite;
+ * clk_p > 0; nbursts*R fits the run's 2^31 unit limit.  Replaces any
+ * previously loaded data (waits for all slots first). */
+int pda_load_bursts(pda_ctx *ctx, const int64_t *start, const int64_t *stop,
+                    const int64_t *istart, const int32_t *phcount, int64_t nbursts,
+                    const int64_t *times, const uint8_t *mask_ad, const uint8_t *mask_dd,
+                    int64_t nphotons, const float *T, float clk_p, float bg_ad, float bg_dd);
+
+/* The same, from the reference's "time if in channel, else 0" photon arrays
+ * burst_ad / burst_dd (exactly what mc::init_data_gpu receives,
+ * src/mc.hpp:158-161).  A photon with both entries 0 belongs to neither
+ * channel. */
+int pda_load_bursts_tagged(pda_ctx *ctx, const int64_t *istart, const int64_t *start,
+                           const int64_t *stop, const int32_t *phcount, int64_t nbursts,
+                           const int64_t *burst_ad, const int64_t *burst_dd, int64_t nphotons,
+                           const float *T, float clk_p, float bg_ad, float bg_dd);
+
+int64_t pda_num_bursts(const pda_ctx *ctx);
+
+/* Observed histogram from the per-burst measured efficiencies SgDivSr
+ * (src/streamWorker.cpp:62-67): nbins regular bins on [0,1), values outside
+ * dropped; chi2 skips the first left_pad bins (src/main.cpp:155). */
+int pda_set_observed(pda_ctx *ctx, const float *sgdivsr, int64_t n, int nbins, int left_pad);
+/* Or an observed histogram directly (nbins counts). */
+int pda_set_observed_hist(pda_ctx *ctx, const float *ohist, int nbins, int left_pad);
+
+/* Replaces mc::set_params (src/mc.cu:790-829) and the matrix builders
+ * genMatK / genP_i2j / genMatP (src/eigenhelper.cpp:4-54):
+ * args = [E(n), K off-diagonal (n*(n-1), row-major over the target state,
+ * skipping the diagonal), v(n)].  2 <= n <= PDA_MAX_STATES; nargs must be
+ * n*(n+1) (the reference returns false; here PDA_E_ARG).  Unlike the
+ * reference it does not synchronize the stream. */
+int pda_set_params(pda_ctx *ctx, int sid, int n, const float *args, int nargs);
+
+/* Photophysics constants, mc_kernel's default arguments (src/mc.cu:89-91):
+ * gamma 0.34, DexDirAem 0.08, Dch2Ach 0.07, r0 52 until set. */
+int pda_set_photophysics(pda_ctx *ctx, int sid, float gamma, float dexdir, float crosstalk,
+                         float r0);
+
+/* Replaces mc::setBurstBd (src/mc.cu:523-565): bursts [begin, end) for the
+ * slot; end < 0 or end > nbursts means nbursts; begin >= end means 0.
+ * Writes the number of bursts to *n_out (may be NULL). */
+int pda_set_burst_range(pda_ctx *ctx, int sid, int64_t begin, int64_t end, int64_t *n_out);
+
+/* Replaces mc::run_kernel (src/mc.cu:567-587), asynchronous on the slot's
+ * stream: R resamples of every burst in the range, the integer E histogram
+ * and its chi2 against the observed histogram, all on the GPU.  Random
+ * numbers are keyed by (seed, set_id, burst, resample). */
+int pda_run(pda_ctx *ctx, int sid, int R, uint32_t seed, uint32_t set_id, int flags);
+
+/* Replaces mc::streamQuery (src/mc.cu:595-601): 1 when the slot's last run
+ * (and its result copy) finished, 0 while it runs, <0 on error. */
+int pda_query(pda_ctx *ctx, int sid);
+int pda_synchronize(pda_ctx *ctx, int sid);
+
+/* Result of the slot's last completed run: int32 histogram of nbins bins,
+ * reduced chi2 in FP64 and with the reference's float accumulation
+ * (src/streamWorker.cpp:197-206), number of bins kept.  Any output may be
+ * NULL.  Call after pda_query returned 1 or pda_synchronize. */
+int pda_get_result(pda_ctx *ctx, int sid, int32_t *hist, int nbins, double *chi2_64,
+                   float *chi2_ref, int *effN);
+
+/* Per-sample values of the last run made with PDA_RUN_SAMPLES, in unit
+ * order (burst - begin) * R + r: E (the reference's mcE / hmcE), acceptor
+ * count and corrected photon count F.  Any output may be NULL. */
+int pda_get_samples(pda_ctx *ctx, int sid, float *E, int32_t *acc, float *F, int64_t n);
+
+/* Host pointer of the slot's pinned per-sample E buffer (the reference's
+ * hmcE[sid], src/mc.cu:559-562); valid after a PDA_RUN_SAMPLES run
+ * completed, NULL before. */
+float *pda_samples_host(pda_ctx *ctx, int sid);
+
+/* Milliseconds spent by the slot's last run on the GPU (HIP events). */
+int pda_last_run_ms(pda_ctx *ctx, int sid, float *ms);
+
+/* Diagnostics (no reference counterpart): evaluates one primitive of the
+ * shared variate library on the GPU for n inputs, so tests can compare it
+ * bit for bit with the host.  op: 0 philox (in: 6 u32 per item -> 4 u32),
+ * 1 flog, 2 fexp, 3 dlog, 4 dexp, 5 cos_turns, 6 gauss (2 u32 -> f32),
+ * 7 binomial (in: n i32, p f64, key 4 u32, w u32, words 2 u32 -> i64),
+ * 8 background_count (in: lam, cap, u1, u2 f32, key 4 u32, w, sub -> f32).
+ * in/out are host arrays; the layout per op is documented in pda_capi.cpp. */
+int pda_debug_primitive(pda_ctx *ctx, int op, const void *in, void *out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,101 @@
+/*
+ * pda_oracle.h -- CPU oracle for the Monte-Carlo PDA hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this code, and only as the checker
+ * or the timed CPU baseline.  The product (gsmfretda_amd) never links it.
+ *
+ * The oracle restates gSMFRETda's mc_kernel (src/mc.cu:83-197) and its
+ * helpers (src/gen_rand.cuh:35-107, src/binom.cuh:135-188,
+ * src/eigenhelper.cpp:4-54, src/streamWorker.cpp:33-38,187-206) literally:
+ * one resample at a time, a linear photon scan with the sticky channel
+ * flags, every window of the dwell walk evaluated (no early exit).  The
+ * random numbers come from the keyed Philox4x32-10 stream layout that the
+ * HIP kernels also use, so results can be compared bit for bit.
+ */
+#ifndef PDA_ORACLE_H
+#define PDA_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAXN 8
+
+/* Burst data in the reference's HDF5 layout (src/loadHdf5.cpp:37-74,101-128). */
+typedef struct {
+    int64_t nbursts;
+    const int64_t *start, *stop, *istart; /* [nbursts] absolute ticks / photon offsets */
+    const int32_t *phcount;               /* [nbursts] = istop - istart + 1            */
+    const int64_t *times;                 /* [nphotons] absolute macrotime ticks       */
+    const uint8_t *mask_ad, *mask_dd;     /* [nphotons] acceptor / donor channel masks */
+    const float *T;                       /* [nbursts] burst duration in seconds       */
+    float clk_p, bg_ad, bg_dd;            /* s/tick, background rates (Hz)             */
+} or_data;
+
+/* Kinetic model + photophysics (src/mc.cu:790-800, src/eigenhelper.cpp:4-54,
+ * kernel defaults src/mc.cu:89-91). K and Pi2j are column-major n x n. */
+typedef struct {
+    int n;
+    float E[OR_MAXN], v[OR_MAXN], p0[OR_MAXN];
+    float K[OR_MAXN * OR_MAXN], Pi2j[OR_MAXN * OR_MAXN];
+    float gamma, dexdir, crosstalk, r0;
+} or_params;
+
+typedef struct {
+    int64_t windows, photons_visited, philox_blocks;
+    int64_t binom_inv_iters, binom_btpe_iters, poisson_iters;
+} or_stats;
+
+/* -- primitives (exported for the oracle's own pinning tests) -- */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float or_u01f(uint32_t x);
+double or_u53(uint32_t a, uint32_t b);
+float or_logf(float x);
+float or_expf(float x);
+double or_log(double x);
+double or_exp(double x);
+float or_cos2pif(float u);
+float or_normal(uint32_t a, uint32_t b);
+int or_categorical(int n, const float *p, float u);
+/* Binomial(n, p) fed from the keyed stream of unit (seed,set,burst,r) window w. */
+int64_t or_binomial(int n, double p, uint32_t seed, uint32_t set, uint32_t burst,
+                    uint32_t r, uint32_t w, uint32_t a2, uint32_t a3, or_stats *st);
+float or_bg_capped(float lam, float cap, float u_first, float u_cap, uint32_t seed,
+                   uint32_t set, uint32_t burst, uint32_t r, uint32_t w, uint32_t sub_base,
+                   or_stats *st);
+
+/* genMatK / genP_i2j / genMatP (src/eigenhelper.cpp:4-54) + arg split
+ * (src/mc.cu:790-800).  Returns 0 on success, -1 when nargs != n*(n+1) or n
+ * is outside [2, OR_MAXN]. */
+int or_set_params(or_params *p, int n, const float *args, int nargs, float gamma,
+                  float dexdir, float crosstalk, float r0);
+
+/* One (burst, resample) unit: mc_kernel's body for thread t = burst*R + r. */
+void or_unit(const or_data *d, const or_params *p, int64_t burst, uint32_t r, uint32_t seed,
+             uint32_t set, int32_t *acc, float *F, float *E, or_stats *st);
+
+/* Bursts [b0, b1) x R resamples; outputs indexed (b - b0) * R + r (any may be NULL).
+ * hist (int64[B], may be NULL) accumulates floor((double)E*B) for 0 <= E < 1. */
+void or_run(const or_data *d, const or_params *p, int64_t b0, int64_t b1, int R, uint32_t seed,
+            uint32_t set, int32_t *acc, float *F, float *E, int B, int64_t *hist,
+            or_stats *st);
+
+/* Same as or_run over [b0,b1), split over nthreads std threads (CPU baseline). */
+void or_run_threaded(const or_data *d, const or_params *p, int64_t b0, int64_t b1, int R,
+                     uint32_t seed, uint32_t set, int B, int64_t *hist, int nthreads);
+
+/* Regular [0,1) histogram of n floats (boost axis::regular(B,0,1), coverage::inner). */
+void or_hist(const float *x, int64_t n, int B, int64_t *hist);
+
+/* Reduced chi2 of src/streamWorker.cpp:197-206.  chi2_ref follows the
+ * reference's float accumulation, chi2_64 is the same sum in double.
+ * effN returns the number of bins kept. */
+void or_chi2(const float *O, const int64_t *M, int B, int left_pad, int R, int nstates,
+             float *chi2_ref, double *chi2_64, int *effN);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
