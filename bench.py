@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-bursts", type=int, default=100_000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is "
@@ -56,6 +58,15 @@ def main():
         sys.exit(2)
 
     from gsmfretda_amd import pda, synth
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        with pda.MC(0, 1) as m:
+            x = np.arange(n, dtype=np.uint32)
+            y = m.debug_primitive(9, x, np.uint32, n)
+            assert np.array_equal(x, y)
+        print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
+        return
 
     cfg = synth.config(1)
     d = cfg["data"]

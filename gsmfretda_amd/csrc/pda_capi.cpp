@@ -21,11 +21,12 @@
 
 namespace pda {
 size_t walk_smem_bytes(int B, uint32_t cap);
-hipError_t walk_occupancy(int B, uint32_t cap, int* blocks_per_cu);
-hipError_t launch_walk(uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
-                       const uint2* links, const Model* model, const RunArgs& a, const float* obs,
-                       int32_t* hist_acc, uint32_t* ticket, ResultHeader* result, float* outE,
-                       int32_t* outAcc, float* outF, hipStream_t stream);
+hipError_t walk_occupancy(int pass, int B, uint32_t cap, int* blocks_per_cu);
+hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
+                       const uint2* links, const ModelBlob* blob, const RunArgs& a,
+                       const float* obs, int32_t* hist_acc, uint32_t* ticket, uint32_t* replay,
+                       uint32_t* replay_count, ResultHeader* result, float* outE, int32_t* outAcc,
+                       float* outF, hipStream_t stream);
 hipError_t launch_primitive(int op, const void* in, void* out, int64_t n, hipStream_t stream);
 }  // namespace pda
 
@@ -57,8 +58,12 @@ constexpr uint32_t kDefaultCap = 384;  // LDS photon capacity of a 64-unit tile
 
 struct Slot {
     hipStream_t stream = nullptr;
-    Model* h_model = nullptr;  // pinned staging of the model tables
-    Model* d_model = nullptr;
+    ModelBlob* h_blob = nullptr;  // pinned staging of the model tables
+    ModelBlob* d_blob = nullptr;
+    bool any_v = false;           // broadening: the general pass walks every unit
+    uint32_t* d_replay = nullptr;  // units the fast pass abandoned
+    uint32_t* d_replay_count = nullptr;
+    size_t replay_cap = 0;
     hipEvent_t model_copied = nullptr;
     bool model_pending = false;
     int n = 0;
@@ -169,8 +174,10 @@ int upload_model(Slot& s) {
         HIP_TRY(hipEventSynchronize(s.model_copied));
         s.model_pending = false;
     }
-    *s.h_model = M;
-    HIP_TRY(hipMemcpyAsync(s.d_model, s.h_model, sizeof(Model), hipMemcpyHostToDevice, s.stream));
+    s.h_blob->m = M;
+    build_binom_table(M, s.h_blob->tab);
+    s.any_v = M.any_v != 0;
+    HIP_TRY(hipMemcpyAsync(s.d_blob, s.h_blob, sizeof(ModelBlob), hipMemcpyHostToDevice, s.stream));
     HIP_TRY(hipEventRecord(s.model_copied, s.stream));
     s.model_pending = true;
     return PDA_OK;
@@ -211,8 +218,10 @@ int pda_create(int device, int nstreams, pda_ctx** out) {
     ctx->slots.resize(size_t(nstreams));
     for (auto& s : ctx->slots) {
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipHostMalloc(reinterpret_cast<void**>(&s.h_model), sizeof(Model), hipHostMallocDefault) != hipSuccess ||
-            hipMalloc(&s.d_model, sizeof(Model)) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void**>(&s.h_blob), sizeof(ModelBlob), hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&s.d_blob, sizeof(ModelBlob)) != hipSuccess ||
+            hipMalloc(&s.d_replay_count, sizeof(uint32_t)) != hipSuccess ||
+            hipMemset(s.d_replay_count, 0, sizeof(uint32_t)) != hipSuccess ||
             hipEventCreateWithFlags(&s.model_copied, hipEventDisableTiming) != hipSuccess ||
             hipMalloc(&s.d_hist_acc, sizeof(int32_t) * kMaxBins) != hipSuccess ||
             hipMalloc(&s.d_ticket, sizeof(uint32_t)) != hipSuccess ||
@@ -239,8 +248,10 @@ int pda_destroy(pda_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     for (auto& s : ctx->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
-        (void)hipFree(s.d_model);
-        (void)hipHostFree(s.h_model);
+        (void)hipFree(s.d_blob);
+        (void)hipHostFree(s.h_blob);
+        (void)hipFree(s.d_replay);
+        (void)hipFree(s.d_replay_count);
         (void)hipFree(s.d_hist_acc);
         (void)hipFree(s.d_ticket);
         (void)hipFree(s.d_result);
@@ -385,8 +396,9 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
     a.bg_dd = ctx->bg_dd;
     a.cap = ctx->cap;
     a.nstates = s->n;
+    const int pass = s->any_v ? 2 : 0;  // general pass for broadening, else fast + replay
     int per_cu = 0;
-    HIP_TRY(walk_occupancy(a.B, a.cap, &per_cu));
+    HIP_TRY(walk_occupancy(pass, a.B, a.cap, &per_cu));
     if (per_cu < 1) per_cu = 1;
     // persistent grid: a multiple of 8 workgroups (one share per XCD)
     const uint32_t max_per_xcd = uint32_t(std::max(1, ctx->num_cu * per_cu / 8));
@@ -422,10 +434,26 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
         outAcc = s->d_acc;
         outF = s->d_F;
     }
+    if (pass == 0 && size_t(nunits) > s->replay_cap) {
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        (void)hipFree(s->d_replay);
+        s->d_replay = nullptr;
+        s->replay_cap = 0;
+        HIP_TRY(hipMalloc(&s->d_replay, sizeof(uint32_t) * size_t(nunits)));
+        s->replay_cap = size_t(nunits);
+    }
+    a.replay_cap = uint32_t(s->replay_cap);
     HIP_TRY(hipEventRecord(s->ev0, s->stream));
-    HIP_TRY(launch_walk(grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_model, a,
-                        ctx->d_obs, s->d_hist_acc, s->d_ticket, s->d_result, outE, outAcc, outF,
-                        s->stream));
+    HIP_TRY(launch_walk(pass, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, a,
+                        ctx->d_obs, s->d_hist_acc, s->d_ticket, s->d_replay, s->d_replay_count,
+                        s->d_result, outE, outAcc, outF, s->stream));
+    if (pass == 0) {
+        // the replayed units are few: a small persistent grid walks the list
+        const uint32_t rgrid = std::min(grid, 8u * 32u);
+        HIP_TRY(launch_walk(1, rgrid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, a,
+                            ctx->d_obs, s->d_hist_acc, s->d_ticket, s->d_replay,
+                            s->d_replay_count, s->d_result, outE, outAcc, outF, s->stream));
+    }
     HIP_TRY(hipEventRecord(s->ev1, s->stream));
     HIP_TRY(hipMemcpyAsync(s->h_result, s->d_result, sizeof(ResultHeader) + sizeof(int32_t) * size_t(a.B),
                            hipMemcpyDeviceToHost, s->stream));
@@ -506,9 +534,9 @@ int pda_last_run_ms(pda_ctx* ctx, int sid, float* ms) {
 
 int pda_debug_primitive(pda_ctx* ctx, int op, const void* in, void* out, int64_t n) {
     // input / output bytes per item for each op
-    static const size_t in_sz[9] = {24, 4, 4, 8, 8, 4, 8, 48, 40};
-    static const size_t out_sz[9] = {16, 4, 4, 8, 8, 4, 4, 8, 4};
-    if (!ctx || !in || !out || op < 0 || op > 8 || n < 0 || n > (int64_t(1) << 26))
+    static const size_t in_sz[10] = {24, 4, 4, 8, 8, 4, 8, 48, 40, 4};
+    static const size_t out_sz[10] = {16, 4, 4, 8, 8, 4, 4, 8, 4, 4};
+    if (!ctx || !in || !out || op < 0 || op > 9 || n < 0 || n > (int64_t(1) << 28))
         return fail(PDA_E_ARG, "bad argument");
     if (n == 0) return PDA_OK;
     HIP_TRY(hipSetDevice(ctx->device));

@@ -19,16 +19,110 @@
 //     launch is the whole evaluation and needs no memset.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pda_math.h"
 #include "pda_types.h"
 #include "pda_walk.h"
 
 namespace pda {
 
+// Which units a launch walks, and how.
+enum Pass : int {
+    kPassFast = 0,     // all units, fast form; abandoned units go to the replay list
+    kPassReplay = 1,   // the replay list, general form; then chi2
+    kPassGeneral = 2,  // all units, general form (broadening); then chi2
+};
+
+// Histogram flush and, in the last workgroup of the evaluation's final pass,
+// the reduced chi2 of src/streamWorker.cpp:194-206 (reference float
+// accumulation and FP64) plus the reset of every per-run counter.
+template <int kPass>
+__device__ void finish(int32_t* s_hist, const RunArgs& a, const float* obs, int32_t* hist_acc,
+                       uint32_t* ticket, uint32_t* replay_count, ResultHeader* result,
+                       uint32_t* s_last) {
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
+        const int32_t v = s_hist[i];
+        if (v) atomicAdd(&hist_acc[i], v);
+    }
+    if (kPass == kPassFast) return;
+    __threadfence();
+    if (lane == 0) *s_last = atomicAdd(ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
+    __syncthreads();
+    if (!*s_last) return;
+    __threadfence();
+    int32_t* rhist = reinterpret_cast<int32_t*>(result + 1);
+    for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
+        const int32_t v = atomicExch(&hist_acc[i], 0);
+        s_hist[i] = v;
+        rhist[i] = v;
+    }
+    __syncthreads();
+    // per-bin terms in parallel (FP64, and the reference's double term of a
+    // float difference), then lane 0 folds them in bin order
+    double* s_t64 = reinterpret_cast<double*>(reinterpret_cast<uint32_t*>(s_hist) +
+                                              ((uint32_t(a.B) + 1u) & ~1u) +
+                                              ((a.cap + 1u) & ~1u));  // the link area
+    double* s_t32 = s_t64 + a.B;
+    const float Rf = float(a.R);
+    for (int b = int(lane); b < a.B; b += kWave) {
+        const int32_t Mb = s_hist[b];
+        double t64 = 0.0, t32 = 0.0;
+        if (Mb > 0) {
+            const float Mf = float(Mb);
+            const float dlt = obs[b] - Mf / Rf;
+            t32 = (double(dlt) * double(dlt)) / double(Mf / Rf);
+            const double md = double(Mb) / double(a.R);
+            const double dd = double(obs[b]) - md;
+            t64 = dd * dd / md;
+        }
+        s_t64[b] = t64;
+        s_t32[b] = t32;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int effN = a.B - a.left_pad;
+        float c32 = 0.0f;
+        double c64 = 0.0;
+        for (int b = a.left_pad; b < a.B; ++b) {
+            if (s_hist[b] > 0) {
+                c32 = float(double(c32) + s_t32[b]);
+                c64 += s_t64[b];
+            } else {
+                --effN;
+            }
+        }
+        const int dof = effN - a.nstates * (a.nstates + 1);
+        result->chi2_ref = c32 / float(dof);
+        result->chi2_64 = c64 / double(dof);
+        result->effN = effN;
+        result->units = int64_t(a.nunits);
+        result->B = a.B;
+        result->pad = 0;
+        atomicExch(replay_count, 0u);
+        atomicExch(ticket, 0u);
+    }
+}
+
+__device__ __forceinline__ void tally(int32_t* s_hist, const RunArgs& a, const WalkOut& o,
+                                      uint32_t u, float* outE, int32_t* outAcc, float* outF) {
+    const float E = o.F > 0.0f ? o.acc / o.F : 0.0f;
+    if (E >= 0.0f && E < 1.0f) {
+        const int bin = int(floor(double(E) * double(a.B)));
+        if (bin < a.B) atomicAdd(&s_hist[bin], 1);
+    }
+    if (outE) outE[u] = E;
+    if (outAcc) outAcc[u] = int32_t(o.acc);
+    if (outF) outF[u] = o.F;
+}
+
+template <int kPass>
 __global__ __launch_bounds__(kWave) void walk_kernel(
     const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
-    const uint2* __restrict__ links, const Model* __restrict__ model, const RunArgs a,
+    const uint2* __restrict__ links, const ModelBlob* __restrict__ blob, const RunArgs a,
     const float* __restrict__ obs, int32_t* __restrict__ hist_acc, uint32_t* __restrict__ ticket,
+    uint32_t* __restrict__ replay, uint32_t* __restrict__ replay_count,
     ResultHeader* __restrict__ result, float* __restrict__ outE, int32_t* __restrict__ outAcc,
     float* __restrict__ outF) {
     extern __shared__ uint32_t smem[];
@@ -40,12 +134,42 @@ __global__ __launch_bounds__(kWave) void walk_kernel(
     uint32_t* s_tick = smem + hist_words;
     uint2* s_link = reinterpret_cast<uint2*>(s_tick + ((a.cap + 1u) & ~1u));
     {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(model);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&blob->m);
         uint32_t* dst = reinterpret_cast<uint32_t*>(&sM);
         for (uint32_t i = lane; i < sizeof(Model) / 4; i += kWave) dst[i] = src[i];
         for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) s_hist[i] = 0;
     }
     __syncthreads();
+
+    if (kPass == kPassReplay) {
+        // units abandoned by the fast pass, general form: one workgroup per
+        // unit, its burst staged in LDS when it fits, lane 0 walks it
+        const uint32_t count = *replay_count < a.replay_cap ? *replay_count : a.replay_cap;
+        for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+            const uint32_t u = replay[i];
+            const uint32_t bl = u / a.R;
+            const uint32_t b = a.begin + bl;
+            const BurstDesc d = bursts[b];
+            const bool staged = d.P <= a.cap;
+            if (staged) {
+                for (uint32_t k = lane; k < d.P; k += kWave) {
+                    s_tick[k] = ticks[d.off + k];
+                    s_link[k] = links[d.off + k];
+                }
+            }
+            __syncthreads();
+            if (lane == 0) {
+                const UnitKey key{a.seed, a.set, u - bl * a.R, b};
+                const WalkOut o = walk_unit<false>(sM, nullptr, key, d.P, d.stop_rel, d.T,
+                                                   staged ? s_tick : ticks + d.off,
+                                                   staged ? s_link : links + d.off, a);
+                tally(s_hist, a, o, u, outE, outAcc, outF);
+            }
+            __syncthreads();
+        }
+        finish<kPass>(s_hist, a, obs, hist_acc, ticket, replay_count, result, &s_last);
+        return;
+    }
 
     // XCD-aware order: workgroup g runs on XCD g % 8 and walks the contiguous
     // tile chunk of that XCD, so neighbouring tiles (which share bursts)
@@ -74,72 +198,23 @@ __global__ __launch_bounds__(kWave) void walk_kernel(
         const uint32_t u = u0 + lane;
         if (u < a.nunits) {
             const uint32_t bl = u / a.R;
-            const uint32_t r = u - bl * a.R;
             const uint32_t b = a.begin + bl;
             const BurstDesc d = bursts[b];
             const uint32_t* tk = staged ? s_tick + (d.off - p0) : ticks + d.off;
             const uint2* lk = staged ? s_link + (d.off - p0) : links + d.off;
-            const UnitKey key{a.seed, a.set, r, b};
-            const WalkOut o = walk_unit(sM, key, d.P, d.stop_rel, d.T, tk, lk, a);
-            const float E = o.F > 0.0f ? o.acc / o.F : 0.0f;
-            if (E >= 0.0f && E < 1.0f) {
-                const int bin = int(floor(double(E) * double(a.B)));
-                if (bin < a.B) atomicAdd(&s_hist[bin], 1);
+            const UnitKey key{a.seed, a.set, u - bl * a.R, b};
+            const WalkOut o = walk_unit<kPass == kPassFast>(sM, &blob->tab, key, d.P, d.stop_rel,
+                                                            d.T, tk, lk, a);
+            if (o.replay) {
+                const uint32_t slot = atomicAdd(replay_count, 1u);
+                if (slot < a.replay_cap) replay[slot] = u;
+            } else {
+                tally(s_hist, a, o, u, outE, outAcc, outF);
             }
-            if (outE) outE[u] = E;
-            if (outAcc) outAcc[u] = int32_t(o.acc);
-            if (outF) outF[u] = o.F;
         }
         __syncthreads();
     }
-
-    for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
-        const int32_t v = s_hist[i];
-        if (v) atomicAdd(&hist_acc[i], v);
-    }
-    __threadfence();
-    if (lane == 0) s_last = atomicAdd(ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
-    __syncthreads();
-    if (!s_last) return;
-
-    // Last workgroup: collect the histogram (and reset it for the next run).
-    __threadfence();
-    int32_t* rhist = reinterpret_cast<int32_t*>(result + 1);
-    for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
-        const int32_t v = atomicExch(&hist_acc[i], 0);
-        s_hist[i] = v;
-        rhist[i] = v;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        // reduced chi2 of src/streamWorker.cpp:194-206, in the reference's
-        // float accumulation and in FP64
-        int effN = a.B - a.left_pad;
-        float c32 = 0.0f;
-        double c64 = 0.0;
-        const float Rf = float(a.R);
-        for (int b = a.left_pad; b < a.B; ++b) {
-            const int32_t Mb = s_hist[b];
-            if (Mb > 0) {
-                const float Mf = float(Mb);
-                const float dlt = obs[b] - Mf / Rf;
-                c32 = float(double(c32) + (double(dlt) * double(dlt)) / double(Mf / Rf));
-                const double md = double(Mb) / double(a.R);
-                const double dd = double(obs[b]) - md;
-                c64 += dd * dd / md;
-            } else {
-                --effN;
-            }
-        }
-        const int dof = effN - a.nstates * (a.nstates + 1);
-        result->chi2_ref = c32 / float(dof);
-        result->chi2_64 = c64 / double(dof);
-        result->effN = effN;
-        result->units = int64_t(a.nunits);
-        result->B = a.B;
-        result->pad = 0;
-        atomicExch(ticket, 0u);
-    }
+    finish<kPass>(s_hist, a, obs, hist_acc, ticket, replay_count, result, &s_last);
 }
 
 // Evaluates shared-library primitives for the bit-exactness tests.
@@ -189,28 +264,48 @@ __global__ void primitive_kernel(int op, const void* __restrict__ in, void* __re
                 background_count(it.lam, it.cap, it.u1, it.u2, key, it.w, it.sub);
             break;
         }
+        case 9: static_cast<uint32_t*>(out)[i] = static_cast<const uint32_t*>(in)[i]; break;
         default: break;
     }
 }
 
 // ---- launchers (called by the C-ABI layer) ---------------------------------
 
+// LDS of a walk workgroup: histogram | photon ticks | photon links, the
+// link area doubling as the chi2 term buffer (2 x B doubles).
 size_t walk_smem_bytes(int B, uint32_t cap) {
-    return (size_t((B + 1) & ~1) + size_t((cap + 1u) & ~1u) + 2u * size_t(cap)) * 4u;
+    const size_t link_words = std::max(2u * size_t(cap), 4u * size_t(B));
+    return (size_t((B + 1) & ~1) + size_t((cap + 1u) & ~1u) + link_words) * 4u;
 }
 
-hipError_t walk_occupancy(int B, uint32_t cap, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel, kWave,
-                                                        walk_smem_bytes(B, cap));
+hipError_t walk_occupancy(int pass, int B, uint32_t cap, int* blocks_per_cu) {
+    const size_t smem = walk_smem_bytes(B, cap);
+    switch (pass) {
+        case kPassFast:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel<kPassFast>, kWave, smem);
+        case kPassReplay:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel<kPassReplay>, kWave, smem);
+        default:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel<kPassGeneral>, kWave, smem);
+    }
 }
 
-hipError_t launch_walk(uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
-                       const uint2* links, const Model* model, const RunArgs& a, const float* obs,
-                       int32_t* hist_acc, uint32_t* ticket, ResultHeader* result, float* outE,
-                       int32_t* outAcc, float* outF, hipStream_t stream) {
-    hipLaunchKernelGGL(walk_kernel, dim3(grid), dim3(kWave), walk_smem_bytes(a.B, a.cap), stream,
-                       bursts, ticks, links, model, a, obs, hist_acc, ticket, result, outE,
-                       outAcc, outF);
+hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
+                       const uint2* links, const ModelBlob* blob, const RunArgs& a,
+                       const float* obs, int32_t* hist_acc, uint32_t* ticket, uint32_t* replay,
+                       uint32_t* replay_count, ResultHeader* result, float* outE, int32_t* outAcc,
+                       float* outF, hipStream_t stream) {
+    const size_t smem = walk_smem_bytes(a.B, a.cap);
+#define PDA_LAUNCH(P)                                                                       \
+    hipLaunchKernelGGL(walk_kernel<P>, dim3(grid), dim3(kWave), smem, stream, bursts, ticks, \
+                       links, blob, a, obs, hist_acc, ticket, replay, replay_count, result,  \
+                       outE, outAcc, outF)
+    switch (pass) {
+        case kPassFast: PDA_LAUNCH(kPassFast); break;
+        case kPassReplay: PDA_LAUNCH(kPassReplay); break;
+        default: PDA_LAUNCH(kPassGeneral); break;
+    }
+#undef PDA_LAUNCH
     return hipGetLastError();
 }
 

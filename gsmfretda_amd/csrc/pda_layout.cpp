@@ -187,8 +187,28 @@ int build_model(const Photophysics& s, int n, const float* args, int nargs, Mode
         M.base[i] = efficiency_distance(E[i], s.r0);
         M.ep0[i] = distance_efficiency(M.base[i], s.r0);
         if (v[i] != 0.0f) M.any_v = 1;
+        const double p = double(M.ep0[i]);
+        M.flip[i] = p > 0.5 ? 1 : 0;
+        M.pp[i] = M.flip[i] ? 1.0 - p : p;
+        M.qq[i] = 1.0 - M.pp[i];
     }
     return PDA_OK;
+}
+
+// Inversion constants of binomial_inversion() (pda_math.h) per state and count.
+void build_binom_table(const Model& M, BinomTable& T) {
+    std::memset(&T, 0, sizeof T);
+    for (int s = 0; s < M.n; ++s) {
+        const double pp = M.pp[s], q = M.qq[s];
+        const double lq = dlog(q);
+        for (int n = 0; n < kBinomTable; ++n) {
+            T.qn[s][n] = dexp(double(n) * lq);
+            const double np = double(n) * pp;
+            double lim = np + 10.0 * std::sqrt(np * q + 1.0);
+            if (double(n) < lim) lim = double(n);
+            T.bound[s][n] = int32_t(lim);
+        }
+    }
 }
 
 #undef fail

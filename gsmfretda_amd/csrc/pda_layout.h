@@ -48,4 +48,7 @@ int encode_bursts(const int64_t* start, const int64_t* stop, const int64_t* ista
 int build_model(const Photophysics& ph, int n, const float* args, int nargs, Model& M,
                 std::string& err);
 
+// Tabulated inversion constants for the fast walk (needs M from build_model).
+void build_binom_table(const Model& M, BinomTable& T);
+
 }  // namespace pda

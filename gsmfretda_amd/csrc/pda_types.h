@@ -9,6 +9,7 @@ constexpr int kMaxBins = 1024;
 constexpr int kWave = 64;            // lanes per workgroup of the walk kernel
 constexpr uint32_t kTickSentinel = 0x80000000u;  // "never closes a window"
 constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
+constexpr int kBinomTable = 128;     // photon counts with tabulated inversion constants
 
 // One burst, original order (the range [begin, end) of mc::setBurstBd
 // addresses these directly).  16 bytes, one dwordx4 load.
@@ -46,6 +47,24 @@ struct Model {
     float v[kMaxStates];
     float base[kMaxStates];                       // r0 (1/E - 1)^(1/6)
     float ep0[kMaxStates];                        // broadened E when v == 0
+    // Binomial(m, ep0[s]) by inversion (src/binom.cuh:135-169) on
+    // pp = min(p, 1-p): pp, q = 1 - pp, and whether p > 1/2 (draw n - X).
+    double pp[kMaxStates];
+    double qq[kMaxStates];
+    int32_t flip[kMaxStates];
+};
+
+// q^m and the inversion bound min(m, mp + 10 sqrt(mpq + 1)) per state and
+// photon count m < kBinomTable, for the fast walk (v == 0).
+struct BinomTable {
+    double qn[kMaxStates][kBinomTable];
+    int32_t bound[kMaxStates][kBinomTable];
+};
+
+// What a stream slot uploads per parameter set.
+struct ModelBlob {
+    Model m;
+    BinomTable tab;
 };
 
 // Arguments of one launch of the walk kernel (passed by value).
@@ -60,6 +79,7 @@ struct RunArgs {
     float clk_p, bg_ad, bg_dd;
     uint32_t cap;        // LDS photon capacity of a tile
     int32_t nstates;     // for the chi2 degrees of freedom
+    uint32_t replay_cap; // capacity of the replay list
 };
 
 // Device result of one run, followed by int32 hist[B].

@@ -140,7 +140,9 @@ int pda_last_run_ms(pda_ctx *ctx, int sid, float *ms);
  * bit for bit with the host.  op: 0 philox (in: 6 u32 per item -> 4 u32),
  * 1 flog, 2 fexp, 3 dlog, 4 dexp, 5 cos_turns, 6 gauss (2 u32 -> f32),
  * 7 binomial (in: n i32, p f64, key 4 u32, w u32, words 2 u32 -> i64),
- * 8 background_count (in: lam, cap, u1, u2 f32, key 4 u32, w, sub -> f32).
+ * 8 background_count (in: lam, cap, u1, u2 f32, key 4 u32, w, sub -> f32),
+ * 9 plain copy of u32 words (a kernel of known traffic, for counter
+ * calibration).
  * in/out are host arrays; the layout per op is documented in pda_capi.cpp. */
 int pda_debug_primitive(pda_ctx *ctx, int op, const void *in, void *out, int64_t n);
 

@@ -14,6 +14,9 @@
 using namespace pda;
 
 static std::string g_err;
+static int64_t replays = 0;
+
+extern "C" int64_t hw_replays() { return replays; }
 
 extern "C" const char* hw_error() { return g_err.c_str(); }
 
@@ -45,6 +48,8 @@ extern "C" int hw_run(const int64_t* start, const int64_t* stop, const int64_t* 
     Model M;
     rc = build_model(ph, n, args, n * (n + 1), M, g_err);
     if (rc) return rc;
+    BinomTable tab;
+    build_binom_table(M, tab);
     RunArgs a{};
     a.R = uint32_t(R);
     a.seed = seed;
@@ -57,8 +62,16 @@ extern "C" int hw_run(const int64_t* start, const int64_t* stop, const int64_t* 
         const BurstDesc& d = bursts[size_t(b)];
         for (int r = 0; r < R; ++r) {
             const UnitKey key{seed, set, uint32_t(r), uint32_t(b)};
-            const WalkOut o = walk_unit(M, key, d.P, d.stop_rel, d.T, ticks.data() + d.off,
-                                        links.data() + d.off, a);
+            // fast form unless the model broadens; abandoned units replay in full
+            WalkOut o{0.0f, 0.0f, true};
+            if (!M.any_v)
+                o = walk_unit<true>(M, &tab, key, d.P, d.stop_rel, d.T, ticks.data() + d.off,
+                                    links.data() + d.off, a);
+            if (o.replay) {
+                ++replays;
+                o = walk_unit<false>(M, nullptr, key, d.P, d.stop_rel, d.T, ticks.data() + d.off,
+                                     links.data() + d.off, a);
+            }
             const int64_t u = (b - b0) * R + r;
             acc[u] = int32_t(o.acc);
             F[u] = o.F;

@@ -201,3 +201,25 @@ def test_c1_full_histogram(gpu_mc, oracle):
     assert hists[0][1] == hists[1][1]
     want = oracle.run_threaded(od, p, R=cfg["R"], seed=3001, nthreads=16)
     assert np.array_equal(hists[0][0].astype(np.int64), want)
+
+
+def test_replayed_units(gpu_mc, oracle):
+    """Units the fast pass abandons (BTPE draws, background means > 64) are
+    replayed by the general pass with identical results."""
+    args = synth.make_args([0.05, 0.2, 0.35, 0.5, 0.65, 0.8, 0.9, 0.97], np.full(56, 150.0),
+                           np.zeros(8))
+    d = synth.generate(200, synth.make_args([0.97, 0.97], [10.0, 10.0]), 2, base_photons=300,
+                       mean_geom=200, seed=3)
+    _load(gpu_mc, d, np.zeros(200, np.float32))
+    g = _gpu_run(gpu_mc, 8, args, 12, 5, ph=(1.0, 0.0, 0.0, 52.0))
+    o = _oracle_run(oracle, d, 8, args, (1.0, 0.0, 0.0, 52.0), 12, 5)
+    _compare_units(g, o)
+    from tests.test_oracle_walk import tiny_bursts
+    d = tiny_bursts(np.random.default_rng(52), 60)
+    d.bg_dd, d.bg_ad = 1e6, 3e6
+    _load(gpu_mc, d, np.zeros(60, np.float32))
+    args = synth.make_args([0.1, 0.3, 0.6, 0.9], np.full(12, 1e4))
+    for R in (9, 70):
+        g = _gpu_run(gpu_mc, 4, args, R, 1002, ph=(0.34, 0.08, 0.07, 52.0))
+        o = _oracle_run(oracle, d, 4, args, (0.34, 0.08, 0.07, 52.0), R, 1002)
+        _compare_units(g, o)

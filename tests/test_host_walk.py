@@ -23,6 +23,7 @@ _P = ctypes.POINTER
 def hostwalk():
     L = ctypes.CDLL(build_hostwalk())
     L.hw_error.restype = ctypes.c_char_p
+    L.hw_replays.restype = ctypes.c_int64
     return L
 
 
@@ -85,8 +86,11 @@ def test_tiny_bursts_all_branches(oracle, hostwalk, seed):
     if seed == 3:
         rates[2] = 0.0
     v = np.where(rng.random(n) < 0.5, rng.uniform(0, 8, n), 0.0)
+    before = hostwalk.hw_replays()
     compare(oracle, hostwalk, d, n, synth.make_args(E, rates, v), (0.34, 0.08, 0.07, 52.0), 9,
             1000 + seed, set_id=seed)
+    if seed == 2 and not v.any():
+        assert hostwalk.hw_replays() > before   # background means > 64 leave the fast form
 
 
 def test_eight_states_and_long_runs(oracle, hostwalk):
@@ -95,7 +99,19 @@ def test_eight_states_and_long_runs(oracle, hostwalk):
                            np.full(56, 150.0), np.zeros(8))
     d = synth.generate(200, synth.make_args([0.97, 0.97], [10.0, 10.0]), 2, base_photons=300,
                        mean_geom=200, seed=3)
+    before = hostwalk.hw_replays()
     compare(oracle, hostwalk, d, 8, args, (1.0, 0.0, 0.0, 52.0), 12, 5)
+    assert hostwalk.hw_replays() > before           # BTPE draws are replayed
+
+
+def test_background_replays(oracle, hostwalk):
+    rng = np.random.default_rng(52)
+    d = tiny_bursts(rng, 60)
+    d.bg_dd, d.bg_ad = 1e6, 3e6
+    before = hostwalk.hw_replays()
+    compare(oracle, hostwalk, d, 4, synth.make_args([0.1, 0.3, 0.6, 0.9], np.full(12, 1e4)),
+            (0.34, 0.08, 0.07, 52.0), 9, 1002)
+    assert hostwalk.hw_replays() > before
 
 
 def test_link_encoding_brute_force(hostwalk):
