@@ -25,8 +25,8 @@ hipError_t walk_occupancy(int pass, int B, uint32_t cap, int* blocks_per_cu);
 hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
                        const uint2* links, const ModelBlob* blob, const RunArgs& a,
                        const float* obs, int32_t* hist_acc, uint32_t* ticket, uint32_t* replay,
-                       uint32_t* replay_count, ResultHeader* result, float* outE, int32_t* outAcc,
-                       float* outF, hipStream_t stream);
+                       uint32_t* replay_count, uint32_t* work, ResultHeader* result, float* outE,
+                       int32_t* outAcc, float* outF, hipStream_t stream);
 hipError_t launch_primitive(int op, const void* in, void* out, int64_t n, hipStream_t stream);
 }  // namespace pda
 
@@ -54,7 +54,8 @@ int fail(int code, const char* fmt, ...) {
                         __LINE__);                                                         \
     } while (0)
 
-constexpr uint32_t kDefaultCap = 384;  // LDS photon capacity of a 64-unit tile
+constexpr uint32_t kDefaultCap = 256;            // LDS photon capacity of a chunk
+constexpr uint32_t kDefaultTilesPerChunk = 0;    // 64-unit tiles per work batch (0 = auto)
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -63,6 +64,7 @@ struct Slot {
     bool any_v = false;           // broadening: the general pass walks every unit
     uint32_t* d_replay = nullptr;  // units the fast pass abandoned
     uint32_t* d_replay_count = nullptr;
+    uint32_t* d_work = nullptr;    // units handed out by the current run
     size_t replay_cap = 0;
     hipEvent_t model_copied = nullptr;
     bool model_pending = false;
@@ -101,6 +103,7 @@ struct pda_ctx {
     int B = 0, left_pad = 2;
     float* d_obs = nullptr;
     uint32_t cap = kDefaultCap;
+    uint32_t tiles_per_chunk = kDefaultTilesPerChunk;
 };
 
 namespace {
@@ -222,6 +225,8 @@ int pda_create(int device, int nstreams, pda_ctx** out) {
             hipMalloc(&s.d_blob, sizeof(ModelBlob)) != hipSuccess ||
             hipMalloc(&s.d_replay_count, sizeof(uint32_t)) != hipSuccess ||
             hipMemset(s.d_replay_count, 0, sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(&s.d_work, sizeof(uint32_t) * kMaxQueues) != hipSuccess ||
+            hipMemset(s.d_work, 0, sizeof(uint32_t) * kMaxQueues) != hipSuccess ||
             hipEventCreateWithFlags(&s.model_copied, hipEventDisableTiming) != hipSuccess ||
             hipMalloc(&s.d_hist_acc, sizeof(int32_t) * kMaxBins) != hipSuccess ||
             hipMalloc(&s.d_ticket, sizeof(uint32_t)) != hipSuccess ||
@@ -252,6 +257,7 @@ int pda_destroy(pda_ctx* ctx) {
         (void)hipHostFree(s.h_blob);
         (void)hipFree(s.d_replay);
         (void)hipFree(s.d_replay_count);
+        (void)hipFree(s.d_work);
         (void)hipFree(s.d_hist_acc);
         (void)hipFree(s.d_ticket);
         (void)hipFree(s.d_result);
@@ -387,7 +393,6 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
     a.R = uint32_t(R);
     a.seed = seed;
     a.set = set_id;
-    a.ntiles = uint32_t((nunits + kWave - 1) / kWave);
     a.B = ctx->B;
     a.left_pad = ctx->left_pad;
     a.bg_on = ctx->bg_dd < 1e-4f ? 0 : 1;  // src/mc.cu:153
@@ -400,12 +405,20 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
     int per_cu = 0;
     HIP_TRY(walk_occupancy(pass, a.B, a.cap, &per_cu));
     if (per_cu < 1) per_cu = 1;
-    // persistent grid: a multiple of 8 workgroups (one share per XCD)
-    const uint32_t max_per_xcd = uint32_t(std::max(1, ctx->num_cu * per_cu / 8));
-    const uint32_t tiles_per_xcd = (a.ntiles + 7u) / 8u;
-    a.tiles_per_xcd = tiles_per_xcd;
-    const uint32_t per_xcd = std::max(1u, std::min(max_per_xcd, tiles_per_xcd));
-    const uint32_t grid = 8u * per_xcd;
+    // persistent grid of one-wave workgroups, as many as fit on the device;
+    // batches of units per work-counter grab: the tuned size, or about two
+    // grabs per wave (a grab is a round trip to the counter that stalls the
+    // whole wave; measured slower below ~512 units on 1e7-unit runs)
+    const uint64_t max_grid = uint64_t(ctx->num_cu) * uint64_t(per_cu);
+    uint64_t batch = uint64_t(kWave) * ctx->tiles_per_chunk;
+    if (ctx->tiles_per_chunk == 0) {
+        batch = (uint64_t(nunits) / (max_grid * 2u) + kWave - 1) / kWave * kWave;
+        batch = std::min<uint64_t>(std::max<uint64_t>(batch, kWave), 4096);
+    }
+    const uint64_t batches = (uint64_t(nunits) + batch - 1) / batch;
+    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, batches)));
+    a.batch = uint32_t(batch);
+    a.queues = std::min<uint32_t>(kMaxQueues, grid);
 
     float* outE = nullptr;
     int32_t* outAcc = nullptr;
@@ -446,13 +459,14 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
     HIP_TRY(hipEventRecord(s->ev0, s->stream));
     HIP_TRY(launch_walk(pass, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, a,
                         ctx->d_obs, s->d_hist_acc, s->d_ticket, s->d_replay, s->d_replay_count,
-                        s->d_result, outE, outAcc, outF, s->stream));
+                        s->d_work, s->d_result, outE, outAcc, outF, s->stream));
     if (pass == 0) {
         // the replayed units are few: a small persistent grid walks the list
-        const uint32_t rgrid = std::min(grid, 8u * 32u);
+        const uint32_t rgrid = 256u;
         HIP_TRY(launch_walk(1, rgrid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, a,
                             ctx->d_obs, s->d_hist_acc, s->d_ticket, s->d_replay,
-                            s->d_replay_count, s->d_result, outE, outAcc, outF, s->stream));
+                            s->d_replay_count, s->d_work, s->d_result, outE, outAcc, outF,
+                            s->stream));
     }
     HIP_TRY(hipEventRecord(s->ev1, s->stream));
     HIP_TRY(hipMemcpyAsync(s->h_result, s->d_result, sizeof(ResultHeader) + sizeof(int32_t) * size_t(a.B),
@@ -529,6 +543,26 @@ int pda_last_run_ms(pda_ctx* ctx, int sid, float* ms) {
     if (!s->ran) return fail(PDA_E_STATE, "stream %d has not run", sid);
     HIP_TRY(hipEventSynchronize(s->ev1));
     HIP_TRY(hipEventElapsedTime(ms, s->ev0, s->ev1));
+    return PDA_OK;
+}
+
+int pda_set_tuning(pda_ctx* ctx, int tiles_per_chunk, int lds_photons) {
+    if (!ctx) return fail(PDA_E_ARG, "null context");
+    if (tiles_per_chunk < 0 || tiles_per_chunk > 64)
+        return fail(PDA_E_ARG, "tiles_per_chunk %d outside [0, 64]", tiles_per_chunk);
+    if (lds_photons < 0 || lds_photons > 8192)
+        return fail(PDA_E_ARG, "lds_photons %d outside [0, 8192]", lds_photons);
+    int rc = sync_all(ctx);
+    if (rc) return rc;
+    ctx->tiles_per_chunk = uint32_t(tiles_per_chunk);
+    ctx->cap = uint32_t(lds_photons);
+    return PDA_OK;
+}
+
+int pda_get_tuning(const pda_ctx* ctx, int* tiles_per_chunk, int* lds_photons) {
+    if (!ctx) return fail(PDA_E_ARG, "null context");
+    if (tiles_per_chunk) *tiles_per_chunk = int(ctx->tiles_per_chunk);
+    if (lds_photons) *lds_photons = int(ctx->cap);
     return PDA_OK;
 }
 

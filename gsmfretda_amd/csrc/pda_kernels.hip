@@ -39,8 +39,8 @@ enum Pass : int {
 // accumulation and FP64) plus the reset of every per-run counter.
 template <int kPass>
 __device__ void finish(int32_t* s_hist, const RunArgs& a, const float* obs, int32_t* hist_acc,
-                       uint32_t* ticket, uint32_t* replay_count, ResultHeader* result,
-                       uint32_t* s_last) {
+                       uint32_t* ticket, uint32_t* replay_count, uint32_t* work,
+                       ResultHeader* result, uint32_t* s_last) {
     const uint32_t lane = threadIdx.x;
     for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
         const int32_t v = s_hist[i];
@@ -101,6 +101,7 @@ __device__ void finish(int32_t* s_hist, const RunArgs& a, const float* obs, int3
         result->B = a.B;
         result->pad = 0;
         atomicExch(replay_count, 0u);
+        for (uint32_t q = 0; q < kMaxQueues; ++q) atomicExch(&work[q], 0u);
         atomicExch(ticket, 0u);
     }
 }
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(
     const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
     const uint2* __restrict__ links, const ModelBlob* __restrict__ blob, const RunArgs a,
     const float* __restrict__ obs, int32_t* __restrict__ hist_acc, uint32_t* __restrict__ ticket,
-    uint32_t* __restrict__ replay, uint32_t* __restrict__ replay_count,
+    uint32_t* __restrict__ replay, uint32_t* __restrict__ replay_count, uint32_t* __restrict__ work,
     ResultHeader* __restrict__ result, float* __restrict__ outE, int32_t* __restrict__ outAcc,
     float* __restrict__ outF) {
     extern __shared__ uint32_t smem[];
@@ -167,54 +168,84 @@ __global__ __launch_bounds__(kWave) void walk_kernel(
             }
             __syncthreads();
         }
-        finish<kPass>(s_hist, a, obs, hist_acc, ticket, replay_count, result, &s_last);
+        finish<kPass>(s_hist, a, obs, hist_acc, ticket, replay_count, work, result, &s_last);
         return;
     }
 
-    // XCD-aware order: workgroup g runs on XCD g % 8 and walks the contiguous
-    // tile chunk of that XCD, so neighbouring tiles (which share bursts)
-    // meet in the same L2.
-    const uint32_t xcd = blockIdx.x & 7u;
-    const uint32_t per_xcd = gridDim.x >> 3;
-    const uint32_t t_begin = xcd * a.tiles_per_xcd;
-    uint32_t t_end = t_begin + a.tiles_per_xcd;
-    if (t_end > a.ntiles) t_end = a.ntiles;
-    for (uint32_t tile = t_begin + (blockIdx.x >> 3); tile < t_end; tile += per_xcd) {
-        const uint32_t u0 = tile * kWave;
-        const uint32_t ulast = (u0 + kWave < a.nunits ? u0 + kWave : a.nunits) - 1u;
-        const uint32_t bfirst = a.begin + u0 / a.R;
-        const uint32_t blast = a.begin + ulast / a.R;
-        const uint32_t p0 = bursts[bfirst].off;
-        const BurstDesc dlast = bursts[blast];
-        const uint32_t span = dlast.off + dlast.P - p0;
-        const bool staged = span <= a.cap;
-        if (staged) {
-            for (uint32_t i = lane; i < span; i += kWave) {
-                s_tick[i] = ticks[p0 + i];
-                s_link[i] = links[p0 + i];
+    // Work stealing at two levels.  A wave takes batches of consecutive units
+    // from the run's work counter; its lanes take units from the batch as
+    // they need them.  One loop pass advances every busy lane by one window,
+    // and a lane whose unit ends starts another in the same pass, so lanes
+    // do not wait for the wave's longest unit and waves do not wait for the
+    // grid's slowest chunk.  Photons are read through the L1/L2 caches.
+    // The units are split into a.queues contiguous ranges with a counter
+    // each; workgroup g draws from queue g % a.queues, i.e. from its own XCD's
+    // counter when a.queues == 8 (workgroups are dispatched to the XCDs
+    // round-robin), so the counters live at 8 addresses instead of one.
+    constexpr bool kFastForm = kPass == kPassFast;
+    const BinomTable* tab = &blob->tab;
+    const uint32_t queue = blockIdx.x % a.queues;
+    const uint32_t q_begin = uint32_t(uint64_t(a.nunits) * queue / a.queues);
+    const uint32_t q_end = uint32_t(uint64_t(a.nunits) * (queue + 1u) / a.queues);
+    uint32_t pool_next = 0, pool_end = 0;  // the wave's current batch
+    bool busy = false;
+    uint32_t u = 0;
+    WalkState<kFastForm> st{};
+    UnitKey key{a.seed, a.set, 0u, 0u};
+    BurstDesc d{0u, 0u, 0u, 0.0f};
+    bool drained = false;
+    while (true) {
+        bool need = !busy;
+        while (!drained) {
+            const uint64_t want = __ballot(need);
+            if (want == 0) break;
+            if (pool_next >= pool_end) {
+                uint32_t off = 0;
+                if (lane == 0) off = atomicAdd(&work[queue], a.batch);
+                off = __shfl(off, 0);
+                if (off >= q_end - q_begin) {
+                    drained = true;
+                    break;
+                }
+                pool_next = q_begin + off;
+                pool_end = pool_next + a.batch < q_end ? pool_next + a.batch : q_end;
+            }
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                uint32_t(want >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(want), 0u));
+            const uint32_t avail = pool_end - pool_next;
+            if (need && rank < avail) {
+                u = pool_next + rank;
+                const uint32_t bl = u / a.R;
+                const uint32_t b = a.begin + bl;
+                d = bursts[b];
+                key.r = u - bl * a.R;
+                key.b = b;
+                if (begin_unit<kFastForm>(sM, key, d.P, d.T, st) == kStepMore) {
+                    busy = true;
+                    need = false;
+                } else {
+                    tally(s_hist, a, WalkOut{0.0f, 0.0f, false}, u, outE, outAcc, outF);
+                }
+            }
+            const uint32_t taken = uint32_t(__popcll(want));
+            pool_next += taken < avail ? taken : avail;
+        }
+        if (!__any(busy)) break;
+        if (busy) {
+            const int r = step_window<kFastForm>(sM, tab, key, d.P, d.stop_rel, d.T,
+                                                 ticks + d.off, links + d.off, a, st);
+            if (r != kStepMore) {
+                if (r == kStepReplay) {
+                    const uint32_t slot = atomicAdd(replay_count, 1u);
+                    if (slot < a.replay_cap) replay[slot] = u;
+                } else {
+                    tally(s_hist, a, WalkOut{st.acc, st.F, false}, u, outE, outAcc, outF);
+                }
+                busy = false;
             }
         }
-        __syncthreads();
-        const uint32_t u = u0 + lane;
-        if (u < a.nunits) {
-            const uint32_t bl = u / a.R;
-            const uint32_t b = a.begin + bl;
-            const BurstDesc d = bursts[b];
-            const uint32_t* tk = staged ? s_tick + (d.off - p0) : ticks + d.off;
-            const uint2* lk = staged ? s_link + (d.off - p0) : links + d.off;
-            const UnitKey key{a.seed, a.set, u - bl * a.R, b};
-            const WalkOut o = walk_unit<kPass == kPassFast>(sM, &blob->tab, key, d.P, d.stop_rel,
-                                                            d.T, tk, lk, a);
-            if (o.replay) {
-                const uint32_t slot = atomicAdd(replay_count, 1u);
-                if (slot < a.replay_cap) replay[slot] = u;
-            } else {
-                tally(s_hist, a, o, u, outE, outAcc, outF);
-            }
-        }
-        __syncthreads();
     }
-    finish<kPass>(s_hist, a, obs, hist_acc, ticket, replay_count, result, &s_last);
+    finish<kPass>(s_hist, a, obs, hist_acc, ticket, replay_count, work, result, &s_last);
 }
 
 // Evaluates shared-library primitives for the bit-exactness tests.
@@ -293,13 +324,13 @@ hipError_t walk_occupancy(int pass, int B, uint32_t cap, int* blocks_per_cu) {
 hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
                        const uint2* links, const ModelBlob* blob, const RunArgs& a,
                        const float* obs, int32_t* hist_acc, uint32_t* ticket, uint32_t* replay,
-                       uint32_t* replay_count, ResultHeader* result, float* outE, int32_t* outAcc,
-                       float* outF, hipStream_t stream) {
+                       uint32_t* replay_count, uint32_t* work, ResultHeader* result, float* outE,
+                       int32_t* outAcc, float* outF, hipStream_t stream) {
     const size_t smem = walk_smem_bytes(a.B, a.cap);
 #define PDA_LAUNCH(P)                                                                       \
     hipLaunchKernelGGL(walk_kernel<P>, dim3(grid), dim3(kWave), smem, stream, bursts, ticks, \
-                       links, blob, a, obs, hist_acc, ticket, replay, replay_count, result,  \
-                       outE, outAcc, outF)
+                       links, blob, a, obs, hist_acc, ticket, replay, replay_count, work,    \
+                       result, outE, outAcc, outF)
     switch (pass) {
         case kPassFast: PDA_LAUNCH(kPassFast); break;
         case kPassReplay: PDA_LAUNCH(kPassReplay); break;

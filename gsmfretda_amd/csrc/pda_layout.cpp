@@ -182,6 +182,9 @@ int build_model(const Photophysics& s, int n, const float* args, int nargs, Mode
         run += acc;
         M.p0cum[i] = run;
     }
+    // drawDisIdx returns 0 for every u in (0,1] once the first running sum
+    // exceeds 1 (an inflow above 1 s^-1): the draw need not be made
+    M.init_state = M.p0cum[0] > 1.0f ? 0 : -1;
     for (int i = 0; i < n; ++i) {
         M.v[i] = v[i];
         M.base[i] = efficiency_distance(E[i], s.r0);

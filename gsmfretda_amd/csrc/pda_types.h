@@ -10,6 +10,7 @@ constexpr int kWave = 64;            // lanes per workgroup of the walk kernel
 constexpr uint32_t kTickSentinel = 0x80000000u;  // "never closes a window"
 constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
 constexpr int kBinomTable = 128;     // photon counts with tabulated inversion constants
+constexpr uint32_t kMaxQueues = 8;   // work counters of a run (one per XCD)
 
 // One burst, original order (the range [begin, end) of mc::setBurstBd
 // addresses these directly).  16 bytes, one dwordx4 load.
@@ -37,10 +38,11 @@ struct BurstDesc {
 struct Model {
     int32_t n;
     int32_t any_v;                // some v[s] != 0: broadening draws needed
+    int32_t init_state;           // >= 0: the initial-state draw always gives this state
     float gmx;                    // gamma - Dch2Ach
     float omdd;                   // 1 - DexDirAem
     float r0;
-    float pad0[3];
+    float pad0[2];
     float p0cum[kMaxStates];                      // running sums of genMatP's inflow
     float nscum[kMaxStates][kMaxStates];          // [s][jj] running sums of P_i2j column s
     float krate[kMaxStates][kMaxStates];          // [s][j] = K(j, s), rate s -> j
@@ -73,8 +75,8 @@ struct RunArgs {
     uint32_t nunits;     // bursts * R
     uint32_t R;
     uint32_t seed, set;
-    uint32_t ntiles;     // ceil(nunits / kWave)
-    uint32_t tiles_per_xcd;
+    uint32_t batch;      // units a wave takes from its work counter at a time
+    uint32_t queues;     // work counters in use, 1..kMaxQueues
     int32_t B, left_pad, bg_on;
     float clk_p, bg_ad, bg_dd;
     uint32_t cap;        // LDS photon capacity of a tile

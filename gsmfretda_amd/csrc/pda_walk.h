@@ -14,6 +14,8 @@
 //     scratch by the general form; the keyed random streams make the replay
 //     produce exactly the draws the fast form would have consumed.
 #pragma once
+#include <type_traits>
+
 #include "pda_math.h"
 #include "pda_types.h"
 
@@ -86,113 +88,163 @@ PDA_HD float background_single(float lam, float cap, float u_first, float u_cap)
     return floorf(cap * u_cap);
 }
 
-// One (burst, resample) unit: the body of mc_kernel (src/mc.cu:92-196).
+// Walk state of one unit between windows.  The fast form keeps the window
+// ends in 32 bits (ticks < 2^31; a larger end is replayed), the general
+// form in 64 bits as the reference.
+template <bool kFast>
+struct WalkState {
+    using Tick = typename std::conditional<kFast, uint32_t, int64_t>::type;
+    int s;        // current state
+    float tsum;   // mcSpendTime
+    Tick w0;      // start tick of the current window (bin0clk_t)
+    uint32_t c;   // photon cursor (binIdxStart)
+    uint32_t w;   // window index
+    float acc, F;
+};
+
+enum StepResult : int { kStepMore = 0, kStepDone = 1, kStepReplay = 2 };
+
+// Start of a unit (src/mc.cu:92-116): initial state from genMatP's inflow.
+// Returns kStepDone when the unit has no window that counts anything
+// (zero or one photon, or T <= 0: E = 0).
+template <bool kFast>
+PDA_HD int begin_unit(const Model& M, const UnitKey& key, uint32_t P, float T,
+                      WalkState<kFast>& st) {
+    st.s = 0;
+    st.tsum = 0.0f;
+    st.w0 = 0;
+    st.c = 0;
+    st.w = 0;
+    st.acc = 0.0f;
+    st.F = 0.0f;
+    if (P <= 1u || !(T > 0.0f)) return kStepDone;
+    st.s = M.init_state >= 0
+               ? M.init_state
+               : first_above(M.p0cum, M.n,
+                             uniform24(draw_block(key, kInitWindow, kSubMain).w[0]));
+    return kStepMore;
+}
+
+// One dwell window of the walk (one pass of the while loop, src/mc.cu:117-191).
 // tick/link point at the burst's photons in LDS or in global memory.
+template <bool kFast>
+PDA_HD int step_window(const Model& M, const BinomTable* tab, const UnitKey& key, uint32_t P,
+                       uint32_t stop_rel, float T, const uint32_t* tick, const uint2* link,
+                       const RunArgs& a, WalkState<kFast>& st) {
+    using Tick = typename WalkState<kFast>::Tick;
+    const int n = M.n;
+    const int s = st.s;
+    const uint32_t w = st.w;
+    const uint32_t last = P - 1u;
+    const Block A = draw_block(key, w, kSubMain);
+    // next state (drawJ_Si2Sj, src/gen_rand.cuh:78-89); forced for n == 2
+    int j;
+    if (n == 2) {
+        j = 1 - s;
+    } else {
+        const int jj = first_above(M.nscum[s], n - 1, uniform24(A.w[0]));
+        j = jj >= s ? jj + 1 : jj;
+    }
+    // dwell at the chosen transition's rate (drawTau, src/mc.cu:122)
+    const float tsum = st.tsum + flog(uniform24(A.w[1])) / (-M.krate[s][j]);
+    Tick w1;
+    uint32_t w1c;
+    if (!(tsum < T)) {
+        w1 = Tick(stop_rel);
+        w1c = stop_rel;
+    } else {
+        const float wf = float(st.w0) + tsum / a.clk_p;  // src/mc.cu:132
+        if (kFast) {
+            if (wf >= 0x1p31f) return kStepReplay;
+            w1 = Tick(uint32_t(wf));
+            w1c = uint32_t(w1);
+        } else {
+            const int64_t w64 = wf >= 0x1p62f ? (int64_t(1) << 62) : int64_t(wf);
+            w1 = Tick(w64);
+            w1c = w64 <= 0 ? 0u : (w64 >= int64_t(kTickSentinel) ? kTickSentinel : uint32_t(w64));
+        }
+    }
+    // window close: first channel photon at or after w1, else the last photon
+    uint32_t lo = st.c, hi = P;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tick[mid] < w1c) lo = mid + 1u;
+        else hi = mid;
+    }
+    uint32_t close = last;
+    if (lo < P) {
+        const uint32_t f1 = link[lo].x & kIndexMask;
+        if (f1 < close) close = f1;
+    }
+    // sticky-flag count over [c, close) (src/mc.cu:171-180)
+    const uint2 L = link[st.c];
+    const uint32_t f = L.x & kIndexMask;
+    uint32_t cnt = 0;
+    if (f < close) cnt = (L.y < close ? L.y : close) - f;
+    const uint32_t colour = L.x >> 30;
+    const float nd = colour == 1u ? float(cnt) : 0.0f;
+    const float na = colour == 2u ? float(cnt) : 0.0f;
+    const float fv = M.gmx * nd + M.omdd * na;
+    float m;
+    if (!a.bg_on) {
+        m = floorf(fv + 0.5f);
+    } else {  // background photons (src/mc.cu:153-166)
+        const Block Bk = draw_block(key, w, kSubBackground);
+        float dt;
+        if (kFast) dt = float(int32_t(uint32_t(w1) - uint32_t(st.w0))) * a.clk_p;
+        else dt = float(int64_t(w1) - int64_t(st.w0)) * a.clk_p;
+        float bga, bgd;
+        if (kFast) {
+            bga = background_single(dt * a.bg_ad, M.omdd * na, uniform24(Bk.w[0]),
+                                    uniform24(Bk.w[2]));
+            bgd = background_single(dt * a.bg_dd, M.gmx * nd, uniform24(Bk.w[1]),
+                                    uniform24(Bk.w[3]));
+            if (bga < 0.0f || bgd < 0.0f) return kStepReplay;
+        } else {
+            bga = background_count(dt * a.bg_ad, M.omdd * na, uniform24(Bk.w[0]),
+                                   uniform24(Bk.w[2]), key, w, kSubPoissonA);
+            bgd = background_count(dt * a.bg_dd, M.gmx * nd, uniform24(Bk.w[1]),
+                                   uniform24(Bk.w[3]), key, w, kSubPoissonD);
+        }
+        m = floorf(((fv - bgd) - bga) + 0.5f);
+    }
+    int64_t x;
+    if (kFast) {
+        x = binomial_table(int(m), s, M, *tab, A.w[2], A.w[3]);
+        if (x < 0) return kStepReplay;
+    } else {
+        // broadened efficiency (drawE, src/gen_rand.cuh:63-68)
+        float de;
+        if (M.v[s] == 0.0f) {
+            de = M.ep0[s];
+        } else {
+            const Block C = draw_block(key, w, kSubNormal);
+            de = distance_efficiency(gauss(C.w[0], C.w[1]) * M.v[s] + M.base[s], M.r0);
+        }
+        x = binomial(int(m), double(de), A.w[2], A.w[3], key, w);
+    }
+    st.F += fv;
+    st.acc += float(x);
+    st.tsum = tsum;
+    st.w0 = w1;
+    st.s = j;
+    st.c = close;
+    st.w = w + 1u;
+    // loop condition of the next window; every window after the cursor
+    // reached the last photon is empty
+    if (close == last || !(T > tsum) || st.w >= kMaxWindows) return kStepDone;
+    return kStepMore;
+}
+
+// One (burst, resample) unit: the body of mc_kernel (src/mc.cu:92-196).
 template <bool kFast>
 PDA_HD WalkOut walk_unit(const Model& M, const BinomTable* tab, const UnitKey& key, uint32_t P,
                          uint32_t stop_rel, float T, const uint32_t* tick, const uint2* link,
                          const RunArgs& a) {
-    WalkOut out{0.0f, 0.0f, false};
-    // Zero or one photon: every window counts nothing, E = 0.
-    if (P <= 1u || !(T > 0.0f)) return out;
-    const int n = M.n;
-    const uint32_t last = P - 1u;
-    int s = first_above(M.p0cum, n, uniform24(draw_block(key, kInitWindow, kSubMain).w[0]));
-    float tsum = 0.0f;
-    int64_t w0 = 0;
-    uint32_t c = 0;
-    for (uint32_t w = 0; T > tsum && w < kMaxWindows; ++w) {
-        const Block A = draw_block(key, w, kSubMain);
-        // next state (drawJ_Si2Sj, src/gen_rand.cuh:78-89); forced for n == 2
-        int j;
-        if (n == 2) {
-            j = 1 - s;
-        } else {
-            const int jj = first_above(M.nscum[s], n - 1, uniform24(A.w[0]));
-            j = jj >= s ? jj + 1 : jj;
-        }
-        // dwell at the chosen transition's rate (drawTau, src/mc.cu:122)
-        tsum = tsum + flog(uniform24(A.w[1])) / (-M.krate[s][j]);
-        int64_t w1;
-        if (!(tsum < T)) {
-            w1 = int64_t(stop_rel);
-        } else {
-            const float wf = float(w0) + tsum / a.clk_p;  // src/mc.cu:132
-            w1 = wf >= 0x1p62f ? (int64_t(1) << 62) : int64_t(wf);
-        }
-        const uint32_t w1c = w1 <= 0 ? 0u : (w1 >= int64_t(kTickSentinel) ? kTickSentinel
-                                                                          : uint32_t(w1));
-        // window close: first channel photon at or after w1, else the last photon
-        uint32_t lo = c, hi = P;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (tick[mid] < w1c) lo = mid + 1u;
-            else hi = mid;
-        }
-        uint32_t close = last;
-        if (lo < P) {
-            const uint32_t f1 = link[lo].x & kIndexMask;
-            if (f1 < close) close = f1;
-        }
-        // sticky-flag count over [c, close) (src/mc.cu:171-180)
-        const uint2 L = link[c];
-        const uint32_t f = L.x & kIndexMask;
-        uint32_t cnt = 0;
-        if (f < close) cnt = (L.y < close ? L.y : close) - f;
-        const uint32_t colour = L.x >> 30;
-        const float nd = colour == 1u ? float(cnt) : 0.0f;
-        const float na = colour == 2u ? float(cnt) : 0.0f;
-        const float fv = M.gmx * nd + M.omdd * na;
-        out.F += fv;
-        float m;
-        if (!a.bg_on) {
-            m = floorf(fv + 0.5f);
-        } else {  // background photons (src/mc.cu:153-166)
-            const Block Bk = draw_block(key, w, kSubBackground);
-            const float dt = float(w1 - w0) * a.clk_p;
-            float bga, bgd;
-            if (kFast) {
-                bga = background_single(dt * a.bg_ad, M.omdd * na, uniform24(Bk.w[0]),
-                                        uniform24(Bk.w[2]));
-                bgd = background_single(dt * a.bg_dd, M.gmx * nd, uniform24(Bk.w[1]),
-                                        uniform24(Bk.w[3]));
-                if (bga < 0.0f || bgd < 0.0f) {
-                    out.replay = true;
-                    return out;
-                }
-            } else {
-                bga = background_count(dt * a.bg_ad, M.omdd * na, uniform24(Bk.w[0]),
-                                       uniform24(Bk.w[2]), key, w, kSubPoissonA);
-                bgd = background_count(dt * a.bg_dd, M.gmx * nd, uniform24(Bk.w[1]),
-                                       uniform24(Bk.w[3]), key, w, kSubPoissonD);
-            }
-            m = floorf(((fv - bgd) - bga) + 0.5f);
-        }
-        int64_t x;
-        if (kFast) {
-            x = binomial_table(int(m), s, M, *tab, A.w[2], A.w[3]);
-            if (x < 0) {
-                out.replay = true;
-                return out;
-            }
-        } else {
-            // broadened efficiency (drawE, src/gen_rand.cuh:63-68)
-            float de;
-            if (M.v[s] == 0.0f) {
-                de = M.ep0[s];
-            } else {
-                const Block C = draw_block(key, w, kSubNormal);
-                de = distance_efficiency(gauss(C.w[0], C.w[1]) * M.v[s] + M.base[s], M.r0);
-            }
-            x = binomial(int(m), double(de), A.w[2], A.w[3], key, w);
-        }
-        out.acc += float(x);
-        w0 = w1;
-        s = j;
-        c = close;
-        if (c == last) break;  // every later window is empty
-    }
-    return out;
+    WalkState<kFast> st;
+    int r = begin_unit<kFast>(M, key, P, T, st);
+    while (r == kStepMore) r = step_window<kFast>(M, tab, key, P, stop_rel, T, tick, link, a, st);
+    return WalkOut{st.acc, st.F, r == kStepReplay};
 }
 
 }  // namespace pda

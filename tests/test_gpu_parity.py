@@ -223,3 +223,18 @@ def test_replayed_units(gpu_mc, oracle):
         g = _gpu_run(gpu_mc, 4, args, R, 1002, ph=(0.34, 0.08, 0.07, 52.0))
         o = _oracle_run(oracle, d, 4, args, (0.34, 0.08, 0.07, 52.0), R, 1002)
         _compare_units(g, o)
+
+
+def test_launch_shape_does_not_change_results(gpu_mc, oracle):
+    cfg = synth.config(0, n_bursts=700)
+    d = cfg["data"]
+    _load(gpu_mc, d, np.zeros(700, np.float32))
+    o = _oracle_run(oracle, d, cfg["n"], cfg["args"], cfg["photophysics"], 13, 21)
+    saved = gpu_mc.get_tuning()
+    try:
+        for tpc, cap in ((1, 0), (3, 100), (16, 2048), (64, 8192), (0, 256)):
+            gpu_mc.set_tuning(tpc, cap)
+            g = _gpu_run(gpu_mc, cfg["n"], cfg["args"], 13, 21, ph=cfg["photophysics"])
+            _compare_units(g, o)
+    finally:
+        gpu_mc.set_tuning(*saved)
