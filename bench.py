@@ -22,11 +22,12 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 METRIC = "burst-resamples/sec on 1 MI355X; fit wall-time to chi2 tol (grad+batch vs plain)"
-VALU_PEAK = 256 * 64 * 2.4e9  # 32-bit VALU lane-ops/s: 256 CU x 4 SIMD16 x 2.4 GHz
+COST_MODEL = os.path.join(ROOT, "gsmfretda_amd", "cost_model.json")
 
 
-def cpu_baseline(cfg, obs_hist, threads, sample_bursts):
-    """The oracle (REF-semantics CPU MC, same streams) on host cores."""
+def cpu_baseline(cfg, threads, sample_bursts, repeats=3):
+    """The oracle (REF-semantics CPU MC on the same streams) on host cores:
+    median of `repeats` evaluations of the first `sample_bursts` bursts."""
     from oracle import oracle as O
     d = cfg["data"]
     od = O.Data.from_bursts(d)
@@ -34,12 +35,33 @@ def cpu_baseline(cfg, obs_hist, threads, sample_bursts):
     R = cfg["R"]
     b1 = min(sample_bursts, d.nbursts)
     O.run_threaded(od, p, R=R, seed=cfg["seeds"]["mc"], b1=min(b1, 2000), nthreads=threads)
-    t0 = time.perf_counter()
-    O.run_threaded(od, p, R=R, seed=cfg["seeds"]["mc"], b1=b1, nthreads=threads)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        O.run_threaded(od, p, R=R, seed=cfg["seeds"]["mc"], b1=b1, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     return dict(value=b1 * R / dt, unit="burst-resamples/s", cores=threads, kind="port",
-                sample=f"{b1} of {d.nbursts} bursts x {R} resamples, oracle/pda_oracle.c "
-                       f"(single-resample literal scan), {threads} threads, {dt:.2f} s wall")
+                sample=f"{b1} of {d.nbursts} bursts x {R} resamples (the whole evaluation), "
+                       f"oracle/pda_oracle.c literal photon scan, median of {repeats} runs, "
+                       f"{threads} threads, {dt:.2f} s wall each")
+
+
+def roofline(units, walk_ms, total_ms):
+    cm = json.load(open(COST_MODEL))
+    t = walk_ms * 1e-3
+    ops = cm["valu_lane_ops_per_unit"] * units
+    achieved = ops / t / 1e12
+    peak = cm["valu_peak_lane_ops_per_s"] / 1e12
+    valu = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
+            "frac": achieved / peak, "traffic": cm["hbm_traffic_bytes_per_launch"],
+            "kernel": "walk_kernel<0>", "kernel_ms_avg": walk_ms,
+            "ops_per_unit": cm["valu_lane_ops_per_unit"]}
+    hb = cm["hbm_algorithmic_bytes_per_unit"] * units / t / 1e9
+    hbm = {"bound": "hbm", "achieved": hb, "peak": cm["hbm_peak_bytes_per_s"] / 1e9,
+           "unit": "GB/s", "frac": hb * 1e9 / cm["hbm_peak_bytes_per_s"],
+           "traffic": cm["hbm_traffic_bytes_per_launch"]}
+    return valu, hbm
 
 
 def main():
@@ -83,19 +105,19 @@ def main():
     m.synchronize(0)
     obs_E, _, _ = m.get_samples(0, d.nbursts)
     m.set_observed(obs_E, 50, 2)
-    obs_hist = np.histogram(obs_E, bins=50, range=(0, 1))[0]
 
     m.reSampleTimes = R
     seed = cfg["seeds"]["mc"]
     for _ in range(args.warmup):
         m.run_kernel(sid=0, seed=seed)
     m.synchronize(0)
-    kernel_ms = []
+    kernel_ms, walk_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         m.run_kernel(sid=0, seed=seed)
         m.synchronize(0)
         kernel_ms.append(m.last_run_ms(0))
+        walk_ms.append(m.last_pass_ms(0)[0])
     dt = time.perf_counter() - t0
     hist, chi2_64, chi2_ref, effN = m.get_hist_chi2(0)
     units = d.nbursts * R
@@ -118,12 +140,14 @@ def main():
                                "50 E bins, chi2 on device",
                    "bursts": d.nbursts, "photons": int(d.nphotons), "resamples": R,
                    "bins": 50, "states": n, "parallelism": "single GPU"},
-        "kernel_ms_avg": float(np.mean(kernel_ms)),
+        "gpu_ms_per_eval_avg": float(np.mean(kernel_ms)),
         "chi2": chi2_64,
     }
+    out["roofline"], out["roofline_hbm"] = roofline(units, float(np.mean(walk_ms)),
+                                                    float(np.mean(kernel_ms)))
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(cfg, obs_hist, threads, args.cpu_bursts)
+        out["cpu_baseline"] = cpu_baseline(cfg, threads, args.cpu_bursts)
     print(json.dumps(out), flush=True)
     m.close()
 

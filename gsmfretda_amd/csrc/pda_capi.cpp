@@ -20,13 +20,17 @@
 #include "pda_types.h"
 
 namespace pda {
-size_t walk_smem_bytes(int B, uint32_t cap);
-hipError_t walk_occupancy(int pass, int B, uint32_t cap, int* blocks_per_cu);
+hipError_t walk_occupancy(int pass, int* blocks_per_cu);
 hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
                        const uint2* links, const ModelBlob* blob, const RunArgs& a,
-                       const float* obs, int32_t* hist_acc, uint32_t* ticket, uint32_t* replay,
-                       uint32_t* replay_count, uint32_t* work, ResultHeader* result, float* outE,
-                       int32_t* outAcc, float* outF, hipStream_t stream);
+                       int32_t* hist_acc, uint32_t* replay, uint32_t* replay_count,
+                       uint32_t* work, float* outE, int32_t* outAcc, float* outF,
+                       hipStream_t stream);
+hipError_t launch_finish(const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
+                         const ModelBlob* blob, const RunArgs& a, const float* obs,
+                         int32_t* hist_acc, const uint32_t* replay, uint32_t* replay_count,
+                         uint32_t* work, ResultHeader* result, float* outE, int32_t* outAcc,
+                         float* outF, hipStream_t stream);
 hipError_t launch_primitive(int op, const void* in, void* out, int64_t n, hipStream_t stream);
 }  // namespace pda
 
@@ -64,7 +68,7 @@ struct Slot {
     bool any_v = false;           // broadening: the general pass walks every unit
     uint32_t* d_replay = nullptr;  // units the fast pass abandoned
     uint32_t* d_replay_count = nullptr;
-    uint32_t* d_work = nullptr;    // units handed out by the current run
+    uint32_t* d_work = nullptr;    // work counters of the run (one per XCD)
     size_t replay_cap = 0;
     hipEvent_t model_copied = nullptr;
     bool model_pending = false;
@@ -74,7 +78,6 @@ struct Slot {
     std::vector<float> args;
     int64_t begin = 0, end = -1;
     int32_t* d_hist_acc = nullptr;
-    uint32_t* d_ticket = nullptr;
     ResultHeader* d_result = nullptr;
     ResultHeader* h_result = nullptr;  // pinned
     int last_B = 0;
@@ -86,7 +89,7 @@ struct Slot {
     float* h_E = nullptr;  // pinned (the reference's hmcE[sid])
     size_t sample_cap = 0;
     bool samples_valid = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev_mid = nullptr, ev1 = nullptr;
 };
 
 }  // namespace
@@ -228,14 +231,15 @@ int pda_create(int device, int nstreams, pda_ctx** out) {
             hipMalloc(&s.d_work, sizeof(uint32_t) * kMaxQueues) != hipSuccess ||
             hipMemset(s.d_work, 0, sizeof(uint32_t) * kMaxQueues) != hipSuccess ||
             hipEventCreateWithFlags(&s.model_copied, hipEventDisableTiming) != hipSuccess ||
-            hipMalloc(&s.d_hist_acc, sizeof(int32_t) * kMaxBins) != hipSuccess ||
-            hipMalloc(&s.d_ticket, sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(&s.d_hist_acc, sizeof(int32_t) * kMaxBins * 8) != hipSuccess ||
+
             hipMalloc(&s.d_result, sizeof(ResultHeader) + sizeof(int32_t) * kMaxBins) != hipSuccess ||
             hipHostMalloc(reinterpret_cast<void**>(&s.h_result),
                           sizeof(ResultHeader) + sizeof(int32_t) * kMaxBins, hipHostMallocDefault) != hipSuccess ||
-            hipEventCreate(&s.ev0) != hipSuccess || hipEventCreate(&s.ev1) != hipSuccess ||
-            hipMemset(s.d_hist_acc, 0, sizeof(int32_t) * kMaxBins) != hipSuccess ||
-            hipMemset(s.d_ticket, 0, sizeof(uint32_t)) != hipSuccess) {
+            hipEventCreate(&s.ev0) != hipSuccess || hipEventCreate(&s.ev_mid) != hipSuccess ||
+            hipEventCreate(&s.ev1) != hipSuccess ||
+            hipMemset(s.d_hist_acc, 0, sizeof(int32_t) * kMaxBins * 8) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess) {
             pda_destroy(ctx);
             return fail(PDA_E_HIP, "stream slot setup failed: %s", hipGetErrorString(hipGetLastError()));
         }
@@ -259,7 +263,6 @@ int pda_destroy(pda_ctx* ctx) {
         (void)hipFree(s.d_replay_count);
         (void)hipFree(s.d_work);
         (void)hipFree(s.d_hist_acc);
-        (void)hipFree(s.d_ticket);
         (void)hipFree(s.d_result);
         (void)hipHostFree(s.h_result);
         (void)hipFree(s.d_E);
@@ -268,6 +271,7 @@ int pda_destroy(pda_ctx* ctx) {
         (void)hipHostFree(s.h_E);
         if (s.model_copied) (void)hipEventDestroy(s.model_copied);
         if (s.ev0) (void)hipEventDestroy(s.ev0);
+        if (s.ev_mid) (void)hipEventDestroy(s.ev_mid);
         if (s.ev1) (void)hipEventDestroy(s.ev1);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
@@ -403,7 +407,7 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
     a.nstates = s->n;
     const int pass = s->any_v ? 2 : 0;  // general pass for broadening, else fast + replay
     int per_cu = 0;
-    HIP_TRY(walk_occupancy(pass, a.B, a.cap, &per_cu));
+    HIP_TRY(walk_occupancy(pass, &per_cu));
     if (per_cu < 1) per_cu = 1;
     // persistent grid of one-wave workgroups, as many as fit on the device;
     // batches of units per work-counter grab: the tuned size, or about two
@@ -458,16 +462,12 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
     a.replay_cap = uint32_t(s->replay_cap);
     HIP_TRY(hipEventRecord(s->ev0, s->stream));
     HIP_TRY(launch_walk(pass, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, a,
-                        ctx->d_obs, s->d_hist_acc, s->d_ticket, s->d_replay, s->d_replay_count,
-                        s->d_work, s->d_result, outE, outAcc, outF, s->stream));
-    if (pass == 0) {
-        // the replayed units are few: a small persistent grid walks the list
-        const uint32_t rgrid = 256u;
-        HIP_TRY(launch_walk(1, rgrid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, a,
-                            ctx->d_obs, s->d_hist_acc, s->d_ticket, s->d_replay,
-                            s->d_replay_count, s->d_work, s->d_result, outE, outAcc, outF,
-                            s->stream));
-    }
+                        s->d_hist_acc, s->d_replay, s->d_replay_count, s->d_work, outE, outAcc,
+                        outF, s->stream));
+    HIP_TRY(hipEventRecord(s->ev_mid, s->stream));
+    HIP_TRY(launch_finish(ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, a, ctx->d_obs,
+                          s->d_hist_acc, s->d_replay, s->d_replay_count, s->d_work, s->d_result,
+                          outE, outAcc, outF, s->stream));
     HIP_TRY(hipEventRecord(s->ev1, s->stream));
     HIP_TRY(hipMemcpyAsync(s->h_result, s->d_result, sizeof(ResultHeader) + sizeof(int32_t) * size_t(a.B),
                            hipMemcpyDeviceToHost, s->stream));
@@ -546,12 +546,25 @@ int pda_last_run_ms(pda_ctx* ctx, int sid, float* ms) {
     return PDA_OK;
 }
 
+int pda_last_pass_ms(pda_ctx* ctx, int sid, float* walk_ms, float* finish_ms) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    if (!s->ran) return fail(PDA_E_STATE, "stream %d has not run", sid);
+    HIP_TRY(hipEventSynchronize(s->ev1));
+    float a = 0.f, b = 0.f;
+    HIP_TRY(hipEventElapsedTime(&a, s->ev0, s->ev_mid));
+    HIP_TRY(hipEventElapsedTime(&b, s->ev_mid, s->ev1));
+    if (walk_ms) *walk_ms = a;
+    if (finish_ms) *finish_ms = b;
+    return PDA_OK;
+}
+
 int pda_set_tuning(pda_ctx* ctx, int tiles_per_chunk, int lds_photons) {
     if (!ctx) return fail(PDA_E_ARG, "null context");
     if (tiles_per_chunk < 0 || tiles_per_chunk > 64)
         return fail(PDA_E_ARG, "tiles_per_chunk %d outside [0, 64]", tiles_per_chunk);
-    if (lds_photons < 0 || lds_photons > 8192)
-        return fail(PDA_E_ARG, "lds_photons %d outside [0, 8192]", lds_photons);
+    if (lds_photons < 0 || lds_photons > 256)
+        return fail(PDA_E_ARG, "lds_photons %d outside [0, 256]", lds_photons);
     int rc = sync_all(ctx);
     if (rc) return rc;
     ctx->tiles_per_chunk = uint32_t(tiles_per_chunk);

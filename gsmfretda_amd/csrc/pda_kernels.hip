@@ -1,25 +1,22 @@
 // pda_kernels.hip -- gfx950 kernels of the Monte-Carlo PDA hot path.
 //
-// walk_kernel replaces gSMFRETda's mc_kernel (src/mc.cu:83-197) together
-// with the host histogram and chi2 of src/streamWorker.cpp:184-206:
-//   * one 64-lane workgroup (one wavefront) per tile of 64 consecutive
-//     (burst, resample) units, burst-major, so the lanes of a wave are
-//     resamples of the same one or two bursts;
-//   * the tile's photons (burst-relative uint32 ticks + channel-run links)
-//     are staged in LDS with coalesced loads, then each lane runs its own
-//     dwell walk, locating every window end by a binary search in LDS
-//     instead of rescanning photons from global memory;
+// walk_kernel + finish_kernel replace gSMFRETda's mc_kernel (src/mc.cu:83-197)
+// together with the host histogram and chi2 of src/streamWorker.cpp:184-206:
+//   * one-wave workgroups; every lane streams through (burst, resample)
+//     units taken from XCD-local work counters, advancing one dwell window
+//     per loop pass, so lanes and waves never wait for the longest unit;
+//   * each window end is located by a binary search over burst-relative
+//     uint32 ticks and the window's sticky-flag count read from per-photon
+//     channel-run links, instead of rescanning photons;
 //   * random numbers are stateless Philox4x32-10 blocks keyed by
 //     (seed, set, window, sub-stream, resample, burst) -- no RNG state in
 //     memory, nothing to initialise (replaces setup_kernel,
 //     src/gen_rand.cuh:7-25);
-//   * each workgroup keeps an LDS int32 histogram, adds it to the run's
-//     accumulator with device-scope atomics, and the last workgroup to
-//     finish computes chi2 and resets the accumulator and ticket, so one
-//     launch is the whole evaluation and needs no memset.
+//   * the common case runs in a lean form (about 80 VGPRs); the rare units
+//     that need BTPE draws, inversion restarts or large background means are
+//     handed to finish_kernel, which replays them in full, reduces the LDS
+//     histograms of all workgroups and computes chi2 on the device.
 #include <hip/hip_runtime.h>
-
-#include <algorithm>
 
 #include "pda_math.h"
 #include "pda_types.h"
@@ -27,84 +24,11 @@
 
 namespace pda {
 
-// Which units a launch walks, and how.
+// Which form a walk launch runs.
 enum Pass : int {
-    kPassFast = 0,     // all units, fast form; abandoned units go to the replay list
-    kPassReplay = 1,   // the replay list, general form; then chi2
-    kPassGeneral = 2,  // all units, general form (broadening); then chi2
+    kPassFast = 0,     // fast form; abandoned units go to the replay list
+    kPassGeneral = 2,  // general form for every unit (models with broadening)
 };
-
-// Histogram flush and, in the last workgroup of the evaluation's final pass,
-// the reduced chi2 of src/streamWorker.cpp:194-206 (reference float
-// accumulation and FP64) plus the reset of every per-run counter.
-template <int kPass>
-__device__ void finish(int32_t* s_hist, const RunArgs& a, const float* obs, int32_t* hist_acc,
-                       uint32_t* ticket, uint32_t* replay_count, uint32_t* work,
-                       ResultHeader* result, uint32_t* s_last) {
-    const uint32_t lane = threadIdx.x;
-    for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
-        const int32_t v = s_hist[i];
-        if (v) atomicAdd(&hist_acc[i], v);
-    }
-    if (kPass == kPassFast) return;
-    __threadfence();
-    if (lane == 0) *s_last = atomicAdd(ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
-    __syncthreads();
-    if (!*s_last) return;
-    __threadfence();
-    int32_t* rhist = reinterpret_cast<int32_t*>(result + 1);
-    for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
-        const int32_t v = atomicExch(&hist_acc[i], 0);
-        s_hist[i] = v;
-        rhist[i] = v;
-    }
-    __syncthreads();
-    // per-bin terms in parallel (FP64, and the reference's double term of a
-    // float difference), then lane 0 folds them in bin order
-    double* s_t64 = reinterpret_cast<double*>(reinterpret_cast<uint32_t*>(s_hist) +
-                                              ((uint32_t(a.B) + 1u) & ~1u) +
-                                              ((a.cap + 1u) & ~1u));  // the link area
-    double* s_t32 = s_t64 + a.B;
-    const float Rf = float(a.R);
-    for (int b = int(lane); b < a.B; b += kWave) {
-        const int32_t Mb = s_hist[b];
-        double t64 = 0.0, t32 = 0.0;
-        if (Mb > 0) {
-            const float Mf = float(Mb);
-            const float dlt = obs[b] - Mf / Rf;
-            t32 = (double(dlt) * double(dlt)) / double(Mf / Rf);
-            const double md = double(Mb) / double(a.R);
-            const double dd = double(obs[b]) - md;
-            t64 = dd * dd / md;
-        }
-        s_t64[b] = t64;
-        s_t32[b] = t32;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        int effN = a.B - a.left_pad;
-        float c32 = 0.0f;
-        double c64 = 0.0;
-        for (int b = a.left_pad; b < a.B; ++b) {
-            if (s_hist[b] > 0) {
-                c32 = float(double(c32) + s_t32[b]);
-                c64 += s_t64[b];
-            } else {
-                --effN;
-            }
-        }
-        const int dof = effN - a.nstates * (a.nstates + 1);
-        result->chi2_ref = c32 / float(dof);
-        result->chi2_64 = c64 / double(dof);
-        result->effN = effN;
-        result->units = int64_t(a.nunits);
-        result->B = a.B;
-        result->pad = 0;
-        atomicExch(replay_count, 0u);
-        for (uint32_t q = 0; q < kMaxQueues; ++q) atomicExch(&work[q], 0u);
-        atomicExch(ticket, 0u);
-    }
-}
 
 __device__ __forceinline__ void tally(int32_t* s_hist, const RunArgs& a, const WalkOut& o,
                                       uint32_t u, float* outE, int32_t* outAcc, float* outF) {
@@ -118,22 +42,21 @@ __device__ __forceinline__ void tally(int32_t* s_hist, const RunArgs& a, const W
     if (outF) outF[u] = o.F;
 }
 
+// The walk over every unit of the run.  Each one-wave workgroup keeps an LDS
+// histogram and adds it to its XCD's accumulator row at the end (workgroups
+// are dispatched to the 8 XCDs round-robin, so the atomics of a launch spread
+// over 8 x B addresses).  No completion ticket and no fence: the finish
+// kernel, launched behind on the same stream, reads the rows.
 template <int kPass>
 __global__ __launch_bounds__(kWave) void walk_kernel(
     const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
     const uint2* __restrict__ links, const ModelBlob* __restrict__ blob, const RunArgs a,
-    const float* __restrict__ obs, int32_t* __restrict__ hist_acc, uint32_t* __restrict__ ticket,
-    uint32_t* __restrict__ replay, uint32_t* __restrict__ replay_count, uint32_t* __restrict__ work,
-    ResultHeader* __restrict__ result, float* __restrict__ outE, int32_t* __restrict__ outAcc,
-    float* __restrict__ outF) {
-    extern __shared__ uint32_t smem[];
+    int32_t* __restrict__ hist_acc, uint32_t* __restrict__ replay,
+    uint32_t* __restrict__ replay_count, uint32_t* __restrict__ work, float* __restrict__ outE,
+    int32_t* __restrict__ outAcc, float* __restrict__ outF) {
     __shared__ Model sM;
-    __shared__ uint32_t s_last;
+    __shared__ int32_t s_hist[kMaxBins];
     const uint32_t lane = threadIdx.x;
-    int32_t* s_hist = reinterpret_cast<int32_t*>(smem);
-    const uint32_t hist_words = (uint32_t(a.B) + 1u) & ~1u;
-    uint32_t* s_tick = smem + hist_words;
-    uint2* s_link = reinterpret_cast<uint2*>(s_tick + ((a.cap + 1u) & ~1u));
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(&blob->m);
         uint32_t* dst = reinterpret_cast<uint32_t*>(&sM);
@@ -142,46 +65,14 @@ __global__ __launch_bounds__(kWave) void walk_kernel(
     }
     __syncthreads();
 
-    if (kPass == kPassReplay) {
-        // units abandoned by the fast pass, general form: one workgroup per
-        // unit, its burst staged in LDS when it fits, lane 0 walks it
-        const uint32_t count = *replay_count < a.replay_cap ? *replay_count : a.replay_cap;
-        for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
-            const uint32_t u = replay[i];
-            const uint32_t bl = u / a.R;
-            const uint32_t b = a.begin + bl;
-            const BurstDesc d = bursts[b];
-            const bool staged = d.P <= a.cap;
-            if (staged) {
-                for (uint32_t k = lane; k < d.P; k += kWave) {
-                    s_tick[k] = ticks[d.off + k];
-                    s_link[k] = links[d.off + k];
-                }
-            }
-            __syncthreads();
-            if (lane == 0) {
-                const UnitKey key{a.seed, a.set, u - bl * a.R, b};
-                const WalkOut o = walk_unit<false>(sM, nullptr, key, d.P, d.stop_rel, d.T,
-                                                   staged ? s_tick : ticks + d.off,
-                                                   staged ? s_link : links + d.off, a);
-                tally(s_hist, a, o, u, outE, outAcc, outF);
-            }
-            __syncthreads();
-        }
-        finish<kPass>(s_hist, a, obs, hist_acc, ticket, replay_count, work, result, &s_last);
-        return;
-    }
-
     // Work stealing at two levels.  A wave takes batches of consecutive units
-    // from the run's work counter; its lanes take units from the batch as
-    // they need them.  One loop pass advances every busy lane by one window,
-    // and a lane whose unit ends starts another in the same pass, so lanes
-    // do not wait for the wave's longest unit and waves do not wait for the
-    // grid's slowest chunk.  Photons are read through the L1/L2 caches.
-    // The units are split into a.queues contiguous ranges with a counter
-    // each; workgroup g draws from queue g % a.queues, i.e. from its own XCD's
-    // counter when a.queues == 8 (workgroups are dispatched to the XCDs
-    // round-robin), so the counters live at 8 addresses instead of one.
+    // from a work counter; its lanes take units from the batch as they need
+    // them.  One loop pass advances every busy lane by one window, and a lane
+    // whose unit ends starts another in the same pass, so lanes do not wait
+    // for the wave's longest unit and waves do not wait for the grid's
+    // slowest batch.  The units are split into a.queues contiguous ranges
+    // with a counter each; workgroup g draws from queue g % a.queues (its own
+    // XCD's counter when a.queues == 8).  Photons are read through L1/L2.
     constexpr bool kFastForm = kPass == kPassFast;
     const BinomTable* tab = &blob->tab;
     const uint32_t queue = blockIdx.x % a.queues;
@@ -245,7 +136,116 @@ __global__ __launch_bounds__(kWave) void walk_kernel(
             }
         }
     }
-    finish<kPass>(s_hist, a, obs, hist_acc, ticket, replay_count, work, result, &s_last);
+    __syncthreads();
+    int32_t* row = hist_acc + size_t(blockIdx.x & 7u) * kMaxBins;
+    for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
+        const int32_t v = s_hist[i];
+        if (v) atomicAdd(&row[i], v);
+    }
+}
+
+constexpr uint32_t kFinishWaves = 16;
+constexpr uint32_t kReplayStage = 256;  // photons of a replayed burst staged in LDS
+
+// Finishes a run in one workgroup of 16 waves: replays the units the fast
+// pass abandoned (general form, one unit per wave at a time, lane 0 walking
+// the burst staged in LDS), reduces the 8 accumulator rows (leaving them zero
+// for the next run), computes the reduced chi2 of
+// src/streamWorker.cpp:194-206 in the reference's float accumulation and in
+// FP64, and resets the run's counters.
+__global__ __launch_bounds__(kWave * kFinishWaves) void finish_kernel(
+    const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
+    const uint2* __restrict__ links, const ModelBlob* __restrict__ blob, const RunArgs a,
+    const float* __restrict__ obs, int32_t* __restrict__ hist_acc,
+    const uint32_t* __restrict__ replay, uint32_t* __restrict__ replay_count,
+    uint32_t* __restrict__ work, ResultHeader* __restrict__ result, float* __restrict__ outE,
+    int32_t* __restrict__ outAcc, float* __restrict__ outF) {
+    __shared__ Model sM;
+    __shared__ int32_t s_hist[kMaxBins];
+    __shared__ uint32_t s_tick[kFinishWaves][kReplayStage];
+    __shared__ uint2 s_link[kFinishWaves][kReplayStage];
+    __shared__ double s_t64[kMaxBins];
+    __shared__ double s_t32[kMaxBins];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid / kWave, lane = tid % kWave;
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&blob->m);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&sM);
+        for (uint32_t i = tid; i < sizeof(Model) / 4; i += blockDim.x) dst[i] = src[i];
+        for (uint32_t i = tid; i < uint32_t(a.B); i += blockDim.x) s_hist[i] = 0;
+    }
+    __syncthreads();
+    const uint32_t count = *replay_count < a.replay_cap ? *replay_count : a.replay_cap;
+    const uint32_t stage = a.cap < kReplayStage ? a.cap : kReplayStage;
+    for (uint32_t i = wave; i < count; i += kFinishWaves) {
+        const uint32_t u = replay[i];
+        const uint32_t bl = u / a.R;
+        const uint32_t b = a.begin + bl;
+        const BurstDesc d = bursts[b];
+        const bool staged = d.P <= stage;
+        if (staged) {
+            for (uint32_t k = lane; k < d.P; k += kWave) {
+                s_tick[wave][k] = ticks[d.off + k];
+                s_link[wave][k] = links[d.off + k];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane == 0) {
+            const UnitKey key{a.seed, a.set, u - bl * a.R, b};
+            const WalkOut o = walk_unit<false>(sM, nullptr, key, d.P, d.stop_rel, d.T,
+                                               staged ? s_tick[wave] : ticks + d.off,
+                                               staged ? s_link[wave] : links + d.off, a);
+            tally(s_hist, a, o, u, outE, outAcc, outF);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    int32_t* rhist = reinterpret_cast<int32_t*>(result + 1);
+    const float Rf = float(a.R);
+    for (uint32_t b = tid; b < uint32_t(a.B); b += blockDim.x) {
+        int32_t Mb = s_hist[b];
+        for (uint32_t x = 0; x < 8u; ++x) Mb += atomicExch(&hist_acc[size_t(x) * kMaxBins + b], 0);
+        s_hist[b] = Mb;
+        rhist[b] = Mb;
+        // per-bin terms: FP64, and the reference's double term of a float
+        // difference (pow(float, 2) / float, src/streamWorker.cpp:200-201)
+        double t64 = 0.0, t32 = 0.0;
+        if (Mb > 0) {
+            const float Mf = float(Mb);
+            const float dlt = obs[b] - Mf / Rf;
+            t32 = (double(dlt) * double(dlt)) / double(Mf / Rf);
+            const double md = double(Mb) / double(a.R);
+            const double dd = double(obs[b]) - md;
+            t64 = dd * dd / md;
+        }
+        s_t64[b] = t64;
+        s_t32[b] = t32;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int effN = a.B - a.left_pad;
+        float c32 = 0.0f;
+        double c64 = 0.0;
+        for (int b = a.left_pad; b < a.B; ++b) {
+            if (s_hist[b] > 0) {
+                c32 = float(double(c32) + s_t32[b]);
+                c64 += s_t64[b];
+            } else {
+                --effN;
+            }
+        }
+        const int dof = effN - a.nstates * (a.nstates + 1);
+        result->chi2_ref = c32 / float(dof);
+        result->chi2_64 = c64 / double(dof);
+        result->effN = effN;
+        result->units = int64_t(a.nunits);
+        result->B = a.B;
+        result->pad = 0;
+        atomicExch(replay_count, 0u);
+    }
+    if (tid < kMaxQueues) atomicExch(&work[tid], 0u);
 }
 
 // Evaluates shared-library primitives for the bit-exactness tests.
@@ -302,41 +302,38 @@ __global__ void primitive_kernel(int op, const void* __restrict__ in, void* __re
 
 // ---- launchers (called by the C-ABI layer) ---------------------------------
 
-// LDS of a walk workgroup: histogram | photon ticks | photon links, the
-// link area doubling as the chi2 term buffer (2 x B doubles).
-size_t walk_smem_bytes(int B, uint32_t cap) {
-    const size_t link_words = std::max(2u * size_t(cap), 4u * size_t(B));
-    return (size_t((B + 1) & ~1) + size_t((cap + 1u) & ~1u) + link_words) * 4u;
-}
-
-hipError_t walk_occupancy(int pass, int B, uint32_t cap, int* blocks_per_cu) {
-    const size_t smem = walk_smem_bytes(B, cap);
-    switch (pass) {
-        case kPassFast:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel<kPassFast>, kWave, smem);
-        case kPassReplay:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel<kPassReplay>, kWave, smem);
-        default:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel<kPassGeneral>, kWave, smem);
-    }
+hipError_t walk_occupancy(int pass, int* blocks_per_cu) {
+    if (pass == kPassFast)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel<kPassFast>,
+                                                            kWave, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel<kPassGeneral>,
+                                                        kWave, 0);
 }
 
 hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
                        const uint2* links, const ModelBlob* blob, const RunArgs& a,
-                       const float* obs, int32_t* hist_acc, uint32_t* ticket, uint32_t* replay,
-                       uint32_t* replay_count, uint32_t* work, ResultHeader* result, float* outE,
-                       int32_t* outAcc, float* outF, hipStream_t stream) {
-    const size_t smem = walk_smem_bytes(a.B, a.cap);
-#define PDA_LAUNCH(P)                                                                       \
-    hipLaunchKernelGGL(walk_kernel<P>, dim3(grid), dim3(kWave), smem, stream, bursts, ticks, \
-                       links, blob, a, obs, hist_acc, ticket, replay, replay_count, work,    \
-                       result, outE, outAcc, outF)
-    switch (pass) {
-        case kPassFast: PDA_LAUNCH(kPassFast); break;
-        case kPassReplay: PDA_LAUNCH(kPassReplay); break;
-        default: PDA_LAUNCH(kPassGeneral); break;
-    }
-#undef PDA_LAUNCH
+                       int32_t* hist_acc, uint32_t* replay, uint32_t* replay_count,
+                       uint32_t* work, float* outE, int32_t* outAcc, float* outF,
+                       hipStream_t stream) {
+    if (pass == kPassFast)
+        hipLaunchKernelGGL(walk_kernel<kPassFast>, dim3(grid), dim3(kWave), 0, stream, bursts,
+                           ticks, links, blob, a, hist_acc, replay, replay_count, work, outE,
+                           outAcc, outF);
+    else
+        hipLaunchKernelGGL(walk_kernel<kPassGeneral>, dim3(grid), dim3(kWave), 0, stream, bursts,
+                           ticks, links, blob, a, hist_acc, replay, replay_count, work, outE,
+                           outAcc, outF);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish(const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
+                         const ModelBlob* blob, const RunArgs& a, const float* obs,
+                         int32_t* hist_acc, const uint32_t* replay, uint32_t* replay_count,
+                         uint32_t* work, ResultHeader* result, float* outE, int32_t* outAcc,
+                         float* outF, hipStream_t stream) {
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kWave * kFinishWaves), 0, stream, bursts,
+                       ticks, links, blob, a, obs, hist_acc, replay, replay_count, work, result,
+                       outE, outAcc, outF);
     return hipGetLastError();
 }
 

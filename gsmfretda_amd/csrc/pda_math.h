@@ -258,9 +258,14 @@ PDA_HD float distance_efficiency(float rd, float r0) {
 }
 
 // ---- binomial -------------------------------------------------------------
-// Split as numpy's legacy generator (src/binom.cuh:172-188): inversion when
-// min(p,1-p) n <= 30, BTPE (Kachitvichyanukul & Schmeiser 1988) otherwise,
-// n - X for p > 1/2.
+// Inversion (src/binom.cuh:135-169) when n < kInversionMaxN or
+// min(p,1-p) n <= 30, BTPE (Kachitvichyanukul & Schmeiser 1988,
+// src/binom.cuh:21-134) otherwise, n - X for p > 1/2.  numpy's legacy split
+// (src/binom.cuh:172-188) switches to BTPE at min(p,1-p) n > 30; both are
+// exact samplers, and below 128 trials q^n >= 2^-127 stays far from
+// underflow, so the cheap inversion covers every count a window of a
+// typical burst produces.
+constexpr int kInversionMaxN = 128;
 
 PDA_HD int64_t binomial_inversion(int n, double p, double U, const UnitKey& key, uint32_t w) {
     const double q = 1.0 - p;
@@ -369,7 +374,7 @@ PDA_HD int64_t binomial(int n, double p, uint32_t u_hi, uint32_t u_lo, const Uni
     if (n <= 0 || !(p >= 0.0 && p <= 1.0)) return 0;
     const bool flip = p > 0.5;
     const double pp = flip ? 1.0 - p : p;
-    const int64_t X = pp * double(n) <= 30.0
+    const int64_t X = (n < kInversionMaxN || pp * double(n) <= 30.0)
                           ? binomial_inversion(n, pp, uniform53(u_hi, u_lo), key, w)
                           : binomial_btpe(n, pp, key, w);
     return flip ? int64_t(n) - X : X;

@@ -79,7 +79,7 @@ struct RunArgs {
     uint32_t queues;     // work counters in use, 1..kMaxQueues
     int32_t B, left_pad, bg_on;
     float clk_p, bg_ad, bg_dd;
-    uint32_t cap;        // LDS photon capacity of a tile
+    uint32_t cap;        // LDS photon capacity for a replayed burst
     int32_t nstates;     // for the chi2 degrees of freedom
     uint32_t replay_cap; // capacity of the replay list
 };

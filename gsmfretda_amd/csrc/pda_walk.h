@@ -51,7 +51,7 @@ PDA_HD int64_t binomial_table(int n, int s, const Model& M, const BinomTable& ta
                               uint32_t lo) {
     if (n <= 0) return 0;
     const double pp = M.pp[s];
-    if (!(pp * double(n) <= 30.0) || n >= kBinomTable) return -1;
+    if (n >= kBinomTable || !(n < kInversionMaxN || pp * double(n) <= 30.0)) return -1;
     const double q = M.qq[s];
     double px = tab.qn[s][n];
     const int64_t bound = tab.bound[s][n];

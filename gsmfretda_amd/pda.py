@@ -33,7 +33,7 @@ EXPORTS = (
     "pda_stream", "pda_load_bursts", "pda_load_bursts_tagged", "pda_num_bursts",
     "pda_set_observed", "pda_set_observed_hist", "pda_set_params", "pda_set_photophysics",
     "pda_set_burst_range", "pda_run", "pda_query", "pda_synchronize", "pda_get_result",
-    "pda_get_samples", "pda_samples_host", "pda_last_run_ms", "pda_set_tuning",
+    "pda_get_samples", "pda_samples_host", "pda_last_run_ms", "pda_last_pass_ms", "pda_set_tuning",
     "pda_get_tuning", "pda_debug_primitive",
 )
 
@@ -90,6 +90,7 @@ def load_library(path=LIB_PATH):
     L.pda_samples_host.argtypes = [_vp, ctypes.c_int]
     L.pda_samples_host.restype = _f32p
     L.pda_last_run_ms.argtypes = [_vp, ctypes.c_int, _f32p]
+    L.pda_last_pass_ms.argtypes = [_vp, ctypes.c_int, _f32p, _f32p]
     L.pda_set_tuning.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
     L.pda_get_tuning.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.pda_debug_primitive.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int64]
@@ -294,6 +295,11 @@ class MC:
         ms = ctypes.c_float()
         _check(self._L.pda_last_run_ms(self._h, sid, ctypes.byref(ms)))
         return ms.value
+
+    def last_pass_ms(self, sid):
+        a, b = ctypes.c_float(), ctypes.c_float()
+        _check(self._L.pda_last_pass_ms(self._h, sid, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def set_tuning(self, tiles_per_chunk, lds_photons):
         _check(self._L.pda_set_tuning(self._h, tiles_per_chunk, lds_photons))

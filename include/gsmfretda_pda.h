@@ -134,13 +134,17 @@ float *pda_samples_host(pda_ctx *ctx, int sid);
 
 /* Milliseconds spent by the slot's last run on the GPU (HIP events). */
 int pda_last_run_ms(pda_ctx *ctx, int sid, float *ms);
+/* The same split into the walk pass over every unit and the finishing pass
+ * (replayed units, histogram reduction, chi2). */
+int pda_last_pass_ms(pda_ctx *ctx, int sid, float *walk_ms, float *finish_ms);
 
 /* Launch shape of the walk kernels (no reference counterpart; the reference
  * takes its block size from cudaOccupancyMaxPotentialBlockSize,
  * src/mc.cu:309): each one-wave workgroup takes batches of
  * tiles_per_chunk x 64 consecutive (burst, resample) units from a work
- * counter (0 = sized from the run, the default); the replay pass stages a
- * burst in LDS when it has at most lds_photons photons (default 256).
+ * counter (0 = sized from the run, the default); the finishing kernel stages
+ * a replayed burst in LDS when it has at most lds_photons photons (0..256,
+ * default 256).
  * Results do not depend on either. */
 int pda_set_tuning(pda_ctx *ctx, int tiles_per_chunk, int lds_photons);
 int pda_get_tuning(const pda_ctx *ctx, int *tiles_per_chunk, int *lds_photons);

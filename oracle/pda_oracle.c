@@ -213,11 +213,14 @@ int or_categorical(int n, const float *p, float u) {
 }
 
 /* ------------------------------------------------------------------ */
-/* Binomial: numpy-legacy split (src/binom.cuh:172-188): inversion for */
-/* min(p,1-p)*n <= 30 (src/binom.cuh:135-169), else BTPE               */
-/* (Kachitvichyanukul & Schmeiser 1988; src/binom.cuh:21-134).         */
+/* Binomial: inversion (src/binom.cuh:135-169) for n < 128 or          */
+/* min(p,1-p)*n <= 30, else BTPE (Kachitvichyanukul & Schmeiser 1988;  */
+/* src/binom.cuh:21-134).  numpy's split (src/binom.cuh:172-188) uses  */
+/* BTPE whenever min(p,1-p)*n > 30; both are exact samplers, and       */
+/* q^n >= 2^-127 keeps the inversion start far above underflow.        */
 /* ------------------------------------------------------------------ */
 #define OR_MAX_RESTARTS 4096
+#define OR_INV_MAX_N 128
 
 static int64_t binom_inversion(int n, double p, double U0, uint32_t seed, uint32_t set,
                                uint32_t burst, uint32_t r, uint32_t w, or_stats *st) {
@@ -338,7 +341,7 @@ int64_t or_binomial(int n, double p, uint32_t seed, uint32_t set, uint32_t burst
     int flip = p > 0.5;
     double pp = flip ? 1.0 - p : p;
     int64_t X;
-    if (pp * (double)n <= 30.0)
+    if (n < OR_INV_MAX_N || pp * (double)n <= 30.0)
         X = binom_inversion(n, pp, or_u53(a2, a3), seed, set, burst, r, w, st);
     else
         X = binom_btpe(n, pp, seed, set, burst, r, w, st);

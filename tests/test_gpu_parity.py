@@ -232,7 +232,7 @@ def test_launch_shape_does_not_change_results(gpu_mc, oracle):
     o = _oracle_run(oracle, d, cfg["n"], cfg["args"], cfg["photophysics"], 13, 21)
     saved = gpu_mc.get_tuning()
     try:
-        for tpc, cap in ((1, 0), (3, 100), (16, 2048), (64, 8192), (0, 256)):
+        for tpc, cap in ((1, 0), (3, 100), (16, 256), (64, 17), (0, 256)):
             gpu_mc.set_tuning(tpc, cap)
             g = _gpu_run(gpu_mc, cfg["n"], cfg["args"], 13, 21, ph=cfg["photophysics"])
             _compare_units(g, o)

@@ -156,7 +156,7 @@ def _gof_discrete(x, pmf, support):
     return stats.chisquare(obs_k, exp_k * obs_k.sum() / exp_k.sum()).pvalue
 
 
-@pytest.mark.parametrize("n,p", [(12, 0.3), (40, 0.75), (200, 0.1), (500, 0.3), (3000, 0.5),
+@pytest.mark.parametrize("n,p", [(12, 0.3), (40, 0.75), (100, 0.5), (127, 0.62), (200, 0.1), (500, 0.3), (3000, 0.5),
                                  (2000, 0.93)])
 def test_binomial_goodness_of_fit(oracle, n, p):
     x = _binom_samples(oracle, n, p, 20_000, 6 + n)
