@@ -238,3 +238,45 @@ def test_launch_shape_does_not_change_results(gpu_mc, oracle):
             _compare_units(g, o)
     finally:
         gpu_mc.set_tuning(*saved)
+
+
+def test_mc_shim_dropin(oracle, tmp_path):
+    """The reference's call sequence through the C++ `class mc` shim
+    (tests/host/mc_dropin.cpp): hmcE bit-exact with the oracle, and the
+    device histogram / chi2 equal to the caller's own binning of hmcE."""
+    import os
+    import subprocess
+    from tests.conftest import ROOT
+    cfg = synth.config(1, n_bursts=1200)
+    d = cfg["data"]
+    obs = _oracle_run(oracle, d, cfg["n"], cfg["args"], (0.34, 0.08, 0.07, 52.0), 1, 5)["E"]
+    inp = tmp_path / "in"
+    inp.mkdir()
+    arrays = dict(start=d.start, stop=d.stop, istart=d.istart, phcount=d.phcount,
+                  burst_ad=d.burst_ad(), burst_dd=d.burst_dd(), T=d.T, SgDivSr=obs,
+                  scalars=np.array([d.clk_p, d.bg_ad, d.bg_dd], np.float32), args=cfg["args"])
+    for k, v in arrays.items():
+        np.ascontiguousarray(v).tofile(str(inp / k))
+    exe = str(tmp_path / "mc_dropin")
+    lib_dir = os.path.join(ROOT, "gsmfretda_amd")
+    subprocess.run(["hipcc", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "host", "mc_dropin.cpp"), "-L", lib_dir,
+                    "-lgsmfretda_pda", f"-Wl,-rpath,{lib_dir}", "-o", exe],
+                   check=True, capture_output=True, timeout=300)
+    R, seed, sid, begin, end = 16, 777, 1, 100, 1100
+    out = str(tmp_path / "out.bin")
+    subprocess.run([exe, str(inp), out, str(cfg["n"]), str(R), str(seed), str(begin), str(end),
+                    str(sid)], check=True, capture_output=True, timeout=120)
+    raw = open(out, "rb").read()
+    nE = (end - begin) * R
+    E = np.frombuffer(raw[:4 * nE], np.float32)
+    hist = np.frombuffer(raw[4 * nE:4 * nE + 800], np.int32)
+    c64, = np.frombuffer(raw[4 * nE + 800:4 * nE + 808], np.float64)
+    c32, chisqr = np.frombuffer(raw[4 * nE + 808:], np.float32)
+    # common random numbers: set id = the stream id
+    o = _oracle_run(oracle, d, cfg["n"], cfg["args"], (0.34, 0.08, 0.07, 52.0), R, seed,
+                    set_id=sid, b0=begin, b1=end, B=200)
+    assert np.array_equal(bits(E), bits(o["E"]))
+    assert np.array_equal(hist.astype(np.int64), o["hist"])
+    assert c32 == chisqr                      # same float accumulation as streamWorker
+    assert abs(c64 - chisqr) <= 1e-5 * abs(c64)
