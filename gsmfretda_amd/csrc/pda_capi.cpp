@@ -15,30 +15,16 @@
 #include <vector>
 
 #include "../../include/gsmfretda_pda.h"
+#include "pda_ctx.h"
 #include "pda_layout.h"
 #include "pda_math.h"
 #include "pda_types.h"
 
 namespace pda {
-hipError_t walk_occupancy(int pass, int* blocks_per_cu);
-hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
-                       const uint2* links, const ModelBlob* blob, const RunArgs& a,
-                       int32_t* hist_acc, uint32_t* replay, uint32_t* replay_count,
-                       uint32_t* work, float* outE, int32_t* outAcc, float* outF,
-                       hipStream_t stream);
-hipError_t launch_finish(const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
-                         const ModelBlob* blob, const RunArgs& a, const float* obs,
-                         int32_t* hist_acc, const uint32_t* replay, uint32_t* replay_count,
-                         uint32_t* work, ResultHeader* result, float* outE, int32_t* outAcc,
-                         float* outF, hipStream_t stream);
-hipError_t launch_primitive(int op, const void* in, void* out, int64_t n, hipStream_t stream);
-}  // namespace pda
-
-using namespace pda;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -50,66 +36,8 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                      \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return fail(PDA_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
-                        __LINE__);                                                         \
-    } while (0)
-
-constexpr uint32_t kDefaultCap = 256;            // LDS photon capacity of a chunk
-constexpr uint32_t kDefaultTilesPerChunk = 0;    // 64-unit tiles per work batch (0 = auto)
-
-struct Slot {
-    hipStream_t stream = nullptr;
-    ModelBlob* h_blob = nullptr;  // pinned staging of the model tables
-    ModelBlob* d_blob = nullptr;
-    bool any_v = false;           // broadening: the general pass walks every unit
-    uint32_t* d_replay = nullptr;  // units the fast pass abandoned
-    uint32_t* d_replay_count = nullptr;
-    uint32_t* d_work = nullptr;    // work counters of the run (one per XCD)
-    size_t replay_cap = 0;
-    hipEvent_t model_copied = nullptr;
-    bool model_pending = false;
-    int n = 0;
-    bool have_params = false;
-    Photophysics ph;
-    std::vector<float> args;
-    int64_t begin = 0, end = -1;
-    int32_t* d_hist_acc = nullptr;
-    ResultHeader* d_result = nullptr;
-    ResultHeader* h_result = nullptr;  // pinned
-    int last_B = 0;
-    int64_t last_units = 0;
-    bool ran = false;
-    float* d_E = nullptr;
-    int32_t* d_acc = nullptr;
-    float* d_F = nullptr;
-    float* h_E = nullptr;  // pinned (the reference's hmcE[sid])
-    size_t sample_cap = 0;
-    bool samples_valid = false;
-    hipEvent_t ev0 = nullptr, ev_mid = nullptr, ev1 = nullptr;
-};
-
-}  // namespace
-
-struct pda_ctx {
-    int device = 0;
-    int num_cu = 256;
-    std::vector<Slot> slots;
-    int64_t nbursts = 0, nphotons = 0;
-    BurstDesc* d_bursts = nullptr;
-    uint32_t* d_ticks = nullptr;
-    uint2* d_links = nullptr;
-    float clk_p = 0.f, bg_ad = 0.f, bg_dd = 0.f;
-    int B = 0, left_pad = 2;
-    float* d_obs = nullptr;
-    uint32_t cap = kDefaultCap;
-    uint32_t tiles_per_chunk = kDefaultTilesPerChunk;
-};
-
-namespace {
+void set_error(const std::string& msg) { g_err = msg; }
+const std::string& last_error() { return g_err; }
 
 Slot* slot_of(pda_ctx* ctx, int sid) {
     if (!ctx || sid < 0 || sid >= int(ctx->slots.size())) return nullptr;
@@ -121,6 +49,48 @@ int sync_all(pda_ctx* ctx) {
     for (auto& s : ctx->slots) HIP_TRY(hipStreamSynchronize(s.stream));
     return PDA_OK;
 }
+
+void range_of(const pda_ctx* ctx, const Slot& s, int64_t* b0, int64_t* b1) {
+    int64_t e = s.end;
+    if (e < 0 || e > ctx->nbursts) e = ctx->nbursts;
+    int64_t b = s.begin;
+    if (b < 0 || b >= e) b = e;  // empty range
+    *b0 = b;
+    *b1 = e;
+}
+
+// Persistent grid of one-wave workgroups, as many as fit on the device (or
+// the tuned number per CU); batches of units per work-counter grab: the tuned
+// size, or about two grabs per wave (a grab is a round trip to the counter
+// that stalls the whole wave; measured slower below ~512 units on 1e7-unit
+// runs).
+Launch plan_launch(const pda_ctx* ctx, int pass, bool population, uint64_t units) {
+    int per_cu = 0;
+    hipError_t e = population ? pop_walk_occupancy(pass, ctx->B, &per_cu)
+                              : walk_occupancy(pass, &per_cu);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    if (ctx->shape.waves_per_cu > 0 && int(ctx->shape.waves_per_cu) < per_cu)
+        per_cu = int(ctx->shape.waves_per_cu);
+    const uint64_t max_grid = uint64_t(ctx->num_cu) * uint64_t(per_cu);
+    uint64_t batch = uint64_t(kWave) * ctx->shape.batch_tiles;
+    if (ctx->shape.batch_tiles == 0) {
+        batch = (units / (max_grid * 2u) + kWave - 1) / kWave * kWave;
+        batch = std::min<uint64_t>(std::max<uint64_t>(batch, kWave), 4096);
+    }
+    const uint64_t batches = (units + batch - 1) / batch;
+    Launch L;
+    L.grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, batches)));
+    L.batch = uint32_t(batch);
+    const uint32_t q = ctx->shape.queues ? ctx->shape.queues : kMaxQueues;
+    L.queues = std::min<uint32_t>(q, L.grid);
+    return L;
+}
+
+}  // namespace pda
+
+using namespace pda;
+
+namespace {
 
 void free_data(pda_ctx* ctx) {
     (void)hipFree(ctx->d_bursts);
@@ -140,9 +110,13 @@ int load_common(pda_ctx* ctx, const int64_t* start, const int64_t* stop, const i
     std::vector<BurstDesc> bursts;
     std::vector<uint32_t> ticks;
     std::vector<uint2> links;
+    std::string err;
     int code = encode_bursts(start, stop, istart, phcount, nbursts, src, nphotons, T, clk_p, bg_ad,
-                             bg_dd, bursts, ticks, links, g_err);
-    if (code) return code;
+                             bg_dd, bursts, ticks, links, err);
+    if (code) {
+        set_error(err);
+        return code;
+    }
     const int64_t total = int64_t(ticks.size());
     int rc = sync_all(ctx);
     if (rc) return rc;
@@ -174,8 +148,12 @@ int load_common(pda_ctx* ctx, const int64_t* start, const int64_t* stop, const i
 
 int upload_model(Slot& s) {
     Model M;
-    int rc = build_model(s.ph, s.n, s.args.data(), int(s.args.size()), M, g_err);
-    if (rc) return rc;
+    std::string err;
+    int rc = build_model(s.ph, s.n, s.args.data(), int(s.args.size()), M, err);
+    if (rc) {
+        set_error(err);
+        return rc;
+    }
     if (s.model_pending) {
         HIP_TRY(hipEventSynchronize(s.model_copied));
         s.model_pending = false;
@@ -189,20 +167,12 @@ int upload_model(Slot& s) {
     return PDA_OK;
 }
 
-void range_of(const pda_ctx* ctx, const Slot& s, int64_t* b0, int64_t* b1) {
-    int64_t e = s.end;
-    if (e < 0 || e > ctx->nbursts) e = ctx->nbursts;
-    int64_t b = s.begin;
-    if (b < 0 || b >= e) b = e;  // empty range
-    *b0 = b;
-    *b1 = e;
-}
 
 }  // namespace
 
 extern "C" {
 
-const char* pda_last_error(void) { return g_err.c_str(); }
+const char* pda_last_error(void) { return last_error().c_str(); }
 
 const char* pda_version(void) { return "gsmfretda_amd 0.1 (gfx950)"; }
 
@@ -273,6 +243,7 @@ int pda_destroy(pda_ctx* ctx) {
         if (s.ev0) (void)hipEventDestroy(s.ev0);
         if (s.ev_mid) (void)hipEventDestroy(s.ev_mid);
         if (s.ev1) (void)hipEventDestroy(s.ev1);
+        free_population(s.pop);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
     free_data(ctx);
@@ -343,8 +314,12 @@ int pda_set_params(pda_ctx* ctx, int sid, int n, const float* args, int nargs) {
     if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
     HIP_TRY(hipSetDevice(ctx->device));
     Model M;
-    int rc = build_model(s->ph, n, args, nargs, M, g_err);  // validate before keeping
-    if (rc) return rc;
+    std::string err;
+    int rc = build_model(s->ph, n, args, nargs, M, err);  // validate before keeping
+    if (rc) {
+        set_error(err);
+        return rc;
+    }
     s->n = n;
     s->args.assign(args, args + nargs);
     s->have_params = true;
@@ -406,23 +381,12 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
     a.cap = ctx->cap;
     a.nstates = s->n;
     const int pass = s->any_v ? 2 : 0;  // general pass for broadening, else fast + replay
-    int per_cu = 0;
-    HIP_TRY(walk_occupancy(pass, &per_cu));
-    if (per_cu < 1) per_cu = 1;
-    // persistent grid of one-wave workgroups, as many as fit on the device;
-    // batches of units per work-counter grab: the tuned size, or about two
-    // grabs per wave (a grab is a round trip to the counter that stalls the
-    // whole wave; measured slower below ~512 units on 1e7-unit runs)
-    const uint64_t max_grid = uint64_t(ctx->num_cu) * uint64_t(per_cu);
-    uint64_t batch = uint64_t(kWave) * ctx->tiles_per_chunk;
-    if (ctx->tiles_per_chunk == 0) {
-        batch = (uint64_t(nunits) / (max_grid * 2u) + kWave - 1) / kWave * kWave;
-        batch = std::min<uint64_t>(std::max<uint64_t>(batch, kWave), 4096);
-    }
-    const uint64_t batches = (uint64_t(nunits) + batch - 1) / batch;
-    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, batches)));
-    a.batch = uint32_t(batch);
-    a.queues = std::min<uint32_t>(kMaxQueues, grid);
+    const Launch L = plan_launch(ctx, pass, false, uint64_t(nunits));
+    const uint32_t grid = L.grid;
+    a.batch = L.batch;
+    a.queues = L.queues;
+    a.nsets = 1;
+    a.bps = 0;
 
     float* outE = nullptr;
     int32_t* outAcc = nullptr;
@@ -567,14 +531,14 @@ int pda_set_tuning(pda_ctx* ctx, int tiles_per_chunk, int lds_photons) {
         return fail(PDA_E_ARG, "lds_photons %d outside [0, 256]", lds_photons);
     int rc = sync_all(ctx);
     if (rc) return rc;
-    ctx->tiles_per_chunk = uint32_t(tiles_per_chunk);
+    ctx->shape.batch_tiles = uint32_t(tiles_per_chunk);
     ctx->cap = uint32_t(lds_photons);
     return PDA_OK;
 }
 
 int pda_get_tuning(const pda_ctx* ctx, int* tiles_per_chunk, int* lds_photons) {
     if (!ctx) return fail(PDA_E_ARG, "null context");
-    if (tiles_per_chunk) *tiles_per_chunk = int(ctx->tiles_per_chunk);
+    if (tiles_per_chunk) *tiles_per_chunk = int(ctx->shape.batch_tiles);
     if (lds_photons) *lds_photons = int(ctx->cap);
     return PDA_OK;
 }

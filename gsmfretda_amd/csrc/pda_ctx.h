@@ -1,0 +1,174 @@
+// pda_ctx.h -- host-side state of a pda_ctx (internal to the library).
+//
+// A context owns the resident bursts and a set of stream slots; a slot is the
+// MI355X counterpart of one of the reference's CUDA streams with its
+// per-stream buffers (src/mc.hpp:32-180, src/mc.cu:523-565).  Besides the
+// single-set run of mc::run_kernel, a slot can hold a population of parameter
+// sets evaluated by one graph-captured launch sequence.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gsmfretda_pda.h"
+#include "pda_layout.h"
+#include "pda_types.h"
+
+namespace pda {
+
+int fail(int code, const char* fmt, ...);
+void set_error(const std::string& msg);
+const std::string& last_error();
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::pda::fail(PDA_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                               __FILE__, __LINE__);                                        \
+    } while (0)
+
+// Launch shape of the walk kernels; 0 = chosen per run.
+struct LaunchShape {
+    uint32_t batch_tiles = 0;   // 64-unit tiles per work-counter grab
+    uint32_t waves_per_cu = 0;  // resident one-wave workgroups per CU (grid size)
+    uint32_t queues = 0;        // work counters (1..8)
+};
+
+// Key of a captured population graph: everything baked into its nodes.
+struct PopGraphKey {
+    int nsets = 0, R = 0, B = 0, left_pad = 0, pass = 0, bg_on = 0;
+    int64_t b0 = 0, b1 = 0;
+    uint32_t grid = 0, batch = 0, queues = 0, replay_cap = 0;
+    float clk_p = 0.f, bg_ad = 0.f, bg_dd = 0.f;
+    const void* data = nullptr;
+    const void* obs = nullptr;
+    bool operator==(const PopGraphKey& o) const;
+};
+
+struct Population {
+    int nsets = 0;
+    std::vector<int> n;              // states per set
+    std::vector<float> args;         // packed args of every set
+    std::vector<int64_t> off;        // start of set s in args
+    bool any_v = false;
+    size_t cap_sets = 0;             // allocated sets
+    uint8_t* h_in = nullptr;         // pinned: PopControl + ModelBlob[cap_sets]
+    uint8_t* d_in = nullptr;
+    int cap_B = 0;
+    int32_t* d_hist_acc = nullptr;   // [cap_sets][8][cap_B], zero between runs
+    uint8_t* d_out = nullptr;        // PopStatus + SetResult[cap_sets] + int32 hist[cap_sets][B]
+    uint8_t* h_out = nullptr;        // pinned
+    size_t out_bytes = 0;
+    uint64_t* d_replay = nullptr;
+    uint32_t* d_replay_count = nullptr;
+    size_t replay_cap = 0;
+    uint32_t* d_work = nullptr;      // kMaxQueues counters
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    PopGraphKey key;
+    int captures = 0;
+    bool ran = false;
+    bool in_flight = false;
+    int last_B = 0, last_R = 0;
+    int64_t last_units = 0;          // per set
+    uint32_t last_seed = 0, last_base = 0, last_stride = 0;
+    bool last_graph = false;
+    hipEvent_t ev0 = nullptr, ev_mid = nullptr, ev1 = nullptr;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    ModelBlob* h_blob = nullptr;  // pinned staging of the model tables
+    ModelBlob* d_blob = nullptr;
+    bool any_v = false;           // broadening: the general pass walks every unit
+    uint32_t* d_replay = nullptr;  // units the fast pass abandoned
+    uint32_t* d_replay_count = nullptr;
+    uint32_t* d_work = nullptr;    // work counters of the run (one per XCD)
+    size_t replay_cap = 0;
+    hipEvent_t model_copied = nullptr;
+    bool model_pending = false;
+    int n = 0;
+    bool have_params = false;
+    Photophysics ph;
+    std::vector<float> args;
+    int64_t begin = 0, end = -1;
+    int32_t* d_hist_acc = nullptr;
+    ResultHeader* d_result = nullptr;
+    ResultHeader* h_result = nullptr;  // pinned
+    int last_B = 0;
+    int64_t last_units = 0;
+    bool ran = false;
+    float* d_E = nullptr;
+    int32_t* d_acc = nullptr;
+    float* d_F = nullptr;
+    float* h_E = nullptr;  // pinned (the reference's hmcE[sid])
+    size_t sample_cap = 0;
+    bool samples_valid = false;
+    hipEvent_t ev0 = nullptr, ev_mid = nullptr, ev1 = nullptr;
+    Population pop;
+};
+
+constexpr uint32_t kDefaultCap = 256;  // LDS photon capacity of a replayed burst
+
+}  // namespace pda
+
+struct pda_ctx {
+    int device = 0;
+    int num_cu = 256;
+    std::vector<pda::Slot> slots;
+    int64_t nbursts = 0, nphotons = 0;
+    pda::BurstDesc* d_bursts = nullptr;
+    uint32_t* d_ticks = nullptr;
+    uint2* d_links = nullptr;
+    float clk_p = 0.f, bg_ad = 0.f, bg_dd = 0.f;
+    int B = 0, left_pad = 2;
+    float* d_obs = nullptr;
+    uint32_t cap = pda::kDefaultCap;
+    pda::LaunchShape shape;
+};
+
+namespace pda {
+
+Slot* slot_of(pda_ctx* ctx, int sid);
+int sync_all(pda_ctx* ctx);
+void range_of(const pda_ctx* ctx, const Slot& s, int64_t* b0, int64_t* b1);
+void free_population(Population& p);
+
+// Grid and batch of a walk launch over `units` work units.
+struct Launch {
+    uint32_t grid = 1, batch = kWave, queues = 1;
+};
+Launch plan_launch(const pda_ctx* ctx, int pass, bool population, uint64_t units);
+
+// kernels (pda_kernels.hip)
+hipError_t walk_occupancy(int pass, int* blocks_per_cu);
+hipError_t pop_walk_occupancy(int pass, int B, int* blocks_per_cu);
+hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
+                       const uint2* links, const ModelBlob* blob, const RunArgs& a,
+                       int32_t* hist_acc, uint32_t* replay, uint32_t* replay_count,
+                       uint32_t* work, float* outE, int32_t* outAcc, float* outF,
+                       hipStream_t stream);
+hipError_t launch_finish(const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
+                         const ModelBlob* blob, const RunArgs& a, const float* obs,
+                         int32_t* hist_acc, const uint32_t* replay, uint32_t* replay_count,
+                         uint32_t* work, ResultHeader* result, float* outE, int32_t* outAcc,
+                         float* outF, hipStream_t stream);
+hipError_t launch_pop_walk(int pass, uint32_t grid, const BurstDesc* bursts,
+                           const uint32_t* ticks, const uint2* links, const PopControl* ctl,
+                           const ModelBlob* blobs, const RunArgs& a, int32_t* hist_acc,
+                           uint64_t* replay, uint32_t* replay_count, uint32_t* work,
+                           hipStream_t stream);
+hipError_t launch_pop_replay(const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
+                             const PopControl* ctl, const ModelBlob* blobs, const RunArgs& a,
+                             int32_t* hist_acc, const uint64_t* replay,
+                             const uint32_t* replay_count, uint32_t grid, hipStream_t stream);
+hipError_t launch_pop_finish(const ModelBlob* blobs, const RunArgs& a, const float* obs,
+                             int32_t* hist_acc, uint32_t* replay_count, uint32_t* work,
+                             PopStatus* status, SetResult* results, int32_t* hists,
+                             hipStream_t stream);
+hipError_t launch_primitive(int op, const void* in, void* out, int64_t n, hipStream_t stream);
+
+}  // namespace pda

@@ -248,6 +248,243 @@ __global__ __launch_bounds__(kWave * kFinishWaves) void finish_kernel(
     if (tid < kMaxQueues) atomicExch(&work[tid], 0u);
 }
 
+// ---- population runs ---------------------------------------------------------
+//
+// A population of parameter sets (an optimiser generation) over the same
+// resident bursts in one launch sequence: pop_walk_kernel walks every
+// (set, burst, resample) unit, pop_replay_kernel recomputes the units the
+// fast form abandoned, pop_finish_kernel computes each set's histogram and
+// chi2.  The reference evaluates one set per stream and per mc::run_kernel
+// call (src/gpuWorker.cpp:71-80); the units and their random streams are the
+// same as a single-set run with set id set_base + s * set_stride.
+
+__device__ __forceinline__ int e_bin(float acc, float F, int B) {
+    const float E = F > 0.0f ? acc / F : 0.0f;
+    if (!(E >= 0.0f && E < 1.0f)) return -1;
+    const int bin = int(floor(double(E) * double(B)));
+    return bin < B ? bin : -1;
+}
+
+// Adds an LDS histogram to its set's accumulator row of this XCD and clears it.
+__device__ __forceinline__ void flush_hist(int32_t* h, int32_t* __restrict__ hist_acc,
+                                           uint32_t set, uint32_t xrow, int B, uint32_t lane) {
+    int32_t* row = hist_acc + (size_t(set) * 8u + xrow) * size_t(B);
+    for (uint32_t i = lane; i < uint32_t(B); i += kWave) {
+        const int32_t v = h[i];
+        if (v) {
+            atomicAdd(&row[i], v);
+            h[i] = 0;
+        }
+    }
+}
+
+// Work is a flat list of batches: set s owns batches [s * bps, (s + 1) * bps),
+// each covering up to a.batch consecutive units of that set; the list is
+// split into a.queues contiguous ranges with a counter each, as in
+// walk_kernel.  A wave's lanes may be walking units of two sets at once: the
+// workgroup keeps two LDS model slots with an LDS histogram each, and a wave
+// moves to a third set only after the older slot's lanes have finished, so it
+// never drains to change sets.
+template <int kPass>
+__global__ __launch_bounds__(kWave) void pop_walk_kernel(
+    const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
+    const uint2* __restrict__ links, const PopControl* __restrict__ ctl,
+    const ModelBlob* __restrict__ blobs, const RunArgs a, int32_t* __restrict__ hist_acc,
+    uint64_t* __restrict__ replay, uint32_t* __restrict__ replay_count,
+    uint32_t* __restrict__ work) {
+    __shared__ Model sM[2];
+    extern __shared__ int32_t s_hist[];  // [2][a.B]
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t i = lane; i < 2u * uint32_t(a.B); i += kWave) s_hist[i] = 0;
+    __syncthreads();
+
+    constexpr bool kFastForm = kPass == kPassFast;
+    const uint32_t seed = ctl->seed, set_base = ctl->set_base, set_stride = ctl->set_stride;
+    const uint32_t queue = blockIdx.x % a.queues;
+    const uint32_t nb = a.nsets * a.bps;
+    const uint32_t q_begin = uint32_t(uint64_t(nb) * queue / a.queues);
+    const uint32_t q_end = uint32_t(uint64_t(nb) * (queue + 1u) / a.queues);
+    const uint32_t xrow = blockIdx.x & 7u;
+    int set0 = -1, set1 = -1;    // wave-uniform: the set each LDS slot holds
+    uint32_t cur = 0;            // slot that receives new units
+    uint32_t pool_set = 0, pool_next = 0, pool_end = 0;
+    bool busy = false;
+    uint32_t my_slot = 0, my_set = 0, u = 0;
+    WalkState<kFastForm> st{};
+    UnitKey key{seed, 0u, 0u, 0u};
+    BurstDesc d{0u, 0u, 0u, 0.0f};
+    bool drained = false;
+    while (true) {
+        bool need = !busy;
+        while (!drained) {
+            const uint64_t want = __ballot(need);
+            if (want == 0) break;
+            if (pool_next >= pool_end) {
+                uint32_t k = 0;
+                if (lane == 0) k = atomicAdd(&work[queue], 1u);
+                k = __shfl(k, 0);
+                if (k >= q_end - q_begin) {
+                    drained = true;
+                    break;
+                }
+                k += q_begin;
+                pool_set = k / a.bps;
+                pool_next = (k - pool_set * a.bps) * a.batch;
+                pool_end = pool_next + a.batch < a.nunits ? pool_next + a.batch : a.nunits;
+            }
+            if (int(pool_set) != (cur ? set1 : set0)) {
+                const uint32_t other = cur ^ 1u;
+                if (__any(busy && my_slot == other)) break;  // the older set is still walking
+                const int old = other ? set1 : set0;
+                if (old >= 0)
+                    flush_hist(s_hist + other * uint32_t(a.B), hist_acc, uint32_t(old), xrow,
+                               a.B, lane);
+                const uint32_t* src = reinterpret_cast<const uint32_t*>(&blobs[pool_set].m);
+                uint32_t* dst = reinterpret_cast<uint32_t*>(&sM[other]);
+                for (uint32_t i = lane; i < sizeof(Model) / 4; i += kWave) dst[i] = src[i];
+                __syncthreads();
+                if (other) set1 = int(pool_set);
+                else set0 = int(pool_set);
+                cur = other;
+            }
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                uint32_t(want >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(want), 0u));
+            const uint32_t avail = pool_end - pool_next;
+            if (need && rank < avail) {
+                u = pool_next + rank;
+                my_slot = cur;
+                my_set = pool_set;
+                const uint32_t bl = u / a.R;
+                const uint32_t b = a.begin + bl;
+                d = bursts[b];
+                key.set = set_base + my_set * set_stride;
+                key.r = u - bl * a.R;
+                key.b = b;
+                if (begin_unit<kFastForm>(sM[my_slot], key, d.P, d.T, st) == kStepMore) {
+                    busy = true;
+                    need = false;
+                } else {
+                    const int bin = e_bin(0.0f, 0.0f, a.B);
+                    if (bin >= 0) atomicAdd(&s_hist[my_slot * uint32_t(a.B) + uint32_t(bin)], 1);
+                }
+            }
+            const uint32_t taken = uint32_t(__popcll(want));
+            pool_next += taken < avail ? taken : avail;
+        }
+        if (!__any(busy)) break;
+        if (busy) {
+            const int r = step_window<kFastForm>(sM[my_slot], &blobs[my_set].tab, key, d.P,
+                                                 d.stop_rel, d.T, ticks + d.off, links + d.off,
+                                                 a, st);
+            if (r != kStepMore) {
+                if (r == kStepReplay) {
+                    const uint32_t slot = atomicAdd(replay_count, 1u);
+                    if (slot < a.replay_cap) replay[slot] = (uint64_t(my_set) << 32) | u;
+                } else {
+                    const int bin = e_bin(st.acc, st.F, a.B);
+                    if (bin >= 0) atomicAdd(&s_hist[my_slot * uint32_t(a.B) + uint32_t(bin)], 1);
+                }
+                busy = false;
+            }
+        }
+    }
+    __syncthreads();
+    if (set0 >= 0) flush_hist(s_hist, hist_acc, uint32_t(set0), xrow, a.B, lane);
+    if (set1 >= 0) flush_hist(s_hist + uint32_t(a.B), hist_acc, uint32_t(set1), xrow, a.B, lane);
+}
+
+// Recomputes abandoned (set, unit) pairs in the general form, one per lane,
+// and adds them to row 0 of the set's accumulator.
+__global__ __launch_bounds__(kWave) void pop_replay_kernel(
+    const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
+    const uint2* __restrict__ links, const PopControl* __restrict__ ctl,
+    const ModelBlob* __restrict__ blobs, const RunArgs a, int32_t* __restrict__ hist_acc,
+    const uint64_t* __restrict__ replay, const uint32_t* __restrict__ replay_count) {
+    const uint32_t count = *replay_count < a.replay_cap ? *replay_count : a.replay_cap;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint64_t e = replay[i];
+        const uint32_t set = uint32_t(e >> 32), u = uint32_t(e);
+        const uint32_t bl = u / a.R;
+        const uint32_t b = a.begin + bl;
+        const BurstDesc d = bursts[b];
+        const UnitKey key{ctl->seed, ctl->set_base + set * ctl->set_stride, u - bl * a.R, b};
+        const WalkOut o = walk_unit<false>(blobs[set].m, nullptr, key, d.P, d.stop_rel, d.T,
+                                           ticks + d.off, links + d.off, a);
+        const int bin = e_bin(o.acc, o.F, a.B);
+        if (bin >= 0) atomicAdd(&hist_acc[size_t(set) * 8u * size_t(a.B) + uint32_t(bin)], 1);
+    }
+}
+
+// One workgroup per set: reduces the set's 8 accumulator rows (leaving them
+// zero), computes its chi2 as finish_kernel does, with the set's own degrees
+// of freedom; workgroup 0 reports and resets the replay count and the work
+// counters.
+__global__ __launch_bounds__(kWave) void pop_finish_kernel(
+    const ModelBlob* __restrict__ blobs, const RunArgs a, const float* __restrict__ obs,
+    int32_t* __restrict__ hist_acc, uint32_t* __restrict__ replay_count,
+    uint32_t* __restrict__ work, PopStatus* __restrict__ status, SetResult* __restrict__ results,
+    int32_t* __restrict__ hists) {
+    __shared__ int32_t s_M[kMaxBins];
+    __shared__ double s_t64[kMaxBins];
+    __shared__ double s_t32[kMaxBins];
+    const uint32_t set = blockIdx.x, lane = threadIdx.x;
+    const uint32_t B = uint32_t(a.B);
+    int32_t* rows = hist_acc + size_t(set) * 8u * B;
+    const float Rf = float(a.R);
+    for (uint32_t b = lane; b < B; b += kWave) {
+        int32_t Mb = 0;
+        for (uint32_t x = 0; x < 8u; ++x) Mb += atomicExch(&rows[x * B + b], 0);
+        s_M[b] = Mb;
+        hists[size_t(set) * B + b] = Mb;
+        double t64 = 0.0, t32 = 0.0;
+        if (Mb > 0) {
+            const float Mf = float(Mb);
+            const float dlt = obs[b] - Mf / Rf;
+            t32 = (double(dlt) * double(dlt)) / double(Mf / Rf);
+            const double md = double(Mb) / double(a.R);
+            const double dd = double(obs[b]) - md;
+            t64 = dd * dd / md;
+        }
+        s_t64[b] = t64;
+        s_t32[b] = t32;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int effN = a.B - a.left_pad;
+        float c32 = 0.0f;
+        double c64 = 0.0;
+        for (int b = a.left_pad; b < a.B; ++b) {
+            if (s_M[b] > 0) {
+                c32 = float(double(c32) + s_t32[b]);
+                c64 += s_t64[b];
+            } else {
+                --effN;
+            }
+        }
+        const int n = blobs[set].m.n;
+        const int dof = effN - n * (n + 1);
+        SetResult r;
+        r.chi2_ref = c32 / float(dof);
+        r.chi2_64 = c64 / double(dof);
+        r.effN = effN;
+        r.n = n;
+        r.pad = 0;
+        results[set] = r;
+    }
+    if (set == 0) {
+        if (lane == 0) {
+            const uint32_t c = atomicExch(replay_count, 0u);
+            PopStatus st;
+            st.replays = c;
+            st.overflow = c > a.replay_cap ? 1u : 0u;
+            st.pad[0] = st.pad[1] = 0;
+            *status = st;
+        }
+        if (lane < kMaxQueues) atomicExch(&work[lane], 0u);
+    }
+}
+
 // Evaluates shared-library primitives for the bit-exactness tests.
 struct BinomItem {
     int32_t n;
@@ -334,6 +571,50 @@ hipError_t launch_finish(const BurstDesc* bursts, const uint32_t* ticks, const u
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kWave * kFinishWaves), 0, stream, bursts,
                        ticks, links, blob, a, obs, hist_acc, replay, replay_count, work, result,
                        outE, outAcc, outF);
+    return hipGetLastError();
+}
+
+hipError_t pop_walk_occupancy(int pass, int B, int* blocks_per_cu) {
+    const size_t lds = sizeof(int32_t) * 2u * size_t(B);
+    if (pass == kPassFast)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                            pop_walk_kernel<kPassFast>, kWave, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                        pop_walk_kernel<kPassGeneral>, kWave, lds);
+}
+
+hipError_t launch_pop_walk(int pass, uint32_t grid, const BurstDesc* bursts,
+                           const uint32_t* ticks, const uint2* links, const PopControl* ctl,
+                           const ModelBlob* blobs, const RunArgs& a, int32_t* hist_acc,
+                           uint64_t* replay, uint32_t* replay_count, uint32_t* work,
+                           hipStream_t stream) {
+    const size_t lds = sizeof(int32_t) * 2u * size_t(a.B);
+    if (pass == kPassFast)
+        hipLaunchKernelGGL(pop_walk_kernel<kPassFast>, dim3(grid), dim3(kWave), lds, stream,
+                           bursts, ticks, links, ctl, blobs, a, hist_acc, replay, replay_count,
+                           work);
+    else
+        hipLaunchKernelGGL(pop_walk_kernel<kPassGeneral>, dim3(grid), dim3(kWave), lds, stream,
+                           bursts, ticks, links, ctl, blobs, a, hist_acc, replay, replay_count,
+                           work);
+    return hipGetLastError();
+}
+
+hipError_t launch_pop_replay(const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
+                             const PopControl* ctl, const ModelBlob* blobs, const RunArgs& a,
+                             int32_t* hist_acc, const uint64_t* replay,
+                             const uint32_t* replay_count, uint32_t grid, hipStream_t stream) {
+    hipLaunchKernelGGL(pop_replay_kernel, dim3(grid), dim3(kWave), 0, stream, bursts, ticks,
+                       links, ctl, blobs, a, hist_acc, replay, replay_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_pop_finish(const ModelBlob* blobs, const RunArgs& a, const float* obs,
+                             int32_t* hist_acc, uint32_t* replay_count, uint32_t* work,
+                             PopStatus* status, SetResult* results, int32_t* hists,
+                             hipStream_t stream) {
+    hipLaunchKernelGGL(pop_finish_kernel, dim3(a.nsets), dim3(kWave), 0, stream, blobs, a, obs,
+                       hist_acc, replay_count, work, status, results, hists);
     return hipGetLastError();
 }
 

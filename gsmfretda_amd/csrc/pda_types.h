@@ -82,6 +82,8 @@ struct RunArgs {
     uint32_t cap;        // LDS photon capacity for a replayed burst
     int32_t nstates;     // for the chi2 degrees of freedom
     uint32_t replay_cap; // capacity of the replay list
+    uint32_t nsets;      // population runs: parameter sets
+    uint32_t bps;        // population runs: work batches per set
 };
 
 // Device result of one run, followed by int32 hist[B].
@@ -91,6 +93,30 @@ struct ResultHeader {
     int32_t effN;
     int64_t units;
     int32_t B;
+    int32_t pad;
+};
+
+// Population runs: random-stream keys of the sets, uploaded with the models
+// (set s draws with key {seed, set_base + s * set_stride}).
+struct PopControl {
+    uint32_t seed, set_base, set_stride, nsets;
+    uint32_t pad[60];  // the models that follow start 256-byte aligned
+};
+
+// Population runs: what the finish kernel reports besides the sets' results.
+struct PopStatus {
+    uint32_t replays;   // units the fast pass abandoned
+    uint32_t overflow;  // 1: more than the replay list holds (results invalid)
+    uint32_t pad[2];
+};
+
+// Population runs: result of one set (its int32 histogram follows the array
+// of all sets' results).
+struct SetResult {
+    double chi2_64;
+    float chi2_ref;
+    int32_t effN;
+    int32_t n;
     int32_t pad;
 };
 

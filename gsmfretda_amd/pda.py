@@ -24,6 +24,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgsmfretda_pda.so")
 
 PDA_RUN_SAMPLES = 1
+PDA_RUN_NO_GRAPH = 2
 MAX_STATES = 8
 MAX_BINS = 1024
 
@@ -34,7 +35,9 @@ EXPORTS = (
     "pda_set_observed", "pda_set_observed_hist", "pda_set_params", "pda_set_photophysics",
     "pda_set_burst_range", "pda_run", "pda_query", "pda_synchronize", "pda_get_result",
     "pda_get_samples", "pda_samples_host", "pda_last_run_ms", "pda_last_pass_ms", "pda_set_tuning",
-    "pda_get_tuning", "pda_debug_primitive",
+    "pda_get_tuning", "pda_set_launch_shape", "pda_get_launch_shape", "pda_autotune",
+    "pda_set_population", "pda_run_population", "pda_get_population_result",
+    "pda_population_stats", "pda_debug_primitive",
 )
 
 
@@ -94,6 +97,18 @@ def load_library(path=LIB_PATH):
     L.pda_set_tuning.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
     L.pda_get_tuning.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.pda_debug_primitive.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int64]
+    _ip = ctypes.POINTER(ctypes.c_int)
+    L.pda_set_launch_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.pda_get_launch_shape.argtypes = [_vp, _ip, _ip, _ip]
+    L.pda_autotune.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
+                               _ip, _ip, _f32p]
+    L.pda_set_population.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _i32p, _f32p,
+                                     ctypes.c_int64]
+    L.pda_run_population.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+    L.pda_get_population_result.argtypes = [_vp, ctypes.c_int, _i32p, ctypes.c_int, _f64p,
+                                            _f32p, _i32p]
+    L.pda_population_stats.argtypes = [_vp, ctypes.c_int, _i64p, _i32p, _i32p, _f32p]
     _lib = L
     return L
 
@@ -308,6 +323,66 @@ class MC:
         t, c = ctypes.c_int(), ctypes.c_int()
         _check(self._L.pda_get_tuning(self._h, ctypes.byref(t), ctypes.byref(c)))
         return t.value, c.value
+
+    # -- launch shape and autotuner ---------------------------------------------
+    def set_launch_shape(self, batch_tiles=0, waves_per_cu=0, queues=0):
+        _check(self._L.pda_set_launch_shape(self._h, batch_tiles, waves_per_cu, queues))
+
+    def get_launch_shape(self):
+        v = [ctypes.c_int() for _ in range(3)]
+        _check(self._L.pda_get_launch_shape(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def autotune(self, sid=0, population=False, R=None, reps=3):
+        """Picks the fastest launch shape for the slot's current work; returns
+        (batch_tiles, waves_per_cu, queues, best_ms)."""
+        v = [ctypes.c_int() for _ in range(3)]
+        ms = ctypes.c_float()
+        R = self.reSampleTimes if R is None else R
+        _check(self._L.pda_autotune(self._h, sid, int(bool(population)), R, reps,
+                                    *[ctypes.byref(x) for x in v], ctypes.byref(ms)))
+        return tuple(x.value for x in v) + (ms.value,)
+
+    # -- population runs -----------------------------------------------------------
+    def set_population(self, nstates, args_list, sid=0):
+        """A generation of parameter sets: nstates[i] states, args_list[i] the
+        mc::set_params vector [E(n), K off-diagonals n(n-1), v(n)] of set i."""
+        nstates = np.ascontiguousarray(nstates, np.int32)
+        packed = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).ravel()
+                                                      for a in args_list]), np.float32)
+        _check(self._L.pda_set_population(self._h, sid, len(nstates), _p(nstates, _i32p),
+                                          _p(packed, _f32p), packed.size))
+        self._pop_sets = len(nstates)
+
+    def run_population(self, R=None, seed=None, set_base=0, set_stride=1, sid=0, graph=True):
+        """Asynchronous evaluation of the slot's population; set i uses the
+        random-stream key (seed, set_base + i * set_stride)."""
+        R = self.reSampleTimes if R is None else R
+        seed = self.seed if seed is None else seed
+        flags = 0 if graph else PDA_RUN_NO_GRAPH
+        _check(self._L.pda_run_population(self._h, sid, R, seed, set_base, set_stride, flags))
+
+    def get_population_result(self, sid=0, nbins=None):
+        """(hist [nsets, nbins] int32, chi2_64 [nsets], chi2_ref [nsets], effN [nsets])."""
+        B = self._hist_bins if nbins is None else nbins
+        S = self._pop_sets
+        hist = np.zeros((S, B), np.int32)
+        c64 = np.zeros(S, np.float64)
+        c32 = np.zeros(S, np.float32)
+        eff = np.zeros(S, np.int32)
+        _check(self._L.pda_get_population_result(self._h, sid, _p(hist, _i32p), B,
+                                                 _p(c64, _f64p), _p(c32, _f32p),
+                                                 _p(eff, _i32p)))
+        return hist, c64, c32, eff
+
+    def population_stats(self, sid=0):
+        """(units per set, replayed units, graph captures so far, GPU ms of the last run)."""
+        u = ctypes.c_int64()
+        r, c = ctypes.c_int32(), ctypes.c_int32()
+        ms = ctypes.c_float()
+        _check(self._L.pda_population_stats(self._h, sid, ctypes.byref(u), ctypes.byref(r),
+                                            ctypes.byref(c), ctypes.byref(ms)))
+        return u.value, r.value, c.value, ms.value
 
     def debug_primitive(self, op, inp, out_dtype, n):
         inp = np.ascontiguousarray(inp)

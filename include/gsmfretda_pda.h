@@ -30,6 +30,7 @@ extern "C" {
 
 /* pda_run flags */
 #define PDA_RUN_SAMPLES 1 /* also keep per-sample E (and acceptor count, F)  */
+#define PDA_RUN_NO_GRAPH 2 /* population runs: launch without graph capture */
 
 typedef struct pda_ctx pda_ctx;
 
@@ -148,6 +149,59 @@ int pda_last_pass_ms(pda_ctx *ctx, int sid, float *walk_ms, float *finish_ms);
  * Results do not depend on either. */
 int pda_set_tuning(pda_ctx *ctx, int tiles_per_chunk, int lds_photons);
 int pda_get_tuning(const pda_ctx *ctx, int *tiles_per_chunk, int *lds_photons);
+
+/* Launch shape of the walk kernels, single-set and population runs (no
+ * reference counterpart; the reference sizes its grid with
+ * cudaOccupancyMaxPotentialBlockSize, src/mc.cu:309): batch_tiles x 64 units
+ * per work-counter grab (0 = sized from the run), resident one-wave
+ * workgroups per CU (0 = as many as fit), work counters (0 = 8, one per
+ * XCD).  pda_set_tuning's tiles_per_chunk is the same knob as batch_tiles.
+ * Results do not depend on the shape. */
+int pda_set_launch_shape(pda_ctx *ctx, int batch_tiles, int waves_per_cu, int queues);
+int pda_get_launch_shape(const pda_ctx *ctx, int *batch_tiles, int *waves_per_cu, int *queues);
+
+/* Autotuner (no reference counterpart): coordinate search over the launch
+ * shape (batch size, then waves per CU, then work counters) on the slot's
+ * current work -- its parameter set (population = 0) or its population
+ * (population = 1) -- with R resamples, each candidate timed `reps` times
+ * with HIP events after one untimed run; keeps the fastest shape in the
+ * context and returns it (outputs may be NULL).  Overwrites the slot's last
+ * results. */
+int pda_autotune(pda_ctx *ctx, int sid, int population, int R, int reps, int *batch_tiles,
+                 int *waves_per_cu, int *queues, float *best_ms);
+
+/* Population of parameter sets (no reference counterpart: the reference
+ * evaluates one set per stream and mc::run_kernel call, the parameter server
+ * feeding a generation's sets to gpuWorker::run one at a time,
+ * src/gpuWorker.cpp:38-88, src/streamWorker.cpp:83-182).  nstates[i] in
+ * [2, 8] per set; args holds the sets' mc::set_params argument vectors
+ * (src/mc.cu:790-800) back to back, sum of n(n+1) floats.  Validates every
+ * set (PDA_E_ARG naming the set) before replacing the slot's population; the
+ * slot's photophysics constants apply to every set. */
+int pda_set_population(pda_ctx *ctx, int sid, int nsets, const int32_t *nstates,
+                       const float *args, int64_t nargs_total);
+
+/* Evaluates the slot's population asynchronously on its stream: every set
+ * over the slot's burst range with R resamples, set i drawing with the key
+ * (seed, set_base + i * set_stride) -- bit-identical to pda_run of that set
+ * with that set id.  The launch sequence (model upload, walk, replay,
+ * per-set histogram and chi2, result download) is captured into a hipGraph
+ * on first use and replayed while the shape of the work stays the same;
+ * flags & PDA_RUN_NO_GRAPH launches it directly. */
+int pda_run_population(pda_ctx *ctx, int sid, int R, uint32_t seed, uint32_t set_base,
+                       uint32_t set_stride, int flags);
+
+/* Results of the slot's last population run (waits for it): int32
+ * histograms [nsets][nbins], reduced chi2 per set in FP64 and with the
+ * reference's float accumulation, bins kept per set.  Outputs may be NULL. */
+int pda_get_population_result(pda_ctx *ctx, int sid, int32_t *hist, int nbins, double *chi2_64,
+                              float *chi2_ref, int32_t *effN);
+
+/* Facts of the slot's last population run (waits for it): units per set,
+ * units replayed in the general form, graph captures so far, GPU
+ * milliseconds of the whole sequence.  Outputs may be NULL. */
+int pda_population_stats(pda_ctx *ctx, int sid, int64_t *units_per_set, int32_t *replays,
+                         int32_t *graph_captures, float *ms);
 
 /* Diagnostics (no reference counterpart): evaluates one primitive of the
  * shared variate library on the GPU for n inputs, so tests can compare it
