@@ -22,7 +22,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
           f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-command-line-argument", "-I", os.path.join(ROOT, "include")]
 
-SOURCES = ["pda_kernels.hip", "pda_layout.cpp", "pda_capi.cpp", "pda_population.cpp", "mc_shim.cpp"]
+SOURCES = ["pda_kernels.hip", "pda_layout.cpp", "pda_capi.cpp", "pda_population.cpp", "pda_rb_kernels.hip", "pda_rb.cpp",
+           "mc_shim.cpp"]
 
 
 def _run(cmd, verbose):

@@ -136,6 +136,8 @@ int load_common(pda_ctx* ctx, const int64_t* start, const int64_t* stop, const i
     }
     ctx->nbursts = nbursts;
     ctx->nphotons = total;
+    ctx->h_P.resize(size_t(nbursts));
+    for (int64_t i = 0; i < nbursts; ++i) ctx->h_P[size_t(i)] = bursts[size_t(i)].P;
     ctx->clk_p = clk_p;
     ctx->bg_ad = bg_ad;
     ctx->bg_dd = bg_dd;
@@ -189,8 +191,10 @@ int pda_create(int device, int nstreams, pda_ctx** out) {
     auto* ctx = new pda_ctx();
     ctx->device = device;
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) {
         ctx->num_cu = prop.multiProcessorCount;
+        if (prop.sharedMemPerBlock > 0) ctx->lds_per_block = prop.sharedMemPerBlock;
+    }
     ctx->slots.resize(size_t(nstreams));
     for (auto& s : ctx->slots) {
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
@@ -244,10 +248,12 @@ int pda_destroy(pda_ctx* ctx) {
         if (s.ev_mid) (void)hipEventDestroy(s.ev_mid);
         if (s.ev1) (void)hipEventDestroy(s.ev1);
         free_population(s.pop);
+        free_rb(s.rb);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
     free_data(ctx);
     (void)hipFree(ctx->d_obs);
+    (void)hipFree(ctx->d_lf);
     delete ctx;
     return PDA_OK;
 }

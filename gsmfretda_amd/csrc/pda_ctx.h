@@ -79,6 +79,32 @@ struct Population {
     hipEvent_t ev0 = nullptr, ev_mid = nullptr, ev1 = nullptr;
 };
 
+// Rao-Blackwellised runs of a slot: per-unit walk summaries of the last
+// forward run (reused by the adjoint), histogram rows, result, gradient.
+struct RbState {
+    void* d_rbm = nullptr;             // RbModel
+    void* h_rbm = nullptr;             // pinned
+    uint2* d_cnt = nullptr;            // per unit: photon counts per state
+    float* d_F = nullptr;              // per unit: F
+    float* d_S = nullptr;              // per unit: scores of the rates
+    size_t cap_units = 0, cap_scores = 0;
+    unsigned long long* d_hist = nullptr;  // [8][kMaxBins]
+    uint32_t* d_work = nullptr;        // 9 counters
+    uint8_t* d_res = nullptr;          // header + uint64 M[kMaxBins]
+    uint8_t* h_res = nullptr;          // pinned
+    double* d_g = nullptr;             // cotangent per bin
+    double* h_g = nullptr;             // pinned
+    double* d_grad = nullptr;          // n + K
+    double* h_grad = nullptr;          // pinned
+    hipEvent_t ev0 = nullptr, ev_walk = nullptr, ev1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;
+    bool ran = false, scores = false, adj_ran = false;
+    int R = 0, B = 0, n = 0, K = 0, cap = 0, left_pad = 0;
+    int64_t b0 = 0, b1 = 0;
+    uint32_t seed = 0, set = 0;
+    int init_state = 0;
+    bool rates_positive = false, p_interior = false;
+};
+
 struct Slot {
     hipStream_t stream = nullptr;
     ModelBlob* h_blob = nullptr;  // pinned staging of the model tables
@@ -109,6 +135,7 @@ struct Slot {
     bool samples_valid = false;
     hipEvent_t ev0 = nullptr, ev_mid = nullptr, ev1 = nullptr;
     Population pop;
+    RbState rb;
 };
 
 constexpr uint32_t kDefaultCap = 256;  // LDS photon capacity of a replayed burst
@@ -128,6 +155,9 @@ struct pda_ctx {
     float* d_obs = nullptr;
     uint32_t cap = pda::kDefaultCap;
     pda::LaunchShape shape;
+    std::vector<uint32_t> h_P;      // photons per burst (host copy)
+    double* d_lf = nullptr;         // log k! for the RB form
+    size_t lds_per_block = 65536;   // LDS a workgroup may allocate
 };
 
 namespace pda {
@@ -136,6 +166,7 @@ Slot* slot_of(pda_ctx* ctx, int sid);
 int sync_all(pda_ctx* ctx);
 void range_of(const pda_ctx* ctx, const Slot& s, int64_t* b0, int64_t* b1);
 void free_population(Population& p);
+void free_rb(RbState& r);
 
 // Grid and batch of a walk launch over `units` work units.
 struct Launch {
@@ -170,5 +201,17 @@ hipError_t launch_pop_finish(const ModelBlob* blobs, const RunArgs& a, const flo
                              PopStatus* status, SetResult* results, int32_t* hists,
                              hipStream_t stream);
 hipError_t launch_primitive(int op, const void* in, void* out, int64_t n, hipStream_t stream);
+struct RbModel;
+hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu);
+hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
+                          const uint2* links, const ModelBlob* blob, const RbModel* rbm,
+                          const RunArgs& a, uint2* cnt, float* F, float* S, uint32_t* work,
+                          hipStream_t stream);
+hipError_t launch_rb_pmf(bool grad_mode, uint32_t grid, int cap, const RbModel* rbm,
+                         const double* lf, const RunArgs& a, const uint2* cnt, const float* F,
+                         const float* S, const double* g, unsigned long long* hist_acc,
+                         double* grad, uint32_t* work, hipStream_t stream);
+hipError_t launch_rb_finish(const RunArgs& a, const float* obs, unsigned long long* hist_acc,
+                            uint32_t* work, void* res, hipStream_t stream);
 
 }  // namespace pda

@@ -1,0 +1,344 @@
+// pda_rb.cpp -- host side of the Rao-Blackwellised (RB) PDA form: forward
+// (expected histogram and chi2) and adjoint (d chi2 / d args) through the C ABI.
+//
+// No counterpart in the reference, whose simulator draws every acceptor count
+// (src/mc.cu:167-170) and is therefore not differentiable.  The forward keeps
+// the reference's walk and binning and replaces the acceptor draw by its
+// exact distribution; the adjoint returns, for a cotangent g_b = d chi2 /
+// d M_b supplied by the caller, the derivative in the efficiencies (exact for
+// the RB estimator) and in the rates (likelihood-ratio estimator with a
+// per-burst leave-one-out baseline, unbiased for the derivative of the
+// expected histogram).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/gsmfretda_pda.h"
+#include "pda_ctx.h"
+#include "pda_layout.h"
+#include "pda_rb.h"
+#include "pda_types.h"
+
+namespace pda {
+
+void free_rb(RbState& r) {
+    (void)hipFree(r.d_rbm);
+    (void)hipHostFree(r.h_rbm);
+    (void)hipFree(r.d_cnt);
+    (void)hipFree(r.d_F);
+    (void)hipFree(r.d_S);
+    (void)hipFree(r.d_hist);
+    (void)hipFree(r.d_work);
+    (void)hipFree(r.d_res);
+    (void)hipHostFree(r.h_res);
+    (void)hipFree(r.d_g);
+    (void)hipHostFree(r.h_g);
+    (void)hipFree(r.d_grad);
+    (void)hipHostFree(r.h_grad);
+    for (hipEvent_t e : {r.ev0, r.ev_walk, r.ev1, r.ev_a0, r.ev_a1})
+        if (e) (void)hipEventDestroy(e);
+    r = RbState();
+}
+
+}  // namespace pda
+
+using namespace pda;
+
+namespace {
+
+struct RbResultHeaderH {  // device RbResultHeader
+    double chi2;
+    int32_t effN;
+    int32_t B;
+};
+constexpr size_t kRbResBytes = sizeof(RbResultHeaderH) + sizeof(uint64_t) * kMaxBins;
+constexpr int kGradLen = kRbMaxStates + kRbMaxPairs;
+constexpr uint32_t kRbCounters = 9;
+
+int ensure_rb(pda_ctx* ctx, RbState& r) {
+    if (!r.d_work) {
+        HIP_TRY(hipMalloc(&r.d_rbm, sizeof(RbModel)));
+        HIP_TRY(hipHostMalloc(&r.h_rbm, sizeof(RbModel), hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&r.d_hist, sizeof(unsigned long long) * 8u * kMaxBins));
+        HIP_TRY(hipMemset(r.d_hist, 0, sizeof(unsigned long long) * 8u * kMaxBins));
+        HIP_TRY(hipMalloc(&r.d_work, sizeof(uint32_t) * kRbCounters));
+        HIP_TRY(hipMemset(r.d_work, 0, sizeof(uint32_t) * kRbCounters));
+        HIP_TRY(hipMalloc(&r.d_res, kRbResBytes));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_res), kRbResBytes, hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&r.d_g, sizeof(double) * kMaxBins));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_g), sizeof(double) * kMaxBins,
+                              hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&r.d_grad, sizeof(double) * kGradLen));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_grad), sizeof(double) * kGradLen,
+                              hipHostMallocDefault));
+        for (hipEvent_t* e : {&r.ev0, &r.ev_walk, &r.ev1, &r.ev_a0, &r.ev_a1})
+            HIP_TRY(hipEventCreate(e));
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    if (!ctx->d_lf) {
+        std::vector<double> lf(kRbMaxCount + 1);
+        log_factorials(lf.data(), kRbMaxCount);
+        HIP_TRY(hipMalloc(&ctx->d_lf, sizeof(double) * lf.size()));
+        HIP_TRY(hipMemcpy(ctx->d_lf, lf.data(), sizeof(double) * lf.size(), hipMemcpyHostToDevice));
+    }
+    return PDA_OK;
+}
+
+void build_rb_model(const Model& M, RbModel& R, bool* rates_positive, bool* p_interior) {
+    std::memset(&R, 0, sizeof R);
+    const int n = M.n;
+    R.n = n;
+    R.K = n * (n - 1);
+    *rates_positive = true;
+    *p_interior = true;
+    for (int s = 0; s < n; ++s) {
+        R.p[s] = M.ep0[s];
+        const double p = double(M.ep0[s]);
+        R.lp[s] = p > 0.0 ? dlog(p) : 0.0;
+        R.lq[s] = p < 1.0 ? dlog(1.0 - p) : 0.0;
+        if (p > 0.0 && p < 1.0) R.dscale[s] = 1.0 / (p * (1.0 - p));
+        else *p_interior = false;
+        float sum = 0.0f;
+        for (int j = 0; j < n; ++j)
+            if (j != s) sum += M.krate[s][j];
+        R.inv_sum[s] = sum > 0.0f ? 1.0f / sum : 0.0f;
+        for (int j = 0; j < n; ++j) {
+            if (j == s) continue;
+            R.pair[s][j] = rate_pair(n, s, j);
+            const float k = M.krate[s][j];
+            R.two_over_k[s][j] = k > 0.0f ? 2.0f / k : 0.0f;
+            if (!(k > 0.0f)) *rates_positive = false;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int flags) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    if (R < 1) return fail(PDA_E_ARG, "R = %d < 1", R);
+    if (!s->have_params) return fail(PDA_E_STATE, "stream %d: no parameters set", sid);
+    if (ctx->B < 1 || !ctx->d_obs) return fail(PDA_E_STATE, "no observed histogram set");
+    if (s->n > kRbMaxStates)
+        return fail(PDA_E_ARG, "the RB form takes n <= %d states (n = %d)", kRbMaxStates, s->n);
+    if (s->any_v)
+        return fail(PDA_E_ARG, "the RB form needs v = 0 (no distance broadening) in every state");
+    HIP_TRY(hipSetDevice(ctx->device));
+    RbState& r = s->rb;
+    int rc = ensure_rb(ctx, r);
+    if (rc) return rc;
+    Model M;
+    {
+        std::string err;
+        rc = build_model(s->ph, s->n, s->args.data(), int(s->args.size()), M, err);
+        if (rc) return fail(rc, "%s", err.c_str());
+    }
+    int64_t b0, b1;
+    range_of(ctx, *s, &b0, &b1);
+    const int64_t units = (b1 - b0) * int64_t(R);
+    if (units > int64_t(0x7FFFFFFF) - kWave)
+        return fail(PDA_E_ARG, "%lld bursts x R = %d exceeds 2^31 units", (long long)(b1 - b0), R);
+    // acceptor totals per unit: m_w <= floor(c (photons of the window) + 1/2)
+    // with c = max(gamma - crosstalk, 1 - dexdir); at most the burst's photons
+    // when c <= 1
+    uint32_t maxP = 0;
+    for (int64_t b = b0; b < b1; ++b) maxP = std::max(maxP, ctx->h_P[size_t(b)]);
+    const double coef = std::max({double(M.gmx), double(M.omdd), 0.0});
+    const double bound = coef <= 1.0 ? double(maxP) : std::ceil((coef + 0.5) * double(maxP));
+    if (bound > double(kRbMaxCount))
+        return fail(PDA_E_DATA, "the RB form supports acceptor totals up to %d (bursts of up to "
+                    "%u photons need %.0f)", kRbMaxCount, maxP, bound);
+    const int cap = int(bound) + 1;
+    const bool scores = (flags & PDA_RB_SCORES) != 0;
+    const size_t lds_grad = sizeof(float) * size_t(cap) * 8u + 8u * kMaxBins + sizeof(RbModel);
+    if (lds_grad > ctx->lds_per_block)
+        return fail(PDA_E_DATA, "bursts of %u photons exceed the LDS of the RB kernels", maxP);
+
+    // buffers of the per-unit summaries
+    if (size_t(units) > r.cap_units) {
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        (void)hipFree(r.d_cnt);
+        (void)hipFree(r.d_F);
+        r.d_cnt = nullptr;
+        r.d_F = nullptr;
+        r.cap_units = 0;
+        HIP_TRY(hipMalloc(&r.d_cnt, sizeof(uint2) * size_t(std::max<int64_t>(units, 1))));
+        HIP_TRY(hipMalloc(&r.d_F, sizeof(float) * size_t(std::max<int64_t>(units, 1))));
+        r.cap_units = size_t(units);
+    }
+    const size_t nsc = scores ? size_t(units) * size_t(s->n * (s->n - 1)) : 0;
+    if (nsc > r.cap_scores) {
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        (void)hipFree(r.d_S);
+        r.d_S = nullptr;
+        r.cap_scores = 0;
+        HIP_TRY(hipMalloc(&r.d_S, sizeof(float) * nsc));
+        r.cap_scores = nsc;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));  // the pinned model may still be in use
+    RbModel* hm = static_cast<RbModel*>(r.h_rbm);
+    bool rates_positive = false, p_interior = false;
+    build_rb_model(M, *hm, &rates_positive, &p_interior);
+
+    RunArgs a{};
+    a.begin = uint32_t(b0);
+    a.nunits = uint32_t(units);
+    a.R = uint32_t(R);
+    a.seed = seed;
+    a.set = set_id;
+    a.B = ctx->B;
+    a.left_pad = ctx->left_pad;
+    a.bg_on = ctx->bg_dd < 1e-4f ? 0 : 1;  // src/mc.cu:153
+    a.clk_p = ctx->clk_p;
+    a.bg_ad = ctx->bg_ad;
+    a.bg_dd = ctx->bg_dd;
+    a.cap = ctx->cap;
+    a.nstates = s->n;
+    a.nsets = 1;
+    // walk grid: persistent, batches of about two grabs per wave
+    int per_cu = 0;
+    HIP_TRY(rb_occupancy(scores ? 1 : 0, 0, &per_cu));
+    per_cu = std::max(per_cu, 1);
+    if (ctx->shape.waves_per_cu > 0) per_cu = std::min(per_cu, int(ctx->shape.waves_per_cu));
+    const uint64_t max_grid = uint64_t(ctx->num_cu) * uint64_t(per_cu);
+    uint64_t batch = uint64_t(kWave) * ctx->shape.batch_tiles;
+    if (ctx->shape.batch_tiles == 0) {
+        batch = (uint64_t(units) / (max_grid * 2u) + kWave - 1) / kWave * kWave;
+        batch = std::min<uint64_t>(std::max<uint64_t>(batch, kWave), 4096);
+    }
+    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, (uint64_t(units) + batch - 1) / batch)));
+    a.batch = uint32_t(batch);
+    a.queues = std::min<uint32_t>(ctx->shape.queues ? ctx->shape.queues : kMaxQueues, grid);
+    const int dyn = int(sizeof(float)) * cap * 3;
+    HIP_TRY(rb_occupancy(2, dyn, &per_cu));
+    per_cu = std::max(per_cu, 1);
+    const uint32_t pgrid = uint32_t(std::max<int64_t>(
+        1, std::min<int64_t>(int64_t(ctx->num_cu) * per_cu, b1 - b0)));
+
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
+    HIP_TRY(hipMemcpyAsync(r.d_rbm, r.h_rbm, sizeof(RbModel), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipEventRecord(r.ev0, s->stream));
+    HIP_TRY(launch_rb_walk(scores, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob,
+                           static_cast<const RbModel*>(r.d_rbm), a, r.d_cnt, r.d_F, r.d_S,
+                           r.d_work, s->stream));
+    HIP_TRY(hipEventRecord(r.ev_walk, s->stream));
+    HIP_TRY(launch_rb_pmf(false, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, a,
+                          r.d_cnt, r.d_F, r.d_S, nullptr, r.d_hist, nullptr, r.d_work, s->stream));
+    HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s->stream));
+    HIP_TRY(hipEventRecord(r.ev1, s->stream));
+    HIP_TRY(hipMemcpyAsync(r.h_res, r.d_res, sizeof(RbResultHeaderH) + sizeof(uint64_t) * size_t(a.B),
+                           hipMemcpyDeviceToHost, s->stream));
+    r.ran = true;
+    r.adj_ran = false;
+    r.scores = scores;
+    r.R = R;
+    r.B = a.B;
+    r.left_pad = a.left_pad;
+    r.n = s->n;
+    r.K = s->n * (s->n - 1);
+    r.cap = cap;
+    r.b0 = b0;
+    r.b1 = b1;
+    r.seed = seed;
+    r.set = set_id;
+    r.init_state = M.init_state;
+    r.rates_positive = rates_positive;
+    r.p_interior = p_interior;
+    return PDA_OK;
+}
+
+int pda_rb_get_result(pda_ctx* ctx, int sid, uint64_t* M_fixed, double* M, int nbins,
+                      double* chi2, int32_t* effN) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    RbState& r = s->rb;
+    if (!r.ran) return fail(PDA_E_STATE, "stream %d has no RB run", sid);
+    if ((M_fixed || M) && nbins != r.B) return fail(PDA_E_ARG, "nbins %d != run's %d", nbins, r.B);
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const RbResultHeaderH* h = reinterpret_cast<const RbResultHeaderH*>(r.h_res);
+    const uint64_t* m = reinterpret_cast<const uint64_t*>(h + 1);
+    for (int b = 0; b < r.B; ++b) {
+        if (M_fixed) M_fixed[b] = m[b];
+        if (M) M[b] = double(m[b]) * 0x1p-32;
+    }
+    if (chi2) *chi2 = h->chi2;
+    if (effN) *effN = h->effN;
+    return PDA_OK;
+}
+
+int pda_rb_adjoint(pda_ctx* ctx, int sid, const double* g, int nbins, double* grad, int nargs) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    RbState& r = s->rb;
+    if (!r.ran) return fail(PDA_E_STATE, "stream %d has no RB forward run", sid);
+    if (!g || !grad) return fail(PDA_E_ARG, "null argument");
+    if (nbins != r.B) return fail(PDA_E_ARG, "nbins %d != run's %d", nbins, r.B);
+    if (nargs != r.n * (r.n + 1)) return fail(PDA_E_ARG, "nargs %d != %d", nargs, r.n * (r.n + 1));
+    if (!r.scores) return fail(PDA_E_STATE, "the forward run kept no scores (PDA_RB_SCORES)");
+    if (r.R < 2) return fail(PDA_E_STATE, "rate gradients need R >= 2 resamples");
+    if (!r.p_interior) return fail(PDA_E_ARG, "efficiency gradients need 0 < E < 1");
+    if (!r.rates_positive) return fail(PDA_E_ARG, "rate gradients need every rate > 0");
+    if (r.init_state < 0)
+        return fail(PDA_E_ARG, "rate gradients need the initial state fixed (inflow of state 0 > 1 s^-1)");
+    for (int b = 0; b < nbins; ++b)
+        if (!std::isfinite(g[b])) return fail(PDA_E_ARG, "g[%d] not finite", b);
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::memcpy(r.h_g, g, sizeof(double) * size_t(nbins));
+    RunArgs a{};
+    a.begin = uint32_t(r.b0);
+    a.nunits = uint32_t((r.b1 - r.b0) * int64_t(r.R));
+    a.R = uint32_t(r.R);
+    a.B = r.B;
+    a.left_pad = r.left_pad;
+    a.nstates = r.n;
+    const int dyn = int(sizeof(float)) * r.cap * 8;
+    int per_cu = 0;
+    HIP_TRY(rb_occupancy(3, dyn, &per_cu));
+    per_cu = std::max(per_cu, 1);
+    const uint32_t pgrid = uint32_t(std::max<int64_t>(
+        1, std::min<int64_t>(int64_t(ctx->num_cu) * per_cu, r.b1 - r.b0)));
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
+    HIP_TRY(hipMemsetAsync(r.d_grad, 0, sizeof(double) * kGradLen, s->stream));
+    HIP_TRY(hipMemcpyAsync(r.d_g, r.h_g, sizeof(double) * size_t(nbins), hipMemcpyHostToDevice,
+                           s->stream));
+    HIP_TRY(hipEventRecord(r.ev_a0, s->stream));
+    HIP_TRY(launch_rb_pmf(true, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, a,
+                          r.d_cnt, r.d_F, r.d_S, r.d_g, nullptr, r.d_grad, r.d_work, s->stream));
+    HIP_TRY(hipEventRecord(r.ev_a1, s->stream));
+    HIP_TRY(hipMemcpyAsync(r.h_grad, r.d_grad, sizeof(double) * kGradLen, hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int n = r.n, K = r.K;
+    for (int i = 0; i < n; ++i) grad[i] = r.h_grad[i];            // d chi2 / dE_i (dp/dE = 1)
+    for (int i = 0; i < K; ++i) grad[n + i] = r.h_grad[n + i];    // rates, args order
+    for (int i = n + K; i < nargs; ++i) grad[i] = 0.0;            // v (held at 0)
+    r.adj_ran = true;
+    return PDA_OK;
+}
+
+int pda_rb_stats(pda_ctx* ctx, int sid, float* walk_ms, float* forward_ms, float* adjoint_ms) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    RbState& r = s->rb;
+    if (!r.ran) return fail(PDA_E_STATE, "stream %d has no RB run", sid);
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (walk_ms) HIP_TRY(hipEventElapsedTime(walk_ms, r.ev0, r.ev_walk));
+    if (forward_ms) HIP_TRY(hipEventElapsedTime(forward_ms, r.ev0, r.ev1));
+    if (adjoint_ms) {
+        *adjoint_ms = 0.f;
+        if (r.adj_ran) HIP_TRY(hipEventElapsedTime(adjoint_ms, r.ev_a0, r.ev_a1));
+    }
+    return PDA_OK;
+}
+
+}  // extern "C"

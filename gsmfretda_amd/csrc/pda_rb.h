@@ -1,0 +1,101 @@
+// pda_rb.h -- the Rao-Blackwellised (RB) form of the Monte-Carlo PDA walk.
+//
+// The plain walk (mc_kernel, src/mc.cu:83-197) draws, per dwell window, an
+// acceptor count X_w ~ Binomial(m_w, E'_w).  Without distance broadening
+// (v = 0) E'_w is the efficiency of the window's state, so the acceptor
+// total of a (burst, resample) unit given its walk is a sum of one binomial
+// per state, Binomial(M_s, p_s) with M_s the photon counts of the windows
+// spent in state s.  The RB form keeps the walk (the same keyed draws of
+// states, dwell times and background) and replaces the acceptor draw by the
+// exact distribution of the total: P(a) = conv_s Binomial(M_s, p_s)(a),
+// spread into the E histogram at E = a / F.  The expected histogram is smooth
+// in E; its derivative in the rates is estimated by the likelihood ratio of
+// the walk's draws.
+//
+// Host-and-device code: the kernels and the host build used by the tests run
+// the same operations in the same order.
+#pragma once
+#include <cstdint>
+
+#include "pda_math.h"
+#include "pda_types.h"
+
+namespace pda {
+
+constexpr int kRbMaxStates = 4;
+constexpr int kRbMaxPairs = kRbMaxStates * (kRbMaxStates - 1);
+constexpr int kRbMaxCount = 4096;  // photons per burst the RB form accepts
+
+// Per parameter set, built on the host from the Model.
+struct RbModel {
+    int32_t n, K;              // states, rates (n (n - 1))
+    int32_t pad0[2];
+    float p[kRbMaxStates];     // acceptor probability per state (Model::ep0)
+    double lp[kRbMaxStates];   // log p, log (1 - p)
+    double lq[kRbMaxStates];
+    double dscale[kRbMaxStates];  // 1 / (p (1 - p)): d pmf / dp = pmf (k - m p) dscale
+    // score of the walk's draws per window (s -> j) (likelihood ratio):
+    // pair index of rate s -> j in the args' off-diagonal list, 2 / K(j, s),
+    // and 1 / sum_{j' != s} K(j', s)
+    int32_t pair[kRbMaxStates][kRbMaxStates];
+    float two_over_k[kRbMaxStates][kRbMaxStates];
+    float inv_sum[kRbMaxStates];
+};
+
+// Args index (after the n efficiencies) of the rate s -> j, K(j, s): row j,
+// column s of genMatK's row-major off-diagonal list (src/eigenhelper.cpp:4-21).
+PDA_HD inline int rate_pair(int n, int s, int j) { return j * (n - 1) + (s < j ? s : s - 1); }
+
+// log k! for k = 0..count, summed in order (lf[k] = lf[k-1] + log k).
+PDA_HD inline void log_factorials(double* lf, int count) {
+    lf[0] = 0.0;
+    for (int k = 1; k <= count; ++k) lf[k] = lf[k - 1] + dlog(double(k));
+}
+
+// Binomial(m, p)(k) in double from log factorials; exact 0/1 at p = 0, 1.
+PDA_HD inline double rb_binom_pmf(int m, int k, const double* lf, float p, double lp, double lq) {
+    if (p <= 0.0f) return k == 0 ? 1.0 : 0.0;
+    if (p >= 1.0f) return k == m ? 1.0 : 0.0;
+    double l = lf[m] - lf[k];
+    l = l - lf[m - k];
+    l = l + double(k) * lp;
+    l = l + double(m - k) * lq;
+    return dexp(l);
+}
+
+// One output of Z = Y (*) X: sum_{k} Y[k] X[a - k], k ascending over the
+// overlap, float accumulation from 0.
+PDA_HD inline float rb_conv_at(const float* Y, int ylen, const float* X, int xlen, int a) {
+    int k0 = a - (xlen - 1);
+    if (k0 < 0) k0 = 0;
+    const int k1 = a < ylen - 1 ? a : ylen - 1;
+    float z = 0.0f;
+    for (int k = k0; k <= k1; ++k) z = z + Y[k] * X[a - k];
+    return z;
+}
+
+// Histogram bin of a acceptors out of F corrected photons, as the plain
+// walk bins acc / F (-1: dropped).
+PDA_HD inline int rb_bin(int a, float F, int B) {
+    const float E = F > 0.0f ? float(a) / F : 0.0f;
+    if (!(E >= 0.0f && E < 1.0f)) return -1;
+    const int b = int(floor(double(E) * double(B)));
+    return b < B ? b : -1;
+}
+
+// Fixed-point weight of a probability (2^-32 units), rounded half up.
+PDA_HD inline uint64_t rb_fixed(float P) { return uint64_t(double(P) * 4294967296.0 + 0.5); }
+
+// Summary of one unit's walk: per-state photon counts (16 bits each), F.
+PDA_HD inline uint2 rb_pack(const uint32_t* cnt) {
+    uint2 v;
+    v.x = (cnt[0] & 0xFFFFu) | (cnt[1] << 16);
+    v.y = (cnt[2] & 0xFFFFu) | (cnt[3] << 16);
+    return v;
+}
+PDA_HD inline uint32_t rb_count(uint2 v, int s) {
+    const uint32_t w = s < 2 ? v.x : v.y;
+    return (s & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+
+}  // namespace pda

@@ -104,6 +104,15 @@ struct WalkState {
 
 enum StepResult : int { kStepMore = 0, kStepDone = 1, kStepReplay = 2 };
 
+// What one window drew, for the counts form (kCounts): the state it dwelt
+// in, the next state, the dwell time and the photon count the binomial
+// acceptor draw would have taken (int(m), <= 0 draws nothing).
+struct WindowInfo {
+    int s, j;
+    float tau;
+    int32_t m;
+};
+
 // Start of a unit (src/mc.cu:92-116): initial state from genMatP's inflow.
 // Returns kStepDone when the unit has no window that counts anything
 // (zero or one photon, or T <= 0: E = 0).
@@ -127,10 +136,13 @@ PDA_HD int begin_unit(const Model& M, const UnitKey& key, uint32_t P, float T,
 
 // One dwell window of the walk (one pass of the while loop, src/mc.cu:117-191).
 // tick/link point at the burst's photons in LDS or in global memory.
-template <bool kFast>
+// kCounts (general form only): no acceptor draw; the window is described in
+// *wi instead, for the Rao-Blackwellised form that sums the binomial pmf.
+template <bool kFast, bool kCounts = false>
 PDA_HD int step_window(const Model& M, const BinomTable* tab, const UnitKey& key, uint32_t P,
                        uint32_t stop_rel, float T, const uint32_t* tick, const uint2* link,
-                       const RunArgs& a, WalkState<kFast>& st) {
+                       const RunArgs& a, WalkState<kFast>& st, WindowInfo* wi = nullptr) {
+    static_assert(!(kFast && kCounts), "the counts form walks in the general form");
     using Tick = typename WalkState<kFast>::Tick;
     const int n = M.n;
     const int s = st.s;
@@ -146,7 +158,8 @@ PDA_HD int step_window(const Model& M, const BinomTable* tab, const UnitKey& key
         j = jj >= s ? jj + 1 : jj;
     }
     // dwell at the chosen transition's rate (drawTau, src/mc.cu:122)
-    const float tsum = st.tsum + flog(uniform24(A.w[1])) / (-M.krate[s][j]);
+    const float tau = flog(uniform24(A.w[1])) / (-M.krate[s][j]);
+    const float tsum = st.tsum + tau;
     Tick w1;
     uint32_t w1c;
     if (!(tsum < T)) {
@@ -209,7 +222,13 @@ PDA_HD int step_window(const Model& M, const BinomTable* tab, const UnitKey& key
         m = floorf(((fv - bgd) - bga) + 0.5f);
     }
     int64_t x;
-    if (kFast) {
+    if (kCounts) {
+        x = 0;
+        wi->s = s;
+        wi->j = j;
+        wi->tau = tau;
+        wi->m = int32_t(m);
+    } else if (kFast) {
         x = binomial_table(int(m), s, M, *tab, A.w[2], A.w[3]);
         if (x < 0) return kStepReplay;
     } else {

@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libgsmfretda_pda.so")
 
 PDA_RUN_SAMPLES = 1
 PDA_RUN_NO_GRAPH = 2
+PDA_RB_SCORES = 1
 MAX_STATES = 8
 MAX_BINS = 1024
 
@@ -37,7 +38,8 @@ EXPORTS = (
     "pda_get_samples", "pda_samples_host", "pda_last_run_ms", "pda_last_pass_ms", "pda_set_tuning",
     "pda_get_tuning", "pda_set_launch_shape", "pda_get_launch_shape", "pda_autotune",
     "pda_set_population", "pda_run_population", "pda_get_population_result",
-    "pda_population_stats", "pda_debug_primitive",
+    "pda_population_stats", "pda_rb_forward", "pda_rb_get_result", "pda_rb_adjoint",
+    "pda_rb_stats", "pda_debug_primitive",
 )
 
 
@@ -109,6 +111,12 @@ def load_library(path=LIB_PATH):
     L.pda_get_population_result.argtypes = [_vp, ctypes.c_int, _i32p, ctypes.c_int, _f64p,
                                             _f32p, _i32p]
     L.pda_population_stats.argtypes = [_vp, ctypes.c_int, _i64p, _i32p, _i32p, _f32p]
+    _u64p = ctypes.POINTER(ctypes.c_uint64)
+    L.pda_rb_forward.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                 ctypes.c_uint32, ctypes.c_int]
+    L.pda_rb_get_result.argtypes = [_vp, ctypes.c_int, _u64p, _f64p, ctypes.c_int, _f64p, _i32p]
+    L.pda_rb_adjoint.argtypes = [_vp, ctypes.c_int, _f64p, ctypes.c_int, _f64p, ctypes.c_int]
+    L.pda_rb_stats.argtypes = [_vp, ctypes.c_int, _f32p, _f32p, _f32p]
     _lib = L
     return L
 
@@ -383,6 +391,39 @@ class MC:
         _check(self._L.pda_population_stats(self._h, sid, ctypes.byref(u), ctypes.byref(r),
                                             ctypes.byref(c), ctypes.byref(ms)))
         return u.value, r.value, c.value, ms.value
+
+    # -- Rao-Blackwellised form ---------------------------------------------------
+    def rb_forward(self, R=None, seed=None, set_id=0, sid=0, scores=True):
+        """Asynchronous RB evaluation of the slot's parameter set (v = 0, n <= 4)."""
+        R = self.reSampleTimes if R is None else R
+        seed = self.seed if seed is None else seed
+        _check(self._L.pda_rb_forward(self._h, sid, R, seed, set_id,
+                                      PDA_RB_SCORES if scores else 0))
+
+    def rb_result(self, sid=0, nbins=None):
+        """(expected counts M_b as float64, the same in 2^-32 units, chi2, effN)."""
+        B = self._hist_bins if nbins is None else nbins
+        fixed = np.zeros(B, np.uint64)
+        M = np.zeros(B, np.float64)
+        c = ctypes.c_double()
+        e = ctypes.c_int32()
+        _check(self._L.pda_rb_get_result(self._h, sid, fixed.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_uint64)), _p(M, _f64p), B, ctypes.byref(c), ctypes.byref(e)))
+        return M, fixed, c.value, e.value
+
+    def rb_adjoint(self, g, n, sid=0):
+        """d chi2 / d args (mc::set_params layout) for the cotangent g = d chi2 / d M."""
+        g = np.ascontiguousarray(g, np.float64)
+        out = np.zeros(n * (n + 1), np.float64)
+        _check(self._L.pda_rb_adjoint(self._h, sid, _p(g, _f64p), len(g), _p(out, _f64p),
+                                      len(out)))
+        return out
+
+    def rb_stats(self, sid=0):
+        """GPU ms of the last RB walk, whole forward, adjoint."""
+        v = [ctypes.c_float() for _ in range(3)]
+        _check(self._L.pda_rb_stats(self._h, sid, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def debug_primitive(self, op, inp, out_dtype, n):
         inp = np.ascontiguousarray(inp)

@@ -31,6 +31,7 @@ extern "C" {
 /* pda_run flags */
 #define PDA_RUN_SAMPLES 1 /* also keep per-sample E (and acceptor count, F)  */
 #define PDA_RUN_NO_GRAPH 2 /* population runs: launch without graph capture */
+#define PDA_RB_SCORES 1    /* RB forward: keep the scores rate gradients need */
 
 typedef struct pda_ctx pda_ctx;
 
@@ -202,6 +203,40 @@ int pda_get_population_result(pda_ctx *ctx, int sid, int32_t *hist, int nbins, d
  * milliseconds of the whole sequence.  Outputs may be NULL. */
 int pda_population_stats(pda_ctx *ctx, int sid, int64_t *units_per_set, int32_t *replays,
                          int32_t *graph_captures, float *ms);
+
+/* Rao-Blackwellised (RB) form (no reference counterpart: the reference draws
+ * every acceptor count, src/mc.cu:167-170, and is not differentiable).
+ * Requires v = 0 in every state and n <= 4.  The walk of every unit uses the
+ * keyed draws of pda_run with the same (seed, set_id); instead of the
+ * acceptor draw, the exact distribution of the unit's acceptor total -- a
+ * convolution of Binomial(M_s, E_s) over the states, M_s the photon counts of
+ * the windows spent in state s -- is spread into the E histogram, giving the
+ * expected counts M_b (accumulated exactly in 2^-32 fixed point, so results
+ * do not depend on the launch shape).  chi2 is the reference's reduced chi2
+ * (src/streamWorker.cpp:197-206) on M_b / R.  flags & PDA_RB_SCORES keeps,
+ * per unit, the score of the walk's draws for pda_rb_adjoint.
+ * Asynchronous on the slot's stream. */
+int pda_rb_forward(pda_ctx *ctx, int sid, int R, uint32_t seed, uint32_t set_id, int flags);
+
+/* Result of the slot's last RB forward run (waits for it): M_b in 2^-32
+ * units and as doubles, chi2 and bins kept.  Outputs may be NULL. */
+int pda_rb_get_result(pda_ctx *ctx, int sid, uint64_t *M_fixed, double *M, int nbins,
+                      double *chi2, int32_t *effN);
+
+/* Adjoint of the slot's last RB forward run (blocking): for a cotangent
+ * g[b] = d chi2 / d M_b, writes grad[nargs] in mc::set_params layout
+ * (src/mc.cu:790-800): d/dE_s exactly for the RB estimator, d/dK for the
+ * off-diagonal rates by the likelihood ratio of the walk's state and dwell
+ * draws with a per-burst leave-one-out baseline (the photon counts are step
+ * functions of the dwell times, so their pathwise derivative is zero almost
+ * everywhere), 0 for v.  Needs a forward run with PDA_RB_SCORES, R >= 2,
+ * 0 < E < 1, rates > 0 and a fixed initial state. */
+int pda_rb_adjoint(pda_ctx *ctx, int sid, const double *g, int nbins, double *grad, int nargs);
+
+/* GPU milliseconds of the slot's last RB runs: the walk, the whole forward
+ * sequence, the adjoint kernel (0 before the first adjoint).  Outputs may be
+ * NULL. */
+int pda_rb_stats(pda_ctx *ctx, int sid, float *walk_ms, float *forward_ms, float *adjoint_ms);
 
 /* Diagnostics (no reference counterpart): evaluates one primitive of the
  * shared variate library on the GPU for n inputs, so tests can compare it

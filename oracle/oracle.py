@@ -97,6 +97,18 @@ def lib():
         L.or_hist.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, _i64p]
         L.or_chi2.argtypes = [_f32p, _i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_int, _f32p, _f64p, ctypes.POINTER(ctypes.c_int)]
+        L.or_rb_unit.argtypes = [ctypes.POINTER(OrData), ctypes.POINTER(OrParams), ctypes.c_int64,
+                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _i32p, _f32p,
+                                 _f64p, _i32p]
+        L.or_rb_run.argtypes = [ctypes.POINTER(OrData), ctypes.POINTER(OrParams), ctypes.c_int64,
+                                ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+        L.or_rb_grad.argtypes = [ctypes.POINTER(OrData), ctypes.POINTER(OrParams), ctypes.c_int64,
+                                 ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_int, _f64p, _f64p, _f64p]
+        L.or_log_factorials.restype = _f64p
+        L.or_state_efficiency.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.or_state_efficiency.restype = ctypes.c_float
         _lib = L
     return _lib
 
@@ -204,3 +216,48 @@ def chi2(O, M, R, nstates, left_pad=2):
     lib().or_chi2(_ptr(O, _f32p), _ptr(M, _i64p), len(O), left_pad, R, nstates,
                   ctypes.byref(c32), ctypes.byref(c64), ctypes.byref(eff))
     return c32.value, c64.value, eff.value
+
+
+def rb_unit(data, params, burst, r, seed, set_id=0, transitions=False):
+    """Walk summary of one unit for the Rao-Blackwellised form:
+    (photon counts per state, F, scores of the rates in args order
+    [, windows per transition in args order])."""
+    n = params.n
+    cnt = np.zeros(n, np.int32)
+    F = ctypes.c_float()
+    S = np.zeros(max(n * (n - 1), 1), np.float64)
+    N = np.zeros(max(n * (n - 1), 1), np.int32)
+    lib().or_rb_unit(ctypes.byref(data.s), ctypes.byref(params), burst, r, seed, set_id,
+                     _ptr(cnt, _i32p), ctypes.byref(F), _ptr(S, _f64p), _ptr(N, _i32p))
+    if transitions:
+        return cnt, F.value, S[:n * (n - 1)], N[:n * (n - 1)]
+    return cnt, F.value, S[:n * (n - 1)]
+
+
+def rb_run(data, params, R, seed, set_id=0, b0=0, b1=None, B=50):
+    """Expected histogram of the RB form in 2^-32 units (float path)."""
+    if b1 is None:
+        b1 = data.nbursts
+    M = np.zeros(B, np.uint64)
+    lib().or_rb_run(ctypes.byref(data.s), ctypes.byref(params), b0, b1, R, seed, set_id, B,
+                    M.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    return M
+
+
+def rb_grad(data, params, R, seed, g, set_id=0, b0=0, b1=None):
+    """FP64 expected histogram M and the gradient of sum_b g_b M_b in the
+    args (E then the off-diagonal rates in args order)."""
+    if b1 is None:
+        b1 = data.nbursts
+    g = np.ascontiguousarray(g, np.float64)
+    n = params.n
+    M = np.zeros(len(g), np.float64)
+    grad = np.zeros(n * n, np.float64)
+    lib().or_rb_grad(ctypes.byref(data.s), ctypes.byref(params), b0, b1, R, seed, set_id,
+                     len(g), _ptr(g, _f64p), _ptr(M, _f64p), _ptr(grad, _f64p))
+    return M, grad
+
+
+def log_factorials(count):
+    p = lib().or_log_factorials()
+    return np.array([p[i] for i in range(count + 1)])

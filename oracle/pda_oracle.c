@@ -449,8 +449,33 @@ static float broaden(float E, float r0, float v, float z) {
 /* ------------------------------------------------------------------ */
 /* One unit = mc_kernel body (src/mc.cu:92-196) for one resample.      */
 /* ------------------------------------------------------------------ */
+/* Windows of the walk as the RB form needs them: photon count per state of
+ * the binomial acceptor draw each window would make, and the score of the
+ * state and dwell draws, d/dK(j',s) log[P(j|s) K(j,s) exp(-K(j,s) tau)]
+ * = [j'=j] (2/K(j,s) - tau) - 1/sum_j'' K(j'',s), in FP64.  Windows are
+ * recorded up to the one whose scan reaches the last photon (every later
+ * window counts nothing; the kernels stop there). */
+typedef struct {
+    int stopped;
+    int32_t cnt[OR_MAXN];
+    double score[OR_MAXN * (OR_MAXN - 1)];
+    int32_t ntrans[OR_MAXN * (OR_MAXN - 1)]; /* windows per transition s -> j */
+} rb_trace;
+
+static int rate_pair(int n, int s, int j) { return j * (n - 1) + (s < j ? s : s - 1); }
+
+static void unit_impl(const or_data *d, const or_params *p, int64_t burst, uint32_t r,
+                      uint32_t seed, uint32_t set, int32_t *acc_out, float *F_out, float *E_out,
+                      or_stats *st, rb_trace *tr);
+
 void or_unit(const or_data *d, const or_params *p, int64_t burst, uint32_t r, uint32_t seed,
              uint32_t set, int32_t *acc_out, float *F_out, float *E_out, or_stats *st) {
+    unit_impl(d, p, burst, r, seed, set, acc_out, F_out, E_out, st, NULL);
+}
+
+static void unit_impl(const or_data *d, const or_params *p, int64_t burst, uint32_t r,
+                      uint32_t seed, uint32_t set, int32_t *acc_out, float *F_out, float *E_out,
+                      or_stats *st, rb_trace *tr) {
     const int n = p->n;
     const int64_t P = d->phcount[burst];
     const int64_t base = d->istart[burst];
@@ -467,6 +492,10 @@ void or_unit(const or_data *d, const or_params *p, int64_t burst, uint32_t r, ui
     float tsum = 0.0f, F = 0.0f, accf = 0.0f;
     int64_t w0 = 0, w1 = 0, cursor = 0;
     uint32_t w = 0;
+    if (tr) {
+        memset(tr, 0, sizeof(*tr));
+        tr->stopped = P <= 1; /* no window of such a burst counts anything */
+    }
     while (T > tsum && w < (1u << 22)) { /* src/mc.cu:117; window cap: see header */
         uint32_t A[4];
         blockw(seed, set, w, 0, r, b, A, st);
@@ -519,6 +548,17 @@ void or_unit(const or_data *d, const or_params *p, int64_t burst, uint32_t r, ui
                 blockw(seed, set, w, 2, r, b, C, st);
                 float de = broaden(p->E[s], p->r0, p->v[s], or_normal(C[0], C[1]));
                 xdraw = or_binomial((int)m, (double)de, seed, set, b, r, w, A[2], A[3], st);
+                if (tr && !tr->stopped) {
+                    if ((int)m > 0) tr->cnt[s] += (int)m;
+                    double sum = 0.0;
+                    for (int ii = 0; ii < n; ++ii)
+                        if (ii != s) sum += (double)p->K[ii + s * n];
+                    tr->score[rate_pair(n, s, j)] += 2.0 / (double)k - (double)tau;
+                    tr->ntrans[rate_pair(n, s, j)]++;
+                    for (int j2 = 0; j2 < n; ++j2)
+                        if (j2 != s) tr->score[rate_pair(n, s, j2)] -= 1.0 / sum;
+                    if (cursor == P - 1) tr->stopped = 1;
+                }
                 break;
             }
             if (isD) sd = 1;
@@ -626,4 +666,193 @@ void or_chi2(const float *O, const int64_t *M, int B, int left_pad, int R, int n
     if (chi2_ref) *chi2_ref = c32 / (float)dof;
     if (chi2_64) *chi2_64 = c64 / (double)dof;
     if (effN_out) *effN_out = effN;
+}
+
+/* ------------------------------------------------------------------ */
+/* Rao-Blackwellised form: the acceptor total of a unit given its walk  */
+/* is a sum over states of Binomial(M_s, p_s); its pmf is spread into   */
+/* the E histogram instead of one draw.                                 */
+/* ------------------------------------------------------------------ */
+
+#define OR_RB_MAXC 4096
+static double g_lf[OR_RB_MAXC + 1];
+static int g_lf_ready = 0;
+
+static const double *log_factorials(void) {
+    if (!g_lf_ready) {
+        g_lf[0] = 0.0;
+        for (int k = 1; k <= OR_RB_MAXC; ++k) g_lf[k] = g_lf[k - 1] + or_log((double)k);
+        g_lf_ready = 1;
+    }
+    return g_lf;
+}
+
+const double *or_log_factorials(void) { return log_factorials(); }
+
+/* Binomial(m, p)(k) in double: exp of log factorials and logs, this order. */
+static double binom_pmf(int m, int k, float p) {
+    if (p <= 0.0f) return k == 0 ? 1.0 : 0.0;
+    if (p >= 1.0f) return k == m ? 1.0 : 0.0;
+    const double *lf = log_factorials();
+    const double lp = or_log((double)p), lq = or_log(1.0 - (double)p);
+    double l = lf[m] - lf[k];
+    l = l - lf[m - k];
+    l = l + (double)k * lp;
+    l = l + (double)(m - k) * lq;
+    return or_exp(l);
+}
+
+static int rb_bin(int a, float F, int B) {
+    const float E = F > 0.0f ? (float)a / F : 0.0f;
+    if (!(E >= 0.0f && E < 1.0f)) return -1;
+    const int b = (int)floor((double)E * (double)B);
+    return b < B ? b : -1;
+}
+
+static float state_p(const or_params *p, int s) { return broaden(p->E[s], p->r0, 0.0f, 0.0f); }
+
+/* Acceptor probability of a state without broadening: E after the distance
+ * round trip of drawE (src/gen_rand.cuh:63-68) with v = 0. */
+float or_state_efficiency(float E, float r0) { return broaden(E, r0, 0.0f, 0.0f); }
+
+int or_rb_unit(const or_data *d, const or_params *p, int64_t burst, uint32_t r, uint32_t seed,
+               uint32_t set, int32_t *cnt, float *F, double *score, int32_t *ntrans) {
+    rb_trace tr;
+    unit_impl(d, p, burst, r, seed, set, NULL, F, NULL, NULL, &tr);
+    for (int s = 0; s < p->n; ++s) cnt[s] = tr.cnt[s];
+    for (int i = 0; i < p->n * (p->n - 1); ++i) {
+        if (score) score[i] = tr.score[i];
+        if (ntrans) ntrans[i] = tr.ntrans[i];
+    }
+    return 0;
+}
+
+/* Float path, the kernels' operations in their order: binomial pmfs rounded
+ * to float, convolutions accumulated in float (k ascending from 0.0f), each
+ * outcome's probability added to its bin in 2^-32 fixed point. */
+void or_rb_run(const or_data *d, const or_params *p, int64_t b0, int64_t b1, int R,
+               uint32_t seed, uint32_t set, int B, uint64_t *M) {
+    static float X[OR_RB_MAXC + 1], Y[OR_RB_MAXC + 1], Z[OR_RB_MAXC + 1];
+    for (int64_t b = b0; b < b1; ++b)
+        for (int r = 0; r < R; ++r) {
+            int32_t cnt[OR_MAXN];
+            float F;
+            or_rb_unit(d, p, b, (uint32_t)r, seed, set, cnt, &F, NULL, NULL);
+            int xl = 1;
+            X[0] = 1.0f;
+            for (int s = 0; s < p->n; ++s) {
+                const int m = cnt[s];
+                if (m == 0) continue;
+                const float ps = state_p(p, s);
+                for (int k = 0; k <= m; ++k) Y[k] = (float)binom_pmf(m, k, ps);
+                if (xl == 1) {
+                    for (int k = 0; k <= m; ++k) X[k] = Y[k];
+                    xl = m + 1;
+                    continue;
+                }
+                const int nl = xl + m;
+                for (int a = 0; a < nl; ++a) {
+                    int k0 = a - (xl - 1);
+                    if (k0 < 0) k0 = 0;
+                    const int k1 = a < m ? a : m;
+                    float z = 0.0f;
+                    for (int k = k0; k <= k1; ++k) z = z + Y[k] * X[a - k];
+                    Z[a] = z;
+                }
+                for (int a = 0; a < nl; ++a) X[a] = Z[a];
+                xl = nl;
+            }
+            for (int a = 0; a < xl; ++a) {
+                const int bin = rb_bin(a, F, B);
+                if (bin >= 0) M[bin] += (uint64_t)((double)X[a] * 4294967296.0 + 0.5);
+            }
+        }
+}
+
+/* FP64 path: expected counts M (double) and, for a cotangent g[B], the
+ * gradient of sum_b g_b M_b in the args: d/dE_s by forward-mode
+ * differentiation of the convolution chain (d Binomial(m,p)(k)/dp =
+ * pmf (k - m p) / (p (1 - p))), d/dK by the likelihood ratio with the
+ * leave-one-out baseline of each burst's R resamples. */
+void or_rb_grad(const or_data *d, const or_params *p, int64_t b0, int64_t b1, int R,
+                uint32_t seed, uint32_t set, int B, const double *g, double *M, double *grad) {
+    static double X[OR_RB_MAXC + 1], Y[OR_RB_MAXC + 1], dY[OR_RB_MAXC + 1], T[OR_RB_MAXC + 1];
+    static double D[OR_MAXN][OR_RB_MAXC + 1];
+    const int n = p->n, K = n * (n - 1);
+    for (int i = 0; i < n + K; ++i) grad[i] = 0.0;
+    for (int i = 0; i < B; ++i) M[i] = 0.0;
+    for (int64_t b = b0; b < b1; ++b) {
+        double H = 0.0, Sg[OR_MAXN * (OR_MAXN - 1)] = {0}, Ac[OR_MAXN * (OR_MAXN - 1)] = {0};
+        for (int r = 0; r < R; ++r) {
+            int32_t cnt[OR_MAXN];
+            float F;
+            double S[OR_MAXN * (OR_MAXN - 1)];
+            or_rb_unit(d, p, b, (uint32_t)r, seed, set, cnt, &F, S, NULL);
+            int xl = 1;
+            X[0] = 1.0;
+            unsigned dmask = 0;
+            for (int s = 0; s < n; ++s) {
+                const int m = cnt[s];
+                if (m == 0) continue;
+                const float psf = state_p(p, s);
+                const double ps = (double)psf;
+                for (int k = 0; k <= m; ++k) {
+                    Y[k] = binom_pmf(m, k, psf);
+                    dY[k] = Y[k] * ((double)k - (double)m * ps) / (ps * (1.0 - ps));
+                }
+                const int nl = xl + m;
+                /* D_s = X (*) dY; D_t = D_t (*) Y; X = X (*) Y */
+                for (int a = 0; a < nl; ++a) {
+                    int k0 = a - (xl - 1);
+                    if (k0 < 0) k0 = 0;
+                    const int k1 = a < m ? a : m;
+                    double z = 0.0;
+                    for (int k = k0; k <= k1; ++k) z += dY[k] * X[a - k];
+                    T[a] = z;
+                }
+                for (int t = 0; t < n; ++t) {
+                    if (!(dmask & (1u << t))) continue;
+                    double tmp[OR_RB_MAXC + 1];
+                    for (int a = 0; a < nl; ++a) {
+                        int k0 = a - (xl - 1);
+                        if (k0 < 0) k0 = 0;
+                        const int k1 = a < m ? a : m;
+                        double z = 0.0;
+                        for (int k = k0; k <= k1; ++k) z += Y[k] * D[t][a - k];
+                        tmp[a] = z;
+                    }
+                    for (int a = 0; a < nl; ++a) D[t][a] = tmp[a];
+                }
+                for (int a = 0; a < nl; ++a) D[s][a] = T[a];
+                for (int a = 0; a < nl; ++a) {
+                    int k0 = a - (xl - 1);
+                    if (k0 < 0) k0 = 0;
+                    const int k1 = a < m ? a : m;
+                    double z = 0.0;
+                    for (int k = k0; k <= k1; ++k) z += Y[k] * X[a - k];
+                    T[a] = z;
+                }
+                for (int a = 0; a < nl; ++a) X[a] = T[a];
+                xl = nl;
+                dmask |= 1u << s;
+            }
+            double h = 0.0;
+            for (int a = 0; a < xl; ++a) {
+                const int bin = rb_bin(a, F, B);
+                if (bin < 0) continue;
+                M[bin] += X[a];
+                h += g[bin] * X[a];
+                for (int t = 0; t < n; ++t)
+                    if (dmask & (1u << t)) grad[t] += g[bin] * D[t][a];
+            }
+            H += h;
+            for (int i = 0; i < K; ++i) {
+                Sg[i] += S[i];
+                Ac[i] += h * S[i];
+            }
+        }
+        if (R > 1)
+            for (int i = 0; i < K; ++i)
+                grad[n + i] += ((double)R / (double)(R - 1)) * (Ac[i] - H * Sg[i] / (double)R);
+    }
 }

@@ -95,6 +95,18 @@ void or_hist(const float *x, int64_t n, int B, int64_t *hist);
 void or_chi2(const float *O, const int64_t *M, int B, int left_pad, int R, int nstates,
              float *chi2_ref, double *chi2_64, int *effN);
 
+/* Rao-Blackwellised form (see pda_oracle.c): per-unit walk summary, the
+ * fixed-point expected histogram (float path, bit-exact with the kernels)
+ * and the FP64 expected histogram with the gradient of sum_b g_b M_b. */
+int or_rb_unit(const or_data *d, const or_params *p, int64_t burst, uint32_t r, uint32_t seed,
+               uint32_t set, int32_t *cnt, float *F, double *score, int32_t *ntrans);
+void or_rb_run(const or_data *d, const or_params *p, int64_t b0, int64_t b1, int R,
+               uint32_t seed, uint32_t set, int B, uint64_t *M);
+void or_rb_grad(const or_data *d, const or_params *p, int64_t b0, int64_t b1, int R,
+                uint32_t seed, uint32_t set, int B, const double *g, double *M, double *grad);
+const double *or_log_factorials(void);
+float or_state_efficiency(float E, float r0);
+
 #ifdef __cplusplus
 }
 #endif

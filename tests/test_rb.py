@@ -1,0 +1,201 @@
+"""GPU: the Rao-Blackwellised (RB) forward, the adjoint kernel and the torch
+autograd Function, through the C ABI.
+
+* Forward: the 2^-32 fixed-point expected histogram is bit-exact against the
+  oracle's float path (2-4 states, background on and off, burst ranges), and
+  it is the conditional expectation of the plain simulator's histogram given
+  the same walks (the plain run with the same keys differs from it by
+  binomial noise only; a 5 % efficiency error is rejected).
+* Adjoint: equal to the oracle's FP64 gradient for the same cotangent within
+  1e-3 relative; the efficiency gradient equals central finite differences
+  of the GPU forward (h = 1e-4 relative) within 1e-3; the rate gradient
+  agrees with central differences over independent seeds.
+* torch: the autograd Function's chi2 and gradient match the library's.
+"""
+import numpy as np
+import pytest
+
+from gsmfretda_amd import pda, synth
+
+pytestmark = pytest.mark.gpu
+
+PH = synth.PHOTOPHYSICS
+ARGS3 = synth.make_args([0.25, 0.5, 0.8], [800, 2000, 600, 2500, 1200, 4000])
+
+
+def _data(n_bursts, seed, bg=True):
+    args = synth.make_args([0.2, 0.55, 0.85], [900, 2500, 700, 3000, 1500, 5000])
+    d = synth.generate(n_bursts, args, 3, seed=seed)
+    if not bg:
+        d.bg_ad = d.bg_dd = 0.0
+    return d
+
+
+def _load(m, d, obs=None, B=50):
+    if obs is None:
+        obs = np.random.default_rng(1).uniform(0, 1, d.nbursts).astype(np.float32)
+    m.load(d, SgDivSr=obs, nbins=B, left_pad=2)
+    m.set_photophysics(0, *PH)
+    m.set_burst_range(0, -1, 0)
+    return obs
+
+
+def _obs_hist(obs, B=50):
+    """The library's binning of SgDivSr: floor(double(x) * B) on [0, 1)."""
+    x = obs[(obs >= 0) & (obs < 1)].astype(np.float64)
+    return np.bincount(np.floor(x * B).astype(np.int64), minlength=B)[:B].astype(np.float64)
+
+
+def _rb(m, n, args, R, seed, set_id=0, scores=True):
+    assert m.set_params(n, 0, np.asarray(args, np.float32))
+    m.rb_forward(R=R, seed=seed, set_id=set_id, scores=scores)
+    return m.rb_result()
+
+
+@pytest.mark.parametrize("n,bg,rng_", [(2, True, (0, -1)), (3, True, (5, 170)), (3, False, (0, -1)),
+                                       (4, True, (0, -1))])
+def test_rb_forward_bit_exact(gpu_mc, oracle, n, bg, rng_):
+    d = _data(200, 31 + n, bg)
+    _load(gpu_mc, d)
+    rng = np.random.default_rng(n)
+    if n == 3:
+        args = ARGS3
+    else:
+        E = np.sort(rng.uniform(0.05, 0.95, n))
+        args = synth.make_args(E, synth.loguniform_rates(rng, n * (n - 1), 3e2, 8e3))
+    gpu_mc.set_burst_range(rng_[0], rng_[1], 0)
+    M, fixed, c2, eff = _rb(gpu_mc, n, args, 12, 77, set_id=3)
+    od = oracle.Data.from_bursts(d)
+    p = oracle.make_params(n, np.asarray(args, np.float32), *PH)
+    b1 = d.nbursts if rng_[1] < 0 else rng_[1]
+    want = oracle.rb_run(od, p, 12, 77, 3, b0=rng_[0], b1=b1)
+    assert np.array_equal(fixed, want), np.nonzero(fixed != want)
+    gpu_mc.set_burst_range(0, -1, 0)
+
+
+def test_rb_is_the_expectation_of_the_plain_histogram(gpu_mc):
+    d = _data(3000, 41)
+    _load(gpu_mc, d)
+    R, seed = 64, 5
+    M, _, _, _ = _rb(gpu_mc, 3, ARGS3, R, seed, set_id=0)
+    H, _, _, _ = pda.evaluate(gpu_mc, 3, ARGS3, R, seed, set_id=0)
+    keep = M > 5
+    stat = np.sum((H[keep] - M[keep]) ** 2 / M[keep])
+    # given the walks the plain histogram is a sum of independent per-unit
+    # outcomes with mean M: E[stat] <= bins kept
+    assert stat < keep.sum() + 6 * np.sqrt(2 * keep.sum()), stat
+    bad = np.array(ARGS3, np.float32)
+    bad[:3] *= 1.05
+    Mb, _, _, _ = _rb(gpu_mc, 3, bad, R, seed, set_id=0)
+    stat_b = np.sum((H[keep] - Mb[keep]) ** 2 / np.maximum(Mb[keep], 1e-9))
+    assert stat_b > 5 * (keep.sum() + 6 * np.sqrt(2 * keep.sum())), stat_b
+
+
+def _chi2_cotangent(M, obs_hist, R, n, left_pad=2):
+    """d chi2 / d M_b of the reduced chi2 on expected counts."""
+    m = M / R
+    keep = (np.arange(len(M)) >= left_pad) & (M > 0)
+    dof = keep.sum() - n * (n + 1)
+    g = np.zeros_like(M)
+    g[keep] = (1.0 - obs_hist[keep] ** 2 / m[keep] ** 2) / (R * dof)
+    return g
+
+
+def test_rb_adjoint_matches_fp64_oracle(gpu_mc, oracle):
+    d = _data(400, 51)
+    obs = _load(gpu_mc, d)
+    R, seed = 16, 8
+    M, _, _, _ = _rb(gpu_mc, 3, ARGS3, R, seed, set_id=1)
+    oh = _obs_hist(obs)
+    g = _chi2_cotangent(M, oh, R, 3)
+    grad = gpu_mc.rb_adjoint(g, 3)
+    od = oracle.Data.from_bursts(d)
+    Mo, go = oracle.rb_grad(od, oracle.make_params(3, ARGS3, *PH), R, seed, g, set_id=1)
+    assert np.allclose(M, Mo, rtol=1e-5, atol=1e-6)
+    scale = np.abs(go).max()
+    assert np.all(np.abs(grad[:9] - go) <= 1e-3 * np.maximum(np.abs(go), 1e-3 * scale)), (grad, go)
+    assert np.all(grad[9:] == 0.0)
+
+
+def test_rb_efficiency_gradient_matches_central_differences(gpu_mc, oracle):
+    d = _data(20000, 61)
+    obs = _load(gpu_mc, d)
+    oh = _obs_hist(obs)
+    R, seed = 32, 13
+    M, _, c2, _ = _rb(gpu_mc, 3, ARGS3, R, seed)
+    g = _chi2_cotangent(M, oh, R, 3)
+    grad = gpu_mc.rb_adjoint(g, 3)
+    a0 = np.asarray(ARGS3, np.float32)
+    for s in range(3):
+        h = np.float32(1e-4) * a0[s]
+        ap, am = a0.copy(), a0.copy()
+        ap[s] += h
+        am[s] -= h
+        _, _, cp, _ = _rb(gpu_mc, 3, ap, R, seed)
+        _, _, cm, _ = _rb(gpu_mc, 3, am, R, seed)
+        # the library differentiates in the acceptor probability p(E), E after
+        # the float distance round trip: divide by the change of p
+        pp = oracle.lib().or_state_efficiency(float(ap[s]), PH[3])
+        pm = oracle.lib().or_state_efficiency(float(am[s]), PH[3])
+        fd = (cp - cm) / (np.float64(np.float32(pp)) - np.float64(np.float32(pm)))
+        assert abs(grad[s] - fd) <= 1e-3 * abs(fd), (s, grad[s], fd)
+
+
+def test_rb_rate_gradient_agrees_with_central_differences(gpu_mc):
+    d = _data(4000, 71)
+    obs = _load(gpu_mc, d)
+    oh = _obs_hist(obs)
+    R = 32
+    a0 = np.asarray(ARGS3, np.float32)
+    M0, _, _, _ = _rb(gpu_mc, 3, a0, R, 1000)
+    g = _chi2_cotangent(M0, oh, R, 3)
+    lr, fd = [], []
+    for seed in range(40):
+        M, _, _, _ = _rb(gpu_mc, 3, a0, R, 2000 + seed)
+        lr.append(gpu_mc.rb_adjoint(g, 3)[3:9])
+        row = []
+        for i in range(6):
+            h = np.float32(0.05) * a0[3 + i]
+            ap, am = a0.copy(), a0.copy()
+            ap[3 + i] += h
+            am[3 + i] -= h
+            Mp, _, _, _ = _rb(gpu_mc, 3, ap, R, 2000 + seed, scores=False)
+            Mm, _, _, _ = _rb(gpu_mc, 3, am, R, 2000 + seed, scores=False)
+            row.append(g @ (Mp - Mm) / (np.float64(ap[3 + i]) - np.float64(am[3 + i])))
+        fd.append(row)
+    lr, fd = np.array(lr), np.array(fd)
+    diff = lr - fd
+    z = diff.mean(0) / (diff.std(0, ddof=1) / np.sqrt(len(diff)))
+    assert np.all(np.abs(z) < 4.0), (lr.mean(0), fd.mean(0), z)
+
+
+def test_torch_autograd_function(gpu_mc):
+    torch = pytest.importorskip("torch")
+    from gsmfretda_amd import rb_torch
+    d = _data(1500, 81)
+    obs = _load(gpu_mc, d)
+    oh = _obs_hist(obs)
+    R, seed = 16, 4
+    args = torch.tensor(np.asarray(ARGS3, np.float64), requires_grad=True)
+    loss = rb_torch.rb_chi2(args, gpu_mc, 3, R, seed, torch.tensor(oh))
+    loss.backward()
+    M, _, c2, _ = gpu_mc.rb_result()
+    assert abs(loss.item() - c2) <= 1e-12 * abs(c2)
+    g = _chi2_cotangent(M, oh, R, 3)
+    # the adjoint of this same forward, called directly
+    want = gpu_mc.rb_adjoint(g, 3)
+    assert np.allclose(args.grad.numpy(), want, rtol=1e-9, atol=0), (args.grad, want)
+
+
+def test_rb_errors(gpu_mc):
+    d = _data(50, 91)
+    _load(gpu_mc, d)
+    with pytest.raises(pda.PdaError, match="v = 0"):
+        a = np.array(ARGS3, np.float32)
+        a[9] = 2.0
+        _rb(gpu_mc, 3, a, 4, 1)
+    with pytest.raises(pda.PdaError, match="n <= 4"):
+        _rb(gpu_mc, 5, synth.make_args(np.linspace(0.1, 0.9, 5), np.full(20, 1e3)), 4, 1)
+    _rb(gpu_mc, 3, ARGS3, 4, 1, scores=False)
+    with pytest.raises(pda.PdaError, match="scores"):
+        gpu_mc.rb_adjoint(np.zeros(50), 3)
