@@ -1,0 +1,238 @@
+"""Fitting loops over the MI355X PDA path: the parameter server's side.
+
+The reference's optimiser is pySMFRETda, an external parameter server that
+hands a genetic algorithm's individuals to the GPU workers one at a time and
+reads their chi2 back (src/streamWorker.cpp:83-182; protocol in
+3rdparty/protobuf/args.proto).  This module provides that loop against the
+C ABI in three forms:
+
+* `GeneticFitter(batched=False)` -- one evaluation per individual through
+  pda_run (the reference's one-set-per-call pattern);
+* `GeneticFitter(batched=True)` -- a generation per pda_run_population call
+  (one graph replay);
+* `AdamFitter` -- Adam on (logit E, log K) with the gradient of the
+  Rao-Blackwellised chi2 from the adjoint kernel (gsmfretda_amd.rb_torch).
+
+Every fitter reports progress in the same currency, the plain simulator's
+chi2 at the current best parameters with fixed random streams (seed, set id
+0), so the wall time each needs to reach a chi2 target can be compared.
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import pda
+
+LOGIT_E_BOUNDS = (-6.0, 6.0)
+LOG_K_BOUNDS = (math.log(1e1), math.log(1e7))
+
+
+def theta_to_args(theta, n):
+    """(logit E (n), log K (n (n - 1))) -> mc::set_params args with v = 0."""
+    theta = np.asarray(theta, np.float64)
+    E = 1.0 / (1.0 + np.exp(-theta[:n]))
+    K = np.exp(theta[n:])
+    return np.concatenate([E, K, np.zeros(n)]).astype(np.float32)
+
+
+def args_to_theta(args, n):
+    args = np.asarray(args, np.float64)
+    E = np.clip(args[:n], 1e-6, 1 - 1e-6)
+    return np.concatenate([np.log(E / (1 - E)), np.log(args[n:n * n])])
+
+
+def rate_index(n, s, j):
+    """Position in the off-diagonal rate list of K(j, s), the rate s -> j
+    (genMatK's row-major order, src/eigenhelper.cpp:4-21)."""
+    return j * (n - 1) + (s if s < j else s - 1)
+
+
+def sort_states(theta, n):
+    """Relabels the states by increasing E (the model is symmetric under a
+    permutation of its states), carrying the rates along."""
+    theta = np.asarray(theta, np.float64)
+    order = np.argsort(theta[:n], kind="stable")
+    if np.all(order == np.arange(n)):
+        return theta.copy()
+    inv = np.empty(n, np.int64)
+    inv[order] = np.arange(n)
+    out = theta.copy()
+    out[:n] = theta[:n][order]
+    for s in range(n):
+        for j in range(n):
+            if s != j:
+                out[n + rate_index(n, inv[s], inv[j])] = theta[n + rate_index(n, s, j)]
+    return out
+
+
+def clip_theta(theta, n):
+    theta = np.asarray(theta, np.float64).copy()
+    theta[:n] = np.clip(theta[:n], *LOGIT_E_BOUNDS)
+    theta[n:] = np.clip(theta[n:], *LOG_K_BOUNDS)
+    return theta
+
+
+@dataclass
+class Problem:
+    """A loaded data set, its observed histogram and the fixed random streams."""
+    mc: pda.MC
+    n: int
+    R: int
+    seed: int
+    observed: np.ndarray          # observed histogram (float64, B bins)
+    evaluations: int = 0
+
+    def chi2(self, theta):
+        """Plain-simulator chi2 of one parameter set (pda_run, set id 0)."""
+        self.evaluations += 1
+        _, c64, _, _ = pda.evaluate(self.mc, self.n, theta_to_args(theta, self.n), self.R,
+                                    self.seed, set_id=0)
+        return c64
+
+    def chi2_population(self, thetas):
+        """Plain-simulator chi2 of a generation in one population run; every
+        set uses set id 0, the streams of `chi2`."""
+        self.evaluations += len(thetas)
+        self.mc.set_population([self.n] * len(thetas),
+                               [theta_to_args(t, self.n) for t in thetas])
+        self.mc.run_population(R=self.R, seed=self.seed, set_base=0, set_stride=0)
+        _, c64, _, _ = self.mc.get_population_result()
+        return c64
+
+
+@dataclass
+class History:
+    method: str
+    times: list = field(default_factory=list)     # wall seconds since start
+    chi2: list = field(default_factory=list)      # best plain chi2 so far
+    evals: list = field(default_factory=list)
+    best_theta: np.ndarray = None
+
+    def record(self, t, c, e, theta):
+        if not self.chi2 or c < self.chi2[-1]:
+            self.best_theta = np.array(theta)
+        best = c if not self.chi2 else min(c, self.chi2[-1])
+        self.times.append(t)
+        self.chi2.append(best)
+        self.evals.append(e)
+
+    def time_to(self, target):
+        for t, c in zip(self.times, self.chi2):
+            if c <= target:
+                return t
+        return None
+
+    def as_dict(self):
+        return dict(method=self.method, times=self.times, chi2=self.chi2, evals=self.evals,
+                    best_theta=None if self.best_theta is None else self.best_theta.tolist())
+
+
+class GeneticFitter:
+    """Real-coded genetic algorithm: tournament selection, blend crossover in
+    (logit E, log K), Gaussian mutation with a decaying scale, elitism; the
+    states of every individual kept ordered by E."""
+
+    def __init__(self, batched, pop_size=64, elite=2, tournament=3, blend=0.3, sigma0=0.3,
+                 sigma_decay=0.97, sigma_min=0.01, rng=None):
+        self.batched = batched
+        self.pop_size = pop_size
+        self.elite = elite
+        self.tournament = tournament
+        self.blend = blend
+        self.sigma = sigma0
+        self.sigma_decay = sigma_decay
+        self.sigma_min = sigma_min
+        self.rng = rng if rng is not None else np.random.default_rng(0)
+
+    def _evaluate(self, prob, pop):
+        if self.batched:
+            return np.asarray(prob.chi2_population(pop))
+        return np.array([prob.chi2(t) for t in pop])
+
+    def _child(self, pop, fit, n):
+        rng = self.rng
+
+        def pick():
+            idx = rng.integers(0, len(pop), self.tournament)
+            return pop[idx[np.argmin(fit[idx])]]
+        a, b = pick(), pick()
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        span = hi - lo
+        child = rng.uniform(lo - self.blend * span, hi + self.blend * span)
+        child = child + rng.normal(0.0, self.sigma, size=child.shape) * (rng.random(child.shape) < 0.5)
+        return sort_states(clip_theta(child, n), n)
+
+    def run(self, prob, pop0, max_seconds, target=None, max_generations=100000):
+        n = prob.n
+        hist = History("ga-batched" if self.batched else "ga-per-set")
+        t0 = time.perf_counter()
+        pop = [sort_states(clip_theta(t, n), n) for t in pop0]
+        fit = self._evaluate(prob, pop)
+        i = int(np.argmin(fit))
+        hist.record(time.perf_counter() - t0, float(fit[i]), prob.evaluations, pop[i])
+        for _ in range(max_generations):
+            if target is not None and hist.chi2[-1] <= target:
+                break
+            if time.perf_counter() - t0 > max_seconds:
+                break
+            order = np.argsort(fit)
+            new = [pop[k] for k in order[:self.elite]]
+            while len(new) < self.pop_size:
+                new.append(self._child(pop, fit, n))
+            pop = new
+            fit = self._evaluate(prob, pop)
+            i = int(np.argmin(fit))
+            hist.record(time.perf_counter() - t0, float(fit[i]), prob.evaluations, pop[i])
+            self.sigma = max(self.sigma_min, self.sigma * self.sigma_decay)
+        return hist
+
+
+class AdamFitter:
+    """Adam on (logit E, log K) with the adjoint gradient of the
+    Rao-Blackwellised chi2; progress measured by the plain chi2 every
+    `check_every` steps."""
+
+    def __init__(self, lr=0.05, betas=(0.9, 0.999), check_every=1, lr_decay=1.0, R_grad=None):
+        self.lr = lr
+        self.betas = betas
+        self.check_every = check_every
+        self.lr_decay = lr_decay
+        self.R_grad = R_grad  # resamples of the RB gradient (default: the problem's R)
+
+    def run(self, prob, theta0, max_seconds, target=None, max_steps=100000):
+        import torch
+
+        from . import rb_torch
+        n = prob.n
+        hist = History("adam-rb")
+        t0 = time.perf_counter()
+        theta = torch.tensor(clip_theta(theta0, n), dtype=torch.float64, requires_grad=True)
+        opt = torch.optim.Adam([theta], lr=self.lr, betas=self.betas)
+        sched = torch.optim.lr_scheduler.ExponentialLR(opt, self.lr_decay)
+        obs = torch.tensor(prob.observed, dtype=torch.float64)
+        c = prob.chi2(theta.detach().numpy())
+        hist.record(time.perf_counter() - t0, c, prob.evaluations, theta.detach().numpy())
+        for step in range(max_steps):
+            if target is not None and hist.chi2[-1] <= target:
+                break
+            if time.perf_counter() - t0 > max_seconds:
+                break
+            opt.zero_grad()
+            E = torch.sigmoid(theta[:n])
+            K = torch.exp(theta[n:])
+            args = torch.cat([E, K, torch.zeros(n, dtype=torch.float64)])
+            loss = rb_torch.rb_chi2(args, prob.mc, n, self.R_grad or prob.R, prob.seed, obs,
+                                    set_id=0)
+            loss.backward()
+            prob.evaluations += 1
+            opt.step()
+            sched.step()
+            with torch.no_grad():
+                theta[:n].clamp_(*LOGIT_E_BOUNDS)
+                theta[n:].clamp_(*LOG_K_BOUNDS)
+            if (step + 1) % self.check_every == 0:
+                th = theta.detach().numpy()
+                hist.record(time.perf_counter() - t0, prob.chi2(th), prob.evaluations, th)
+        return hist

@@ -1,0 +1,117 @@
+"""Fit wall-time study on one GPU: gradient + batch against plain evaluation.
+
+Workload (fast dynamics): 4 states, true rates log-uniform in [1e3, 1e5]
+s^-1, 1,000,000 synthetic bursts of 30 + Geom(mean 80) photons and
+LogNormal(2 ms, 0.6) durations, 32 resamples per burst, 50 bins.  The
+observed histogram is the simulator at the true parameters with R = 1 and a
+held-out seed; chi2(truth) is the plain simulator's chi2 at the true
+parameters with the evaluation streams.  From the same initial population of
+64 random parameter sets (states from the fitted model's range), three
+fitters run under the same wall-clock limit:
+
+  ga-per-set   genetic algorithm, one pda_run per individual (the reference's
+               one-set-per-call evaluation)
+  ga-batched   the same algorithm, one pda_run_population per generation
+  adam-rb      Adam on (logit E, log K) with the adjoint gradient of the
+               Rao-Blackwellised chi2 (8 resamples per burst), started from
+               the best initial set found by one batched evaluation
+
+Each is timed to reach chi2 <= 1.01 chi2(truth), and to reach within 1 % of
+the chi2 the per-set genetic algorithm ends with.
+
+    python -m gsmfretda_amd.fit_study [--bursts N] [--seconds S] [--out FILE]
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import fit, pda, synth
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--bursts", type=int, default=1_000_000)
+    ap.add_argument("--seconds", type=float, default=60.0, help="wall limit per fitter")
+    ap.add_argument("--pop", type=int, default=64)
+    ap.add_argument("--lr", type=float, default=0.1)
+    ap.add_argument("--r-grad", type=int, default=8,
+                    help="resamples per burst of the RB gradient (the plain chi2 keeps R = 32)")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+
+    def log(*x):
+        print(*x, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    cfg = synth.config(4, n_bursts=a.bursts)
+    d, n, R = cfg["data"], cfg["n"], cfg["R"]
+    truth = np.asarray(cfg["args"], np.float32)
+    log(f"data: {d.nbursts} bursts, {d.nphotons} photons, {time.perf_counter() - t0:.1f} s")
+    m = pda.MC(0, 1)
+    m.load(d, SgDivSr=None)
+    m.set_observed_hist(np.zeros(50, np.float32))
+    m.set_photophysics(0, *cfg["photophysics"])
+    assert m.set_params(n, 0, truth)
+    m.set_burst_range(0, -1, 0)
+    m.reSampleTimes = 1
+    m.run_kernel(sid=0, seed=cfg["seeds"]["observed"], samples=True)
+    m.synchronize(0)
+    obs_E, _, _ = m.get_samples(0, d.nbursts)
+    m.set_observed(obs_E, 50, 2)
+    x = obs_E[(obs_E >= 0) & (obs_E < 1)].astype(np.float64)
+    observed = np.bincount(np.floor(x * 50).astype(np.int64), minlength=50)[:50].astype(np.float64)
+
+    prob = fit.Problem(m, n, R, cfg["seeds"]["mc"], observed)
+    chi2_truth = prob.chi2(fit.args_to_theta(truth, n))
+    target = 1.01 * chi2_truth
+    log(f"chi2(truth) = {chi2_truth:.5f}, target {target:.5f}")
+    pop0 = [fit.args_to_theta(args, n) for _, args in cfg["population"][:a.pop]]
+
+    results = {}
+    for name, make in (("ga-per-set", lambda: fit.GeneticFitter(batched=False, pop_size=a.pop,
+                                                                 rng=np.random.default_rng(7))),
+                       ("ga-batched", lambda: fit.GeneticFitter(batched=True, pop_size=a.pop,
+                                                                 rng=np.random.default_rng(7)))):
+        prob.evaluations = 0
+        h = make().run(prob, pop0, a.seconds, target=target)
+        results[name] = h
+        log(f"{name}: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s, "
+            f"{prob.evaluations} evaluations, {len(h.times)} generations")
+
+    # Adam from the best initial set (found by one batched evaluation, timed)
+    prob.evaluations = 0
+    ts = time.perf_counter()
+    fit0 = prob.chi2_population(pop0)
+    start = pop0[int(np.argmin(fit0))]
+    t_init = time.perf_counter() - ts
+    h = fit.AdamFitter(lr=a.lr, R_grad=a.r_grad, check_every=5).run(prob, start,
+                                                                     a.seconds - t_init,
+                                                                     target=target)
+    h.times = [t + t_init for t in h.times]
+    results["adam-rb"] = h
+    log(f"adam-rb: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s, "
+        f"{prob.evaluations} evaluations")
+
+    ga_final = results["ga-per-set"].chi2[-1]
+    out = {"workload": "4 states, rates 1e3-1e5, %d bursts x %d resamples, 50 bins" % (d.nbursts, R),
+           "chi2_truth": chi2_truth, "target_truth": target, "ga_per_set_final": ga_final,
+           "seconds_limit": a.seconds,
+           "time_to_truth_target": {k: v.time_to(target) for k, v in results.items()},
+           "time_to_ga_final": {k: v.time_to(1.01 * ga_final) for k, v in results.items()},
+           "final_chi2": {k: v.chi2[-1] for k, v in results.items()},
+           "rb_ms": dict(zip(("walk", "forward", "adjoint"), m.rb_stats())),
+           "truth": truth.tolist(),
+           "histories": {k: v.as_dict() for k, v in results.items()}}
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(out, f)
+    short = {k: v for k, v in out.items() if k != "histories"}
+    print(json.dumps(short), flush=True)
+    m.close()
+
+
+if __name__ == "__main__":
+    main()
