@@ -207,10 +207,12 @@ hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, c
                           const uint2* links, const ModelBlob* blob, const RbModel* rbm,
                           const RunArgs& a, uint2* cnt, float* F, float* S, uint32_t* work,
                           hipStream_t stream);
-hipError_t launch_rb_pmf(bool grad_mode, uint32_t grid, int cap, const RbModel* rbm,
-                         const double* lf, const RunArgs& a, const uint2* cnt, const float* F,
-                         const float* S, const double* g, unsigned long long* hist_acc,
-                         double* grad, uint32_t* work, hipStream_t stream);
+size_t rb_pmf_lds(int B, int cap_big);
+hipError_t launch_rb_pmf(bool grad_mode, uint32_t grid, int cap_big, const RbModel* rbm,
+                         const double* lf, const RunArgs& a, const BurstDesc* bursts,
+                         const uint2* cnt, const float* F, const float* S, const double* g,
+                         unsigned long long* hist_acc, double* grad, uint32_t* work,
+                         hipStream_t stream);
 hipError_t launch_rb_finish(const RunArgs& a, const float* obs, unsigned long long* hist_acc,
                             uint32_t* work, void* res, hipStream_t stream);
 

@@ -92,6 +92,7 @@ void build_rb_model(const Model& M, RbModel& R, bool* rates_positive, bool* p_in
     const int n = M.n;
     R.n = n;
     R.K = n * (n - 1);
+    R.coef = std::max({M.gmx, M.omdd, 0.0f});
     *rates_positive = true;
     *p_interior = true;
     for (int s = 0; s < n; ++s) {
@@ -149,15 +150,14 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     // when c <= 1
     uint32_t maxP = 0;
     for (int64_t b = b0; b < b1; ++b) maxP = std::max(maxP, ctx->h_P[size_t(b)]);
-    const double coef = std::max({double(M.gmx), double(M.omdd), 0.0});
-    const double bound = coef <= 1.0 ? double(maxP) : std::ceil((coef + 0.5) * double(maxP));
-    if (bound > double(kRbMaxCount))
+    const float coef = std::max({M.gmx, M.omdd, 0.0f});
+    const int bound = rb_total_bound(maxP, coef);
+    if (bound > kRbMaxCount)
         return fail(PDA_E_DATA, "the RB form supports acceptor totals up to %d (bursts of up to "
-                    "%u photons need %.0f)", kRbMaxCount, maxP, bound);
-    const int cap = int(bound) + 1;
+                    "%u photons need %d)", kRbMaxCount, maxP, bound);
+    const int cap = bound + 1;
     const bool scores = (flags & PDA_RB_SCORES) != 0;
-    const size_t lds_grad = sizeof(float) * size_t(cap) * 8u + 8u * kMaxBins + sizeof(RbModel);
-    if (lds_grad > ctx->lds_per_block)
+    if (rb_pmf_lds(ctx->B, cap) + sizeof(RbModel) > ctx->lds_per_block)
         return fail(PDA_E_DATA, "bursts of %u photons exceed the LDS of the RB kernels", maxP);
 
     // buffers of the per-unit summaries
@@ -215,7 +215,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, (uint64_t(units) + batch - 1) / batch)));
     a.batch = uint32_t(batch);
     a.queues = std::min<uint32_t>(ctx->shape.queues ? ctx->shape.queues : kMaxQueues, grid);
-    const int dyn = int(sizeof(float)) * cap * 3;
+    const int dyn = int(rb_pmf_lds(ctx->B, cap));
     HIP_TRY(rb_occupancy(2, dyn, &per_cu));
     per_cu = std::max(per_cu, 1);
     const uint32_t pgrid = uint32_t(std::max<int64_t>(
@@ -229,7 +229,8 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
                            r.d_work, s->stream));
     HIP_TRY(hipEventRecord(r.ev_walk, s->stream));
     HIP_TRY(launch_rb_pmf(false, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, a,
-                          r.d_cnt, r.d_F, r.d_S, nullptr, r.d_hist, nullptr, r.d_work, s->stream));
+                          ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, r.d_hist, nullptr,
+                          r.d_work, s->stream));
     HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s->stream));
     HIP_TRY(hipEventRecord(r.ev1, s->stream));
     HIP_TRY(hipMemcpyAsync(r.h_res, r.d_res, sizeof(RbResultHeaderH) + sizeof(uint64_t) * size_t(a.B),
@@ -299,7 +300,7 @@ int pda_rb_adjoint(pda_ctx* ctx, int sid, const double* g, int nbins, double* gr
     a.B = r.B;
     a.left_pad = r.left_pad;
     a.nstates = r.n;
-    const int dyn = int(sizeof(float)) * r.cap * 8;
+    const int dyn = int(rb_pmf_lds(r.B, r.cap));
     int per_cu = 0;
     HIP_TRY(rb_occupancy(3, dyn, &per_cu));
     per_cu = std::max(per_cu, 1);
@@ -311,7 +312,8 @@ int pda_rb_adjoint(pda_ctx* ctx, int sid, const double* g, int nbins, double* gr
                            s->stream));
     HIP_TRY(hipEventRecord(r.ev_a0, s->stream));
     HIP_TRY(launch_rb_pmf(true, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, a,
-                          r.d_cnt, r.d_F, r.d_S, r.d_g, nullptr, r.d_grad, r.d_work, s->stream));
+                          ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, r.d_g, nullptr, r.d_grad,
+                          r.d_work, s->stream));
     HIP_TRY(hipEventRecord(r.ev_a1, s->stream));
     HIP_TRY(hipMemcpyAsync(r.h_grad, r.d_grad, sizeof(double) * kGradLen, hipMemcpyDeviceToHost,
                            s->stream));

@@ -40,7 +40,19 @@ struct RbModel {
     int32_t pair[kRbMaxStates][kRbMaxStates];
     float two_over_k[kRbMaxStates][kRbMaxStates];
     float inv_sum[kRbMaxStates];
+    // acceptor totals of a unit are at most rb_total_bound(P) for a burst of
+    // P photons: c = max(gamma - crosstalk, 1 - dexdir, 0)
+    float coef;
+    int32_t pad1;
 };
+
+// Largest acceptor total of a unit in a burst of P photons: the windows
+// count disjoint photon runs and window w contributes at most
+// floor(c n_w + 1/2), which is <= n_w when c <= 1.
+PDA_HD inline int rb_total_bound(uint32_t P, float coef) {
+    if (coef <= 1.0f) return int(P);
+    return int(ceil((double(coef) + 0.5) * double(P)));
+}
 
 // Args index (after the n efficiencies) of the rate s -> j, K(j, s): row j,
 // column s of genMatK's row-major off-diagonal list (src/eigenhelper.cpp:4-21).

@@ -5,14 +5,17 @@
 //                    acceptor draw; per unit it stores the photon counts per
 //                    state, F, and (for rate gradients) the score of the
 //                    walk's draws
-//   rb_pmf_kernel    one wave per burst: for each of its units, the pmf of
-//                    the acceptor total (a convolution of one binomial per
-//                    state, in LDS, lanes over outcomes) spread into the E
-//                    histogram as 2^-32 fixed point -- or, for the adjoint,
-//                    the cotangent-weighted pmf and its derivatives
+//   rb_pmf_kernel    one wave per burst, 4 units at a time on groups of 16
+//                    lanes (the whole wave per unit for large bursts): the
+//                    pmf of each unit's acceptor total (a convolution of one
+//                    binomial per state, in LDS, lanes over outcomes) spread
+//                    into the E histogram as 2^-32 fixed point -- or, for the
+//                    adjoint, the cotangent-weighted pmf and its derivatives
 //   rb_finish_kernel reduces the histogram rows, computes chi2 on the
 //                    expected counts, resets the counters
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "pda_math.h"
 #include "pda_rb.h"
@@ -125,32 +128,172 @@ __global__ __launch_bounds__(kWave) void rb_walk_kernel(
     }
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
+// Sum over the L lanes of a lane group (every lane gets the total).
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, L);
     return v;
 }
 
-// Z[0 .. xl + ylen - 1) = Y (*) X, lanes over outputs.
-__device__ __forceinline__ void conv_into(float* Z, const float* Y, int ylen, const float* X,
-                                          int xl, uint32_t lane) {
-    const int nl = xl + ylen - 1;
-    for (int q = int(lane); q < nl; q += kWave) Z[q] = rb_conv_at(Y, ylen, X, xl, q);
+constexpr int kRbSmallLanes = 16;  // lanes per unit for small bursts (4 units per wave)
+constexpr int kRbSmallCap = 256;   // acceptor totals below this use the small layout
+
+// All R units of one burst, kWave / L at a time (one unit per group of L
+// lanes, lanes over acceptor totals).  Per group, LDS buffers X (the pmf so
+// far), Z (a free buffer) and Y (the next state's binomial pmf) of `cap`
+// floats each; the group's buffer roles are swapped instead of copied.
+//
+// Forward (kGrad = false): the unit's pmf P(a), a = 0..Mt, goes into the LDS
+// histogram at E = a / F in 2^-32 fixed point.
+//
+// Adjoint (kGrad = true): with G(a) the cotangent of a's bin, the unit
+// contributes h = sum_a G(a) P(a) and, per state s with m_s photons,
+//   d h / d p_s = m_s sum_c P_s(c) (G(c + 1) - G(c))
+// where P_s is the pmf with one trial of state s removed,
+// P(a) = q_s P_s(a) + p_s P_s(a - 1).  P_s follows from P by a first-order
+// recurrence run as a lane-parallel scan of affine maps, upwards when
+// p_s <= 1/2 and downwards otherwise (the direction in which the recurrence
+// damps rounding errors).  Per burst, the rate gradient is the likelihood
+// ratio R / (R - 1) sum_r (h_r - mean h) S_r.
+template <int L, bool kGrad>
+__device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __restrict__ lf,
+                                         const RunArgs& a, uint32_t bl,
+                                         const uint2* __restrict__ cnt_in,
+                                         const float* __restrict__ F_in,
+                                         const float* __restrict__ S_in, const double* s_g,
+                                         unsigned long long* s_h, float* s_buf, int cap,
+                                         uint32_t lane, double& gp_acc, double& gk_acc) {
+    constexpr int G = kWave / L;
+    const int g = int(lane) / L, lg = int(lane) % L;
+    const int n = sR.n, K = sR.K, B = a.B;
+    float* base = s_buf + g * 3 * cap;
+    double H = 0.0, Sg = 0.0, Ac = 0.0;  // lane lg of each group: rate pair lg
+    for (uint32_t r0 = 0; r0 < a.R; r0 += G) {
+        const uint32_t r = r0 + uint32_t(g);
+        const bool act = r < a.R;
+        const uint32_t u = bl * a.R + (act ? r : 0u);
+        const uint2 c = act ? cnt_in[u] : make_uint2(0u, 0u);
+        const float F = act ? F_in[u] : 0.0f;
+        int bX = 0, bZ = 1, bY = 2;
+        int xl = 1;
+        if (lg == 0) base[0] = 1.0f;
+#pragma unroll
+        for (int s = 0; s < kRbMaxStates; ++s) {
+            const int m = (act && s < n) ? int(rb_count(c, s)) : 0;
+            float* Y = base + bY * cap;
+            for (int k = lg; m > 0 && k <= m; k += L)
+                Y[k] = float(rb_binom_pmf(m, k, lf, sR.p[s], sR.lp[s], sR.lq[s]));
+            __syncthreads();
+            if (m > 0) {
+                if (xl == 1) {  // X = [1]: the product is Y itself, bit for bit
+                    const int t = bX;
+                    bX = bY;
+                    bY = t;
+                    xl = m + 1;
+                } else {
+                    const float* X = base + bX * cap;
+                    float* Z = base + bZ * cap;
+                    for (int q = lg; q < xl + m; q += L) Z[q] = rb_conv_at(Y, m + 1, X, xl, q);
+                    const int t = bX;
+                    bX = bZ;
+                    bZ = t;
+                    xl += m;
+                }
+            }
+            __syncthreads();
+        }
+        const float* P = base + bX * cap;
+        if (!kGrad) {
+            for (int q = lg; act && q < xl; q += L) {
+                const int bin = rb_bin(q, F, B);
+                if (bin >= 0) atomicAdd(&s_h[bin], (unsigned long long)rb_fixed(P[q]));
+            }
+        } else {
+            float* dG = base + bZ * cap;  // G(c + 1) - G(c)
+            double hp = 0.0;
+            for (int q = lg; q < xl; q += L) {
+                const int b0 = rb_bin(q, F, B);
+                const double G0 = b0 >= 0 ? s_g[b0] : 0.0;
+                hp += G0 * double(P[q]);
+                if (q + 1 < xl) {
+                    const int b1 = rb_bin(q + 1, F, B);
+                    dG[q] = float((b1 >= 0 ? s_g[b1] : 0.0) - G0);
+                }
+            }
+            __syncthreads();
+            const double h = group_sum<L>(hp);
+            const int Mt = xl - 1;
+#pragma unroll
+            for (int s = 0; s < kRbMaxStates; ++s) {
+                const int m = (act && s < n) ? int(rb_count(c, s)) : 0;
+                double part = 0.0;
+                if (m > 0) {  // uniform within the group
+                    const double p = double(sR.p[s]), q = 1.0 - p;
+                    const bool up = p <= 0.5;
+                    const double alpha = up ? -p / q : -q / p;
+                    const double inv = up ? 1.0 / q : 1.0 / p;
+                    double carry = 0.0;
+                    for (int j0 = 0; j0 < Mt; j0 += L) {
+                        const int j = j0 + lg;
+                        const bool v = j < Mt;
+                        double A = v ? alpha : 1.0;
+                        double Bv = v ? double(P[up ? j : Mt - j]) * inv : 0.0;
+#pragma unroll
+                        for (int d = 1; d < L; d <<= 1) {
+                            const double A2 = __shfl_up(A, d, L);
+                            const double B2 = __shfl_up(Bv, d, L);
+                            if (lg >= d) {
+                                Bv = A * B2 + Bv;
+                                A = A * A2;
+                            }
+                        }
+                        const double x = A * carry + Bv;  // P_s(up ? j : Mt - 1 - j)
+                        carry = __shfl(x, L - 1, L);
+                        if (v) part += x * double(dG[up ? j : Mt - 1 - j]);
+                    }
+                    part = group_sum<L>(part) * double(m);
+                }
+                if (lg == s) gp_acc += part;
+            }
+            if (act) {
+                H += h;
+                if (lg < K) {
+                    const double si = double(S_in[size_t(u) * size_t(K) + size_t(lg)]);
+                    Sg += si;
+                    Ac += h * si;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (kGrad) {
+#pragma unroll
+        for (int o = L; o < kWave; o <<= 1) {
+            H += __shfl_xor(H, o);
+            Sg += __shfl_xor(Sg, o);
+            Ac += __shfl_xor(Ac, o);
+        }
+        if (int(lane) < K && a.R > 1)
+            gk_acc += (double(a.R) / double(a.R - 1)) * (Ac - H * Sg / double(a.R));
+    }
 }
 
-// One wave per burst (taken from work[8]); its R units in order.  Dynamic
-// LDS: kGrad ? 8 : 3 buffers of `cap` floats (X, free, Y [, dY, D0..D3]).
+// One wave per burst (taken from work[8]).  Dynamic LDS: B 8-byte entries
+// (the fixed-point histogram, or the cotangent) followed by the float
+// buffers: max(4 groups x 3 x 256, 3 x cap_big).
 template <bool kGrad>
 __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
     const RbModel* __restrict__ rbm, const double* __restrict__ lf, const RunArgs a,
-    const uint2* __restrict__ cnt_in, const float* __restrict__ F_in,
-    const float* __restrict__ S_in, const double* __restrict__ g,
-    unsigned long long* __restrict__ hist_acc, double* __restrict__ grad,
-    uint32_t* __restrict__ work, int cap) {
-    extern __shared__ float s_buf[];
-    __shared__ unsigned long long s_h[kMaxBins];
-    __shared__ double s_g[kMaxBins];
+    const BurstDesc* __restrict__ bursts, const uint2* __restrict__ cnt_in,
+    const float* __restrict__ F_in, const float* __restrict__ S_in,
+    const double* __restrict__ g, unsigned long long* __restrict__ hist_acc,
+    double* __restrict__ grad, uint32_t* __restrict__ work, int cap_big) {
+    extern __shared__ double s_dyn[];
     __shared__ RbModel sR;
+    unsigned long long* s_h = reinterpret_cast<unsigned long long*>(s_dyn);
+    double* s_g = s_dyn;
+    float* s_buf = reinterpret_cast<float*>(s_dyn + a.B);
     const uint32_t lane = threadIdx.x;
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(rbm);
@@ -162,7 +305,6 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
         }
     }
     __syncthreads();
-    const int n = sR.n, K = sR.K, B = a.B;
     const uint32_t nb = a.nunits / a.R;
     double gp_acc = 0.0, gk_acc = 0.0;  // lane s: d/dp_s; lane i: d/dk_i
     while (true) {
@@ -170,108 +312,25 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
         if (lane == 0) bl = atomicAdd(&work[kRbPmfCounter], 1u);
         bl = __shfl(bl, 0);
         if (bl >= nb) break;
-        double H = 0.0, Sg = 0.0, Ac = 0.0;
-        for (uint32_t r = 0; r < a.R; ++r) {
-            const uint32_t u = bl * a.R + r;
-            const uint2 c = cnt_in[u];
-            const float F = F_in[u];
-            // buffer roles (wave-uniform indices into s_buf)
-            int bX = 0, bFree = 1, bY = 2, bdY = 3;
-            int bD[kRbMaxStates] = {4, 5, 6, 7};
-            uint32_t dmask = 0;
-            int xl = 1;
-            if (lane == 0) s_buf[bX * cap] = 1.0f;
-#pragma unroll
-            for (int s = 0; s < kRbMaxStates; ++s) {
-                const int m = s < n ? int(rb_count(c, s)) : 0;
-                if (m == 0) continue;
-                float* Y = s_buf + bY * cap;
-                float* dY = s_buf + bdY * cap;
-                for (int k = int(lane); k <= m; k += kWave) {
-                    const double b = rb_binom_pmf(m, k, lf, sR.p[s], sR.lp[s], sR.lq[s]);
-                    Y[k] = float(b);
-                    if (kGrad)
-                        dY[k] = float(b * ((double(k) - double(m) * double(sR.p[s])) * sR.dscale[s]));
-                }
-                __syncthreads();
-                if (xl == 1) {  // X = [1]: the product is Y itself
-                    const int t = bX;
-                    bX = bY;
-                    bY = t;
-                    if (kGrad) {
-                        const int t2 = bD[s];
-                        bD[s] = bdY;
-                        bdY = t2;
-                    }
-                    xl = m + 1;
-                    dmask |= 1u << s;
-                    continue;
-                }
-                const float* X = s_buf + bX * cap;
-                if (kGrad) {
-                    conv_into(s_buf + bD[s] * cap, dY, m + 1, X, xl, lane);  // D_s = X (*) dB_s
-                    for (int t = 0; t < kRbMaxStates; ++t) {
-                        if (!(dmask & (1u << t))) continue;
-                        conv_into(s_buf + bFree * cap, Y, m + 1, s_buf + bD[t] * cap, xl, lane);
-                        __syncthreads();
-                        const int t2 = bD[t];
-                        bD[t] = bFree;
-                        bFree = t2;
-                    }
-                }
-                conv_into(s_buf + bFree * cap, Y, m + 1, X, xl, lane);
-                __syncthreads();
-                const int t = bX;
-                bX = bFree;
-                bFree = t;
-                xl += m;
-                dmask |= 1u << s;
-            }
-            const float* P = s_buf + bX * cap;
-            if (!kGrad) {
-                for (int q = int(lane); q < xl; q += kWave) {
-                    const int bin = rb_bin(q, F, B);
-                    if (bin >= 0) atomicAdd(&s_h[bin], (unsigned long long)rb_fixed(P[q]));
-                }
-            } else {
-                double hp = 0.0, dp[kRbMaxStates] = {0.0, 0.0, 0.0, 0.0};
-                for (int q = int(lane); q < xl; q += kWave) {
-                    const int bin = rb_bin(q, F, B);
-                    const double G = bin >= 0 ? s_g[bin] : 0.0;
-                    hp += G * double(P[q]);
-#pragma unroll
-                    for (int t = 0; t < kRbMaxStates; ++t)
-                        if (dmask & (1u << t)) dp[t] += G * double(s_buf[bD[t] * cap + q]);
-                }
-                const double h = wave_sum(hp);
-#pragma unroll
-                for (int t = 0; t < kRbMaxStates; ++t) {
-                    if (!(dmask & (1u << t))) continue;
-                    const double v = wave_sum(dp[t]);
-                    if (int(lane) == t) gp_acc += v;
-                }
-                H += h;
-                if (int(lane) < K) {
-                    const double si = double(S_in[size_t(u) * size_t(K) + lane]);
-                    Sg += si;
-                    Ac += h * si;
-                }
-            }
-            __syncthreads();
-        }
-        // likelihood-ratio estimate with the leave-one-out baseline of the
-        // burst's other resamples: R / (R - 1) sum_r (h_r - mean h) S_r
-        if (kGrad && int(lane) < K && a.R > 1)
-            gk_acc += (double(a.R) / double(a.R - 1)) * (Ac - H * Sg / double(a.R));
+        const int bound = rb_total_bound(bursts[a.begin + bl].P, sR.coef);
+        if (bound < kRbSmallCap)
+            rb_burst<kRbSmallLanes, kGrad>(sR, lf, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf,
+                                           kRbSmallCap, lane, gp_acc, gk_acc);
+        else
+            rb_burst<kWave, kGrad>(sR, lf, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf, cap_big,
+                                   lane, gp_acc, gk_acc);
     }
     if (!kGrad) {
         __syncthreads();
         unsigned long long* row = hist_acc + size_t(blockIdx.x & 7u) * kMaxBins;
-        for (uint32_t i = lane; i < uint32_t(B); i += kWave)
+        for (uint32_t i = lane; i < uint32_t(a.B); i += kWave)
             if (s_h[i]) atomicAdd(&row[i], s_h[i]);
     } else {
-        if (int(lane) < n) atomicAdd(&grad[lane], gp_acc);
-        if (int(lane) < K) atomicAdd(&grad[n + int(lane)], gk_acc);
+        // every group's lane s holds a share of d/dp_s
+#pragma unroll
+        for (int o = kRbSmallLanes; o < kWave; o <<= 1) gp_acc += __shfl_xor(gp_acc, o);
+        if (int(lane) < sR.n) atomicAdd(&grad[lane], gp_acc);
+        if (int(lane) < sR.K) atomicAdd(&grad[sR.n + int(lane)], gk_acc);
     }
 }
 
@@ -342,17 +401,24 @@ hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, c
     return hipGetLastError();
 }
 
-hipError_t launch_rb_pmf(bool grad_mode, uint32_t grid, int cap, const RbModel* rbm,
-                         const double* lf, const RunArgs& a, const uint2* cnt, const float* F,
-                         const float* S, const double* g, unsigned long long* hist_acc,
-                         double* grad, uint32_t* work, hipStream_t stream) {
-    const size_t lds = sizeof(float) * size_t(cap) * (grad_mode ? 8u : 3u);
+size_t rb_pmf_lds(int B, int cap_big) {
+    const size_t floats = std::max<size_t>(size_t(kWave / kRbSmallLanes) * 3u * kRbSmallCap,
+                                           3u * size_t(cap_big));
+    return sizeof(double) * size_t(B) + sizeof(float) * floats;
+}
+
+hipError_t launch_rb_pmf(bool grad_mode, uint32_t grid, int cap_big, const RbModel* rbm,
+                         const double* lf, const RunArgs& a, const BurstDesc* bursts,
+                         const uint2* cnt, const float* F, const float* S, const double* g,
+                         unsigned long long* hist_acc, double* grad, uint32_t* work,
+                         hipStream_t stream) {
+    const size_t lds = rb_pmf_lds(a.B, cap_big);
     if (grad_mode)
         hipLaunchKernelGGL(rb_pmf_kernel<true>, dim3(grid), dim3(kWave), lds, stream, rbm, lf, a,
-                           cnt, F, S, g, hist_acc, grad, work, cap);
+                           bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
     else
         hipLaunchKernelGGL(rb_pmf_kernel<false>, dim3(grid), dim3(kWave), lds, stream, rbm, lf, a,
-                           cnt, F, S, g, hist_acc, grad, work, cap);
+                           bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
     return hipGetLastError();
 }
 
