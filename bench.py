@@ -108,6 +108,51 @@ def population_leg(pda, synth, steps, warmup):
             "chi2_median": float(np.median(c64))}
 
 
+def rb_leg(pda, synth, steps):
+    """Rao-Blackwellised differentiable form: 3 states (6 rates + 3 E),
+    100,000 bursts x 64 resamples per evaluation, expected histogram and chi2
+    (forward), then d chi2 / d(E, K) (adjoint)."""
+    cfg = synth.config(3)
+    d, R, n = cfg["data"], cfg["R"], cfg["n"]
+    m = pda.MC(0, 1)
+    m.load(d, SgDivSr=None)
+    m.set_observed_hist(np.zeros(50, np.float32))
+    m.set_photophysics(0, *cfg["photophysics"])
+    assert m.set_params(n, 0, cfg["args"])
+    m.set_burst_range(0, -1, 0)
+    m.reSampleTimes = 1
+    m.run_kernel(sid=0, seed=cfg["seeds"]["observed"], samples=True)
+    m.synchronize(0)
+    obs_E, _, _ = m.get_samples(0, d.nbursts)
+    m.set_observed(obs_E, 50, 2)
+    x = obs_E[(obs_E >= 0) & (obs_E < 1)].astype(np.float64)
+    O = np.bincount(np.floor(x * 50).astype(np.int64), minlength=50)[:50].astype(np.float64)
+    seed = cfg["seeds"]["mc"]
+    fwd, adj, walk = [], [], []
+    for i in range(steps + 1):
+        m.rb_forward(R=R, seed=seed, scores=True)
+        M, _, c2, _ = m.rb_result()
+        mm = M / R
+        keep = (np.arange(50) >= 2) & (M > 0)
+        g = np.zeros(50)
+        g[keep] = (1.0 - O[keep] ** 2 / mm[keep] ** 2) / (R * (keep.sum() - n * (n + 1)))
+        m.rb_adjoint(g, n)
+        w, f, a = m.rb_stats()
+        if i:
+            fwd.append(f)
+            adj.append(a)
+            walk.append(w)
+    m.close()
+    units = d.nbursts * R
+    return {"workload": "Rao-Blackwellised PDA, 3 states, 100000 bursts x 64 resamples, 50 bins: "
+                        "expected histogram + chi2, then d chi2 / d(E, K)",
+            "forward_units_per_s": units / (np.mean(fwd) * 1e-3),
+            "forward_gpu_ms": float(np.mean(fwd)), "walk_gpu_ms": float(np.mean(walk)),
+            "adjoint_gpu_ms": float(np.mean(adj)),
+            "forward_plus_adjoint_units_per_s": units / ((np.mean(fwd) + np.mean(adj)) * 1e-3),
+            "chi2": c2}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +161,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-population", action="store_true",
                     help="skip the batched-population leg")
+    ap.add_argument("--no-rb", action="store_true",
+                    help="skip the Rao-Blackwellised (differentiable) leg")
     ap.add_argument("--cpu-bursts", type=int, default=100_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (HBM counter calibration)")
@@ -194,6 +241,8 @@ def main():
     m.close()
     if not args.no_population:
         out["population"] = population_leg(pda, synth, max(1, args.steps // 4), 1)
+    if not args.no_rb:
+        out["rao_blackwellised"] = rb_leg(pda, synth, max(2, args.steps // 4))
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, threads, args.cpu_bursts)
