@@ -43,6 +43,27 @@ def build_hostwalk():
     return HOSTWALK
 
 
+HOST_PROGRAMS = ("mc_dropin", "ps_loop")
+
+
+def host_program(name):
+    """tests/host/<name>.cpp built against the HIP library into tests/_build/
+    (built by __graft_entry__.build(); rebuilt here only when stale)."""
+    src = os.path.join(ROOT, "tests", "host", name + ".cpp")
+    lib = os.path.join(ROOT, "gsmfretda_amd", "libgsmfretda_pda.so")
+    exe = os.path.join(ROOT, "tests", "_build", name)
+    deps = [src, os.path.join(ROOT, "include", "gsmfretda_mc.hpp")]
+    if os.path.exists(exe) and os.path.getmtime(exe) >= max(os.path.getmtime(p) for p in deps):
+        return exe
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    cmd = ["hipcc", "-O2", "-std=c++17", "-pthread", "-I", os.path.join(ROOT, "include"), src,
+           "-L", os.path.dirname(lib), "-lgsmfretda_pda",
+           "-Wl,-rpath,$ORIGIN/../../gsmfretda_amd", "-o", exe + ".tmp"]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=600)
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 @pytest.fixture(scope="session")
 def gpu_mc():
     """One MC context on cuda:0 for the whole GPU session."""

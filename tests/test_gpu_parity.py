@@ -244,9 +244,8 @@ def test_mc_shim_dropin(oracle, tmp_path):
     """The reference's call sequence through the C++ `class mc` shim
     (tests/host/mc_dropin.cpp): hmcE bit-exact with the oracle, and the
     device histogram / chi2 equal to the caller's own binning of hmcE."""
-    import os
     import subprocess
-    from tests.conftest import ROOT
+    from tests.conftest import host_program
     cfg = synth.config(1, n_bursts=1200)
     d = cfg["data"]
     obs = _oracle_run(oracle, d, cfg["n"], cfg["args"], (0.34, 0.08, 0.07, 52.0), 1, 5)["E"]
@@ -257,12 +256,7 @@ def test_mc_shim_dropin(oracle, tmp_path):
                   scalars=np.array([d.clk_p, d.bg_ad, d.bg_dd], np.float32), args=cfg["args"])
     for k, v in arrays.items():
         np.ascontiguousarray(v).tofile(str(inp / k))
-    exe = str(tmp_path / "mc_dropin")
-    lib_dir = os.path.join(ROOT, "gsmfretda_amd")
-    subprocess.run(["hipcc", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
-                    os.path.join(ROOT, "tests", "host", "mc_dropin.cpp"), "-L", lib_dir,
-                    "-lgsmfretda_pda", f"-Wl,-rpath,{lib_dir}", "-o", exe],
-                   check=True, capture_output=True, timeout=300)
+    exe = host_program("mc_dropin")
     R, seed, sid, begin, end = 16, 777, 1, 100, 1100
     out = str(tmp_path / "out.bin")
     subprocess.run([exe, str(inp), out, str(cfg["n"]), str(R), str(seed), str(begin), str(end),
