@@ -220,6 +220,15 @@ class MC:
                            bursts.times, bursts.mask_ad, bursts.mask_dd, bursts.T, SgDivSr,
                            bursts.clk_p, bursts.bg_ad, bursts.bg_dd, nbins, left_pad)
 
+    def load_hdf5(self, path, nbins=None, left_pad=2):
+        """Bursts and SgDivSr from a file in the reference's HDF5 schema
+        (src/loadHdf5.cpp:37-74), loaded as init_data_gpu does; returns the
+        BurstData."""
+        from .hdf5 import read_bursts
+        d, sg = read_bursts(path)
+        self.load(d, SgDivSr=sg, nbins=nbins or self._hist_bins, left_pad=left_pad)
+        return d
+
     def set_observed(self, SgDivSr, nbins=50, left_pad=2):
         x = np.ascontiguousarray(SgDivSr, np.float32)
         _check(self._L.pda_set_observed(self._h, _p(x, _f32p), len(x), nbins, left_pad))
