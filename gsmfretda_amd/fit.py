@@ -194,12 +194,18 @@ class AdamFitter:
     Rao-Blackwellised chi2; progress measured by the plain chi2 every
     `check_every` steps."""
 
-    def __init__(self, lr=0.05, betas=(0.9, 0.999), check_every=1, lr_decay=1.0, R_grad=None):
+    def __init__(self, lr=0.05, betas=(0.9, 0.999), check_every=1, lr_decay=1.0, R_grad=None,
+                 plateau=0, plateau_factor=0.5, lr_min=1e-3):
         self.lr = lr
         self.betas = betas
         self.check_every = check_every
-        self.lr_decay = lr_decay
+        self.lr_decay = lr_decay      # per-step multiplicative decay
         self.R_grad = R_grad  # resamples of the RB gradient (default: the problem's R)
+        # reduce on plateau: after `plateau` steps without a 0.1 % improvement
+        # of the RB chi2, multiply the learning rate by plateau_factor
+        self.plateau = plateau
+        self.plateau_factor = plateau_factor
+        self.lr_min = lr_min
 
     def run(self, prob, theta0, max_seconds, target=None, max_steps=100000):
         import torch
@@ -214,6 +220,7 @@ class AdamFitter:
         obs = torch.tensor(prob.observed, dtype=torch.float64)
         c = prob.chi2(theta.detach().numpy())
         hist.record(time.perf_counter() - t0, c, prob.evaluations, theta.detach().numpy())
+        best_loss, since = float("inf"), 0
         for step in range(max_steps):
             if target is not None and hist.chi2[-1] <= target:
                 break
@@ -229,6 +236,16 @@ class AdamFitter:
             prob.evaluations += 1
             opt.step()
             sched.step()
+            if self.plateau:
+                lv = float(loss.detach())
+                if lv < best_loss * 0.999:
+                    best_loss, since = lv, 0
+                else:
+                    since += 1
+                    if since >= self.plateau:
+                        for grp in opt.param_groups:
+                            grp["lr"] = max(self.lr_min, grp["lr"] * self.plateau_factor)
+                        since = 0
             with torch.no_grad():
                 theta[:n].clamp_(*LOGIT_E_BOUNDS)
                 theta[n:].clamp_(*LOG_K_BOUNDS)

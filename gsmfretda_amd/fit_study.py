@@ -39,6 +39,10 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=0.1)
     ap.add_argument("--r-grad", type=int, default=8,
                     help="resamples per burst of the RB gradient (the plain chi2 keeps R = 32)")
+    ap.add_argument("--plateau", type=int, default=0,
+                    help="Adam: halve the learning rate after this many steps without progress")
+    ap.add_argument("--lr-decay", type=float, default=1.0, help="Adam: per-step decay")
+    ap.add_argument("--only-adam", action="store_true", help="skip the genetic fitters")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
 
@@ -71,10 +75,12 @@ def main(argv=None):
     pop0 = [fit.args_to_theta(args, n) for _, args in cfg["population"][:a.pop]]
 
     results = {}
-    for name, make in (("ga-per-set", lambda: fit.GeneticFitter(batched=False, pop_size=a.pop,
-                                                                 rng=np.random.default_rng(7))),
-                       ("ga-batched", lambda: fit.GeneticFitter(batched=True, pop_size=a.pop,
-                                                                 rng=np.random.default_rng(7)))):
+    ga_specs = [] if a.only_adam else [
+        ("ga-per-set", lambda: fit.GeneticFitter(batched=False, pop_size=a.pop,
+                                                 rng=np.random.default_rng(7))),
+        ("ga-batched", lambda: fit.GeneticFitter(batched=True, pop_size=a.pop,
+                                                 rng=np.random.default_rng(7)))]
+    for name, make in ga_specs:
         prob.evaluations = 0
         h = make().run(prob, pop0, a.seconds, target=target)
         results[name] = h
@@ -87,15 +93,14 @@ def main(argv=None):
     fit0 = prob.chi2_population(pop0)
     start = pop0[int(np.argmin(fit0))]
     t_init = time.perf_counter() - ts
-    h = fit.AdamFitter(lr=a.lr, R_grad=a.r_grad, check_every=5).run(prob, start,
-                                                                     a.seconds - t_init,
-                                                                     target=target)
+    h = fit.AdamFitter(lr=a.lr, R_grad=a.r_grad, check_every=5, plateau=a.plateau,
+                       lr_decay=a.lr_decay).run(prob, start, a.seconds - t_init, target=target)
     h.times = [t + t_init for t in h.times]
     results["adam-rb"] = h
     log(f"adam-rb: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s, "
         f"{prob.evaluations} evaluations")
 
-    ga_final = results["ga-per-set"].chi2[-1]
+    ga_final = results["ga-per-set"].chi2[-1] if "ga-per-set" in results else float("nan")
     out = {"workload": "4 states, rates 1e3-1e5, %d bursts x %d resamples, 50 bins" % (d.nbursts, R),
            "chi2_truth": chi2_truth, "target_truth": target, "ga_per_set_final": ga_final,
            "seconds_limit": a.seconds,
