@@ -95,6 +95,7 @@ struct RbState {
     double* d_g = nullptr;             // cotangent per bin
     double* h_g = nullptr;             // pinned
     double* d_grad = nullptr;          // n + K
+    float* d_pmf_tab = nullptr;        // binomial pmfs of the states, counts < 256
     double* h_grad = nullptr;          // pinned
     hipEvent_t ev0 = nullptr, ev_walk = nullptr, ev1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;
     bool ran = false, scores = false, adj_ran = false;
@@ -208,11 +209,13 @@ hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, c
                           const RunArgs& a, uint2* cnt, float* F, float* S, uint32_t* work,
                           hipStream_t stream);
 size_t rb_pmf_lds(int B, int cap_big);
+hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
+                           hipStream_t stream);
 hipError_t launch_rb_pmf(bool grad_mode, uint32_t grid, int cap_big, const RbModel* rbm,
-                         const double* lf, const RunArgs& a, const BurstDesc* bursts,
-                         const uint2* cnt, const float* F, const float* S, const double* g,
-                         unsigned long long* hist_acc, double* grad, uint32_t* work,
-                         hipStream_t stream);
+                         const double* lf, const float* pmf_tab, const RunArgs& a,
+                         const BurstDesc* bursts, const uint2* cnt, const float* F,
+                         const float* S, const double* g, unsigned long long* hist_acc,
+                         double* grad, uint32_t* work, hipStream_t stream);
 hipError_t launch_rb_finish(const RunArgs& a, const float* obs, unsigned long long* hist_acc,
                             uint32_t* work, void* res, hipStream_t stream);
 

@@ -37,6 +37,7 @@ void free_rb(RbState& r) {
     (void)hipFree(r.d_g);
     (void)hipHostFree(r.h_g);
     (void)hipFree(r.d_grad);
+    (void)hipFree(r.d_pmf_tab);
     (void)hipHostFree(r.h_grad);
     for (hipEvent_t e : {r.ev0, r.ev_walk, r.ev1, r.ev_a0, r.ev_a1})
         if (e) (void)hipEventDestroy(e);
@@ -72,6 +73,7 @@ int ensure_rb(pda_ctx* ctx, RbState& r) {
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_g), sizeof(double) * kMaxBins,
                               hipHostMallocDefault));
         HIP_TRY(hipMalloc(&r.d_grad, sizeof(double) * kGradLen));
+        HIP_TRY(hipMalloc(&r.d_pmf_tab, sizeof(float) * size_t(kRbMaxStates) * kRbPmfStride));
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_grad), sizeof(double) * kGradLen,
                               hipHostMallocDefault));
         for (hipEvent_t* e : {&r.ev0, &r.ev_walk, &r.ev1, &r.ev_a0, &r.ev_a1})
@@ -228,8 +230,10 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
                            static_cast<const RbModel*>(r.d_rbm), a, r.d_cnt, r.d_F, r.d_S,
                            r.d_work, s->stream));
     HIP_TRY(hipEventRecord(r.ev_walk, s->stream));
-    HIP_TRY(launch_rb_pmf(false, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, a,
-                          ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, r.d_hist, nullptr,
+    HIP_TRY(launch_rb_table(static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, r.d_pmf_tab, s->n,
+                            s->stream));
+    HIP_TRY(launch_rb_pmf(false, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
+                          r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, r.d_hist, nullptr,
                           r.d_work, s->stream));
     HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s->stream));
     HIP_TRY(hipEventRecord(r.ev1, s->stream));
@@ -311,8 +315,8 @@ int pda_rb_adjoint(pda_ctx* ctx, int sid, const double* g, int nbins, double* gr
     HIP_TRY(hipMemcpyAsync(r.d_g, r.h_g, sizeof(double) * size_t(nbins), hipMemcpyHostToDevice,
                            s->stream));
     HIP_TRY(hipEventRecord(r.ev_a0, s->stream));
-    HIP_TRY(launch_rb_pmf(true, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, a,
-                          ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, r.d_g, nullptr, r.d_grad,
+    HIP_TRY(launch_rb_pmf(true, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
+                          r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, r.d_g, nullptr, r.d_grad,
                           r.d_work, s->stream));
     HIP_TRY(hipEventRecord(r.ev_a1, s->stream));
     HIP_TRY(hipMemcpyAsync(r.h_grad, r.d_grad, sizeof(double) * kGradLen, hipMemcpyDeviceToHost,

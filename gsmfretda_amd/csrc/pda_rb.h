@@ -95,6 +95,13 @@ PDA_HD inline int rb_bin(int a, float F, int B) {
     return b < B ? b : -1;
 }
 
+// Binomial pmfs tabulated per parameter set for counts below kRbPmfTable:
+// entry (s, m, k) at s * kRbPmfStride + m (m + 1) / 2 + k, the float value of
+// rb_binom_pmf(m, k, ...).
+constexpr int kRbPmfTable = 256;
+constexpr int kRbPmfStride = kRbPmfTable * (kRbPmfTable + 1) / 2;
+PDA_HD inline int rb_pmf_index(int s, int m, int k) { return s * kRbPmfStride + m * (m + 1) / 2 + k; }
+
 // Fixed-point weight of a probability (2^-32 units), rounded half up.
 PDA_HD inline uint64_t rb_fixed(float P) { return uint64_t(double(P) * 4294967296.0 + 0.5); }
 

@@ -158,6 +158,7 @@ constexpr int kRbSmallCap = 256;   // acceptor totals below this use the small l
 // ratio R / (R - 1) sum_r (h_r - mean h) S_r.
 template <int L, bool kGrad>
 __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __restrict__ lf,
+                                         const float* __restrict__ pmf_tab,
                                          const RunArgs& a, uint32_t bl,
                                          const uint2* __restrict__ cnt_in,
                                          const float* __restrict__ F_in,
@@ -183,7 +184,9 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
             const int m = (act && s < n) ? int(rb_count(c, s)) : 0;
             float* Y = base + bY * cap;
             for (int k = lg; m > 0 && k <= m; k += L)
-                Y[k] = float(rb_binom_pmf(m, k, lf, sR.p[s], sR.lp[s], sR.lq[s]));
+                Y[k] = m < kRbPmfTable
+                           ? pmf_tab[rb_pmf_index(s, m, k)]
+                           : float(rb_binom_pmf(m, k, lf, sR.p[s], sR.lp[s], sR.lq[s]));
             __syncthreads();
             if (m > 0) {
                 if (xl == 1) {  // X = [1]: the product is Y itself, bit for bit
@@ -284,11 +287,12 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
 // buffers: max(4 groups x 3 x 256, 3 x cap_big).
 template <bool kGrad>
 __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
-    const RbModel* __restrict__ rbm, const double* __restrict__ lf, const RunArgs a,
-    const BurstDesc* __restrict__ bursts, const uint2* __restrict__ cnt_in,
-    const float* __restrict__ F_in, const float* __restrict__ S_in,
-    const double* __restrict__ g, unsigned long long* __restrict__ hist_acc,
-    double* __restrict__ grad, uint32_t* __restrict__ work, int cap_big) {
+    const RbModel* __restrict__ rbm, const double* __restrict__ lf,
+    const float* __restrict__ pmf_tab, const RunArgs a, const BurstDesc* __restrict__ bursts,
+    const uint2* __restrict__ cnt_in, const float* __restrict__ F_in,
+    const float* __restrict__ S_in, const double* __restrict__ g,
+    unsigned long long* __restrict__ hist_acc, double* __restrict__ grad,
+    uint32_t* __restrict__ work, int cap_big) {
     extern __shared__ double s_dyn[];
     __shared__ RbModel sR;
     unsigned long long* s_h = reinterpret_cast<unsigned long long*>(s_dyn);
@@ -314,11 +318,11 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
         if (bl >= nb) break;
         const int bound = rb_total_bound(bursts[a.begin + bl].P, sR.coef);
         if (bound < kRbSmallCap)
-            rb_burst<kRbSmallLanes, kGrad>(sR, lf, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf,
-                                           kRbSmallCap, lane, gp_acc, gk_acc);
+            rb_burst<kRbSmallLanes, kGrad>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h,
+                                           s_buf, kRbSmallCap, lane, gp_acc, gk_acc);
         else
-            rb_burst<kWave, kGrad>(sR, lf, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf, cap_big,
-                                   lane, gp_acc, gk_acc);
+            rb_burst<kWave, kGrad>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf,
+                                   cap_big, lane, gp_acc, gk_acc);
     }
     if (!kGrad) {
         __syncthreads();
@@ -332,6 +336,22 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
         if (int(lane) < sR.n) atomicAdd(&grad[lane], gp_acc);
         if (int(lane) < sR.K) atomicAdd(&grad[sR.n + int(lane)], gk_acc);
     }
+}
+
+// The binomial pmfs of every state for counts m < kRbPmfTable (one thread
+// per entry), read by rb_pmf_kernel instead of recomputing them per unit.
+__global__ void rb_table_kernel(const RbModel* __restrict__ rbm, const double* __restrict__ lf,
+                                float* __restrict__ pmf_tab) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int n = rbm->n;
+    if (i >= n * kRbPmfStride) return;
+    const int s = i / kRbPmfStride, r = i - s * kRbPmfStride;
+    // r = m (m + 1) / 2 + k with 0 <= k <= m
+    int m = int((sqrt(8.0 * double(r) + 1.0) - 1.0) * 0.5);
+    while (m * (m + 1) / 2 > r) --m;
+    while ((m + 1) * (m + 2) / 2 <= r) ++m;
+    const int k = r - m * (m + 1) / 2;
+    pmf_tab[i] = float(rb_binom_pmf(m, k, lf, rbm->p[s], rbm->lp[s], rbm->lq[s]));
 }
 
 // Device result of an RB run, followed by uint64 M[B] (2^-32 units).
@@ -407,18 +427,26 @@ size_t rb_pmf_lds(int B, int cap_big) {
     return sizeof(double) * size_t(B) + sizeof(float) * floats;
 }
 
+hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
+                           hipStream_t stream) {
+    const int total = n * kRbPmfStride;
+    hipLaunchKernelGGL(rb_table_kernel, dim3(uint32_t((total + 255) / 256)), dim3(256), 0, stream,
+                       rbm, lf, pmf_tab);
+    return hipGetLastError();
+}
+
 hipError_t launch_rb_pmf(bool grad_mode, uint32_t grid, int cap_big, const RbModel* rbm,
-                         const double* lf, const RunArgs& a, const BurstDesc* bursts,
-                         const uint2* cnt, const float* F, const float* S, const double* g,
-                         unsigned long long* hist_acc, double* grad, uint32_t* work,
-                         hipStream_t stream) {
+                         const double* lf, const float* pmf_tab, const RunArgs& a,
+                         const BurstDesc* bursts, const uint2* cnt, const float* F,
+                         const float* S, const double* g, unsigned long long* hist_acc,
+                         double* grad, uint32_t* work, hipStream_t stream) {
     const size_t lds = rb_pmf_lds(a.B, cap_big);
     if (grad_mode)
-        hipLaunchKernelGGL(rb_pmf_kernel<true>, dim3(grid), dim3(kWave), lds, stream, rbm, lf, a,
-                           bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
+        hipLaunchKernelGGL(rb_pmf_kernel<true>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
+                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
     else
-        hipLaunchKernelGGL(rb_pmf_kernel<false>, dim3(grid), dim3(kWave), lds, stream, rbm, lf, a,
-                           bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
+        hipLaunchKernelGGL(rb_pmf_kernel<false>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
+                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
     return hipGetLastError();
 }
 
