@@ -114,6 +114,23 @@ def test_background_replays(oracle, hostwalk):
     assert hostwalk.hw_replays() > before
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_windows_past_the_stop(oracle, hostwalk, seed):
+    # The last window of a unit ends at the burst's stop tick.  Stops before
+    # the last photons and durations T above (stop - start) clk_p let earlier
+    # windows end past the stop, so the last window's search starts beyond
+    # the stop's lower bound (8-36 such final windows per case here).
+    rng = np.random.default_rng(70 + seed)
+    d = tiny_bursts(rng, 80)
+    dur = d.stop - d.start + 1
+    d.stop = d.start + (dur * rng.uniform(0.1, 1.0, d.nbursts)).astype(np.int64)
+    d.T = (d.T * np.float32([1.0, 1.2, 1.0][seed])).astype(np.float32)
+    d.bg_dd, d.bg_ad = [0.0, 1e4, 1e3][seed], [0.0, 5e3, 1e3][seed]
+    n = 3
+    args = synth.make_args([0.2, 0.5, 0.8], np.full(6, 2e5 * (seed + 1)))
+    compare(oracle, hostwalk, d, n, args, (0.34, 0.08, 0.07, 52.0), 12, 300 + seed)
+
+
 def test_link_encoding_brute_force(hostwalk):
     rng = np.random.default_rng(8)
     d = tiny_bursts(rng, 40)
