@@ -96,6 +96,8 @@ struct RbState {
     double* h_g = nullptr;             // pinned
     double* d_grad = nullptr;          // n + K
     float* d_pmf_tab = nullptr;        // binomial pmfs of the states, counts < 256
+    double* d_jac = nullptr;           // [kMaxBins][n + K] rows of 16
+    double* h_jac = nullptr;           // pinned
     double* h_grad = nullptr;          // pinned
     hipEvent_t ev0 = nullptr, ev_walk = nullptr, ev1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;
     bool ran = false, scores = false, adj_ran = false;
@@ -208,10 +210,11 @@ hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, c
                           const uint2* links, const ModelBlob* blob, const RbModel* rbm,
                           const RunArgs& a, uint2* cnt, float* F, float* S, uint32_t* work,
                           hipStream_t stream);
-size_t rb_pmf_lds(int B, int cap_big);
+// mode: 0 forward, 1 adjoint, 2 Jacobian
+size_t rb_pmf_lds(int B, int cap_big, int mode);
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
                            hipStream_t stream);
-hipError_t launch_rb_pmf(bool grad_mode, uint32_t grid, int cap_big, const RbModel* rbm,
+hipError_t launch_rb_pmf(int mode, uint32_t grid, int cap_big, const RbModel* rbm,
                          const double* lf, const float* pmf_tab, const RunArgs& a,
                          const BurstDesc* bursts, const uint2* cnt, const float* F,
                          const float* S, const double* g, unsigned long long* hist_acc,

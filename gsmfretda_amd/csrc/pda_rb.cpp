@@ -38,6 +38,8 @@ void free_rb(RbState& r) {
     (void)hipHostFree(r.h_g);
     (void)hipFree(r.d_grad);
     (void)hipFree(r.d_pmf_tab);
+    (void)hipFree(r.d_jac);
+    (void)hipHostFree(r.h_jac);
     (void)hipHostFree(r.h_grad);
     for (hipEvent_t e : {r.ev0, r.ev_walk, r.ev1, r.ev_a0, r.ev_a1})
         if (e) (void)hipEventDestroy(e);
@@ -74,6 +76,9 @@ int ensure_rb(pda_ctx* ctx, RbState& r) {
                               hipHostMallocDefault));
         HIP_TRY(hipMalloc(&r.d_grad, sizeof(double) * kGradLen));
         HIP_TRY(hipMalloc(&r.d_pmf_tab, sizeof(float) * size_t(kRbMaxStates) * kRbPmfStride));
+        HIP_TRY(hipMalloc(&r.d_jac, sizeof(double) * kGradLen * kMaxBins));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_jac), sizeof(double) * kGradLen * kMaxBins,
+                              hipHostMallocDefault));
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_grad), sizeof(double) * kGradLen,
                               hipHostMallocDefault));
         for (hipEvent_t* e : {&r.ev0, &r.ev_walk, &r.ev1, &r.ev_a0, &r.ev_a1})
@@ -159,7 +164,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
                     "%u photons need %d)", kRbMaxCount, maxP, bound);
     const int cap = bound + 1;
     const bool scores = (flags & PDA_RB_SCORES) != 0;
-    if (rb_pmf_lds(ctx->B, cap) + sizeof(RbModel) > ctx->lds_per_block)
+    if (rb_pmf_lds(ctx->B, cap, 0) + sizeof(RbModel) > ctx->lds_per_block)
         return fail(PDA_E_DATA, "bursts of %u photons exceed the LDS of the RB kernels", maxP);
 
     // buffers of the per-unit summaries
@@ -217,7 +222,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, (uint64_t(units) + batch - 1) / batch)));
     a.batch = uint32_t(batch);
     a.queues = std::min<uint32_t>(ctx->shape.queues ? ctx->shape.queues : kMaxQueues, grid);
-    const int dyn = int(rb_pmf_lds(ctx->B, cap));
+    const int dyn = int(rb_pmf_lds(ctx->B, cap, 0));
     HIP_TRY(rb_occupancy(2, dyn, &per_cu));
     per_cu = std::max(per_cu, 1);
     const uint32_t pgrid = uint32_t(std::max<int64_t>(
@@ -232,7 +237,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     HIP_TRY(hipEventRecord(r.ev_walk, s->stream));
     HIP_TRY(launch_rb_table(static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, r.d_pmf_tab, s->n,
                             s->stream));
-    HIP_TRY(launch_rb_pmf(false, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
+    HIP_TRY(launch_rb_pmf(0, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
                           r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, r.d_hist, nullptr,
                           r.d_work, s->stream));
     HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s->stream));
@@ -278,12 +283,13 @@ int pda_rb_get_result(pda_ctx* ctx, int sid, uint64_t* M_fixed, double* M, int n
     return PDA_OK;
 }
 
-int pda_rb_adjoint(pda_ctx* ctx, int sid, const double* g, int nbins, double* grad, int nargs) {
-    Slot* s = slot_of(ctx, sid);
-    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
-    RbState& r = s->rb;
+}  // extern "C"
+
+namespace {
+
+// Preconditions shared by the adjoint and the Jacobian of an RB forward run.
+int check_derivative_run(const RbState& r, int sid, int nbins, int nargs) {
     if (!r.ran) return fail(PDA_E_STATE, "stream %d has no RB forward run", sid);
-    if (!g || !grad) return fail(PDA_E_ARG, "null argument");
     if (nbins != r.B) return fail(PDA_E_ARG, "nbins %d != run's %d", nbins, r.B);
     if (nargs != r.n * (r.n + 1)) return fail(PDA_E_ARG, "nargs %d != %d", nargs, r.n * (r.n + 1));
     if (!r.scores) return fail(PDA_E_STATE, "the forward run kept no scores (PDA_RB_SCORES)");
@@ -292,30 +298,54 @@ int pda_rb_adjoint(pda_ctx* ctx, int sid, const double* g, int nbins, double* gr
     if (!r.rates_positive) return fail(PDA_E_ARG, "rate gradients need every rate > 0");
     if (r.init_state < 0)
         return fail(PDA_E_ARG, "rate gradients need the initial state fixed (inflow of state 0 > 1 s^-1)");
-    for (int b = 0; b < nbins; ++b)
-        if (!std::isfinite(g[b])) return fail(PDA_E_ARG, "g[%d] not finite", b);
-    HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    std::memcpy(r.h_g, g, sizeof(double) * size_t(nbins));
-    RunArgs a{};
+    return PDA_OK;
+}
+
+// Launch arguments and grid of an rb_pmf_kernel pass over the run's bursts.
+int derivative_pass(pda_ctx* ctx, const RbState& r, int mode, RunArgs& a, uint32_t& pgrid) {
+    a = RunArgs{};
     a.begin = uint32_t(r.b0);
     a.nunits = uint32_t((r.b1 - r.b0) * int64_t(r.R));
     a.R = uint32_t(r.R);
     a.B = r.B;
     a.left_pad = r.left_pad;
     a.nstates = r.n;
-    const int dyn = int(rb_pmf_lds(r.B, r.cap));
+    const size_t lds = rb_pmf_lds(r.B, r.cap, mode);
+    if (lds + sizeof(RbModel) > ctx->lds_per_block)
+        return fail(PDA_E_DATA, "%d bins and bursts needing %d acceptor outcomes exceed the LDS of "
+                    "the %s kernel", r.B, r.cap, mode == 2 ? "Jacobian" : "adjoint");
     int per_cu = 0;
-    HIP_TRY(rb_occupancy(3, dyn, &per_cu));
+    HIP_TRY(rb_occupancy(mode == 2 ? 4 : 3, int(lds), &per_cu));
     per_cu = std::max(per_cu, 1);
-    const uint32_t pgrid = uint32_t(std::max<int64_t>(
+    pgrid = uint32_t(std::max<int64_t>(
         1, std::min<int64_t>(int64_t(ctx->num_cu) * per_cu, r.b1 - r.b0)));
+    return PDA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pda_rb_adjoint(pda_ctx* ctx, int sid, const double* g, int nbins, double* grad, int nargs) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    RbState& r = s->rb;
+    if (!g || !grad) return fail(PDA_E_ARG, "null argument");
+    if (int rc = check_derivative_run(r, sid, nbins, nargs)) return rc;
+    for (int b = 0; b < nbins; ++b)
+        if (!std::isfinite(g[b])) return fail(PDA_E_ARG, "g[%d] not finite", b);
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::memcpy(r.h_g, g, sizeof(double) * size_t(nbins));
+    RunArgs a;
+    uint32_t pgrid = 0;
+    if (int rc = derivative_pass(ctx, r, 1, a, pgrid)) return rc;
     HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
     HIP_TRY(hipMemsetAsync(r.d_grad, 0, sizeof(double) * kGradLen, s->stream));
     HIP_TRY(hipMemcpyAsync(r.d_g, r.h_g, sizeof(double) * size_t(nbins), hipMemcpyHostToDevice,
                            s->stream));
     HIP_TRY(hipEventRecord(r.ev_a0, s->stream));
-    HIP_TRY(launch_rb_pmf(true, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
+    HIP_TRY(launch_rb_pmf(1, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
                           r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, r.d_g, nullptr, r.d_grad,
                           r.d_work, s->stream));
     HIP_TRY(hipEventRecord(r.ev_a1, s->stream));
@@ -327,6 +357,37 @@ int pda_rb_adjoint(pda_ctx* ctx, int sid, const double* g, int nbins, double* gr
     for (int i = 0; i < n; ++i) grad[i] = r.h_grad[i];            // d chi2 / dE_i (dp/dE = 1)
     for (int i = 0; i < K; ++i) grad[n + i] = r.h_grad[n + i];    // rates, args order
     for (int i = n + K; i < nargs; ++i) grad[i] = 0.0;            // v (held at 0)
+    r.adj_ran = true;
+    return PDA_OK;
+}
+
+int pda_rb_jacobian(pda_ctx* ctx, int sid, double* J, int nbins, int nargs) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    RbState& r = s->rb;
+    if (!J) return fail(PDA_E_ARG, "null argument");
+    if (int rc = check_derivative_run(r, sid, nbins, nargs)) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    RunArgs a;
+    uint32_t pgrid = 0;
+    if (int rc = derivative_pass(ctx, r, 2, a, pgrid)) return rc;
+    const size_t bytes = sizeof(double) * kGradLen * size_t(r.B);
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
+    HIP_TRY(hipMemsetAsync(r.d_jac, 0, bytes, s->stream));
+    HIP_TRY(hipEventRecord(r.ev_a0, s->stream));
+    HIP_TRY(launch_rb_pmf(2, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
+                          r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, nullptr,
+                          r.d_jac, r.d_work, s->stream));
+    HIP_TRY(hipEventRecord(r.ev_a1, s->stream));
+    HIP_TRY(hipMemcpyAsync(r.h_jac, r.d_jac, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int w = r.n + r.K;
+    for (int b = 0; b < nbins; ++b) {
+        for (int i = 0; i < w; ++i) J[size_t(b) * size_t(nargs) + size_t(i)] = r.h_jac[size_t(b) * kGradLen + size_t(i)];
+        for (int i = w; i < nargs; ++i) J[size_t(b) * size_t(nargs) + size_t(i)] = 0.0;  // v
+    }
     r.adj_ran = true;
     return PDA_OK;
 }

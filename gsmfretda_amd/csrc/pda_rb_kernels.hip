@@ -10,7 +10,8 @@
 //                    pmf of each unit's acceptor total (a convolution of one
 //                    binomial per state, in LDS, lanes over outcomes) spread
 //                    into the E histogram as 2^-32 fixed point -- or, for the
-//                    adjoint, the cotangent-weighted pmf and its derivatives
+//                    adjoint, the cotangent-weighted pmf and its derivatives,
+//                    or, for the Jacobian, the derivatives of every bin
 //   rb_finish_kernel reduces the histogram rows, computes chi2 on the
 //                    expected counts, resets the counters
 #include <hip/hip_runtime.h>
@@ -25,6 +26,20 @@
 namespace pda {
 
 constexpr uint32_t kRbPmfCounter = 8;  // work[8]: bursts taken by rb_pmf_kernel
+
+// Jacobian scratch in LDS (doubles, after the float buffers): per bin the
+// efficiency and rate derivatives, the current unit's histogram per lane
+// group, the burst's histogram, the burst's score sums.
+struct JacLds {
+    double* JE;  // [B][kRbMaxStates]
+    double* JK;  // [B][kRbMaxPairs]
+    double* hu;  // [kWave / kRbSmallLanes][B]
+    double* Hb;  // [B]
+    double* Sk;  // [kRbMaxPairs]
+};
+__host__ __device__ inline size_t rb_jac_doubles(int B) {
+    return size_t(B) * (kRbMaxStates + kRbMaxPairs + kWave / 16 + 1) + kRbMaxPairs;
+}
 
 template <bool kScores>
 __global__ __launch_bounds__(kWave) void rb_walk_kernel(
@@ -147,7 +162,7 @@ constexpr int kRbSmallCap = 256;   // acceptor totals below this use the small l
 // Forward (kGrad = false): the unit's pmf P(a), a = 0..Mt, goes into the LDS
 // histogram at E = a / F in 2^-32 fixed point.
 //
-// Adjoint (kGrad = true): with G(a) the cotangent of a's bin, the unit
+// Adjoint (kMode = kRbGrad): with G(a) the cotangent of a's bin, the unit
 // contributes h = sum_a G(a) P(a) and, per state s with m_s photons,
 //   d h / d p_s = m_s sum_c P_s(c) (G(c + 1) - G(c))
 // where P_s is the pmf with one trial of state s removed,
@@ -156,7 +171,15 @@ constexpr int kRbSmallCap = 256;   // acceptor totals below this use the small l
 // p_s <= 1/2 and downwards otherwise (the direction in which the recurrence
 // damps rounding errors).  Per burst, the rate gradient is the likelihood
 // ratio R / (R - 1) sum_r (h_r - mean h) S_r.
-template <int L, bool kGrad>
+//
+// Jacobian (kMode = kRbJac): the adjoint for every indicator cotangent
+// G_b(a) = [bin(a) = b] at once.  G_b(c + 1) - G_b(c) is nonzero only where
+// c and c + 1 fall in different bins, so d M_b / d p_s gathers m_s P_s(c) at
+// the unit's bin boundaries; the rate columns use the unit's histogram h_rb
+// in place of h_r (sum_r h_rb S_r here, the mean term per burst).
+constexpr int kRbFwd = 0, kRbGrad = 1, kRbJac = 2;
+
+template <int L, int kMode>
 __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __restrict__ lf,
                                          const float* __restrict__ pmf_tab,
                                          const RunArgs& a, uint32_t bl,
@@ -164,7 +187,9 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
                                          const float* __restrict__ F_in,
                                          const float* __restrict__ S_in, const double* s_g,
                                          unsigned long long* s_h, float* s_buf, int cap,
-                                         uint32_t lane, double& gp_acc, double& gk_acc) {
+                                         uint32_t lane, double& gp_acc, double& gk_acc,
+                                         const JacLds& jl) {
+    constexpr bool kGrad = kMode != kRbFwd;
     constexpr int G = kWave / L;
     const int g = int(lane) / L, lg = int(lane) % L;
     const int n = sR.n, K = sR.K, B = a.B;
@@ -215,8 +240,13 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
         } else {
             float* dG = base + bZ * cap;  // G(c + 1) - G(c)
             double hp = 0.0;
+            double* hu = jl.hu + g * B;   // Jacobian: this unit's histogram
             for (int q = lg; q < xl; q += L) {
                 const int b0 = rb_bin(q, F, B);
+                if (kMode == kRbJac) {
+                    if (act && b0 >= 0) atomicAdd(&hu[b0], double(P[q]));
+                    continue;
+                }
                 const double G0 = b0 >= 0 ? s_g[b0] : 0.0;
                 hp += G0 * double(P[q]);
                 if (q + 1 < xl) {
@@ -253,13 +283,37 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
                         }
                         const double x = A * carry + Bv;  // P_s(up ? j : Mt - 1 - j)
                         carry = __shfl(x, L - 1, L);
-                        if (v) part += x * double(dG[up ? j : Mt - 1 - j]);
+                        const int ci = up ? j : Mt - 1 - j;
+                        if (kMode == kRbJac) {
+                            const int b0 = rb_bin(ci, F, B), b1 = rb_bin(ci + 1, F, B);
+                            if (v && b0 != b1) {
+                                const double w = double(m) * x;
+                                if (b1 >= 0) atomicAdd(&jl.JE[b1 * kRbMaxStates + s], w);
+                                if (b0 >= 0) atomicAdd(&jl.JE[b0 * kRbMaxStates + s], -w);
+                            }
+                        } else if (v) {
+                            part += x * double(dG[ci]);
+                        }
                     }
-                    part = group_sum<L>(part) * double(m);
+                    if (kMode == kRbGrad) part = group_sum<L>(part) * double(m);
                 }
-                if (lg == s) gp_acc += part;
+                if (kMode == kRbGrad && lg == s) gp_acc += part;
             }
-            if (act) {
+            if (kMode == kRbJac) {
+                // rates: sum_r h_rb S_r per (bin, rate); the burst's histogram
+                if (act && lg < K) {
+                    const double si = double(S_in[size_t(u) * size_t(K) + size_t(lg)]);
+                    Sg += si;
+                    for (int b = 0; b < B; ++b) {
+                        const double hb = hu[b];
+                        if (hb != 0.0) atomicAdd(&jl.JK[b * kRbMaxPairs + lg], hb * si);
+                    }
+                }
+                for (int b = lg; act && b < B; b += L)
+                    if (hu[b] != 0.0) atomicAdd(&jl.Hb[b], hu[b]);
+                __syncthreads();
+                for (int b = lg; b < B; b += L) hu[b] = 0.0;
+            } else if (act) {
                 H += h;
                 if (lg < K) {
                     const double si = double(S_in[size_t(u) * size_t(K) + size_t(lg)]);
@@ -270,7 +324,7 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
         }
         __syncthreads();
     }
-    if (kGrad) {
+    if (kMode == kRbGrad) {
 #pragma unroll
         for (int o = L; o < kWave; o <<= 1) {
             H += __shfl_xor(H, o);
@@ -279,13 +333,35 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
         }
         if (int(lane) < K && a.R > 1)
             gk_acc += (double(a.R) / double(a.R - 1)) * (Ac - H * Sg / double(a.R));
+    } else if (kMode == kRbJac) {
+        // the burst's mean term: JK[b][k] -= H_b S_k / R (the factor
+        // R / (R - 1) is applied once, when the kernel flushes JK)
+#pragma unroll
+        for (int o = L; o < kWave; o <<= 1) Sg += __shfl_xor(Sg, o);
+        if (int(lane) < K) jl.Sk[lane] = Sg;
+        __syncthreads();
+        const double inv_r = 1.0 / double(a.R);
+        for (int i = int(lane); i < B * K; i += kWave) {
+            const int b = i / K, k = i - b * K;
+            jl.JK[b * kRbMaxPairs + k] -= jl.Hb[b] * jl.Sk[k] * inv_r;
+        }
+        __syncthreads();
+        for (int b = int(lane); b < B; b += kWave) jl.Hb[b] = 0.0;
+        __syncthreads();
     }
+}
+
+// Float entries of the pmf buffers: 4 groups x 3 x 256 or 3 x cap_big.
+__host__ __device__ inline size_t rb_pmf_floats(int cap_big) {
+    const size_t small = size_t(kWave / kRbSmallLanes) * 3u * kRbSmallCap;
+    const size_t big = 3u * size_t(cap_big);
+    return ((small > big ? small : big) + 1u) & ~size_t(1);  // even: doubles follow
 }
 
 // One wave per burst (taken from work[8]).  Dynamic LDS: B 8-byte entries
 // (the fixed-point histogram, or the cotangent) followed by the float
-// buffers: max(4 groups x 3 x 256, 3 x cap_big).
-template <bool kGrad>
+// buffers, then (Jacobian) the JacLds arrays.
+template <int kMode>
 __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
     const RbModel* __restrict__ rbm, const double* __restrict__ lf,
     const float* __restrict__ pmf_tab, const RunArgs a, const BurstDesc* __restrict__ bursts,
@@ -293,20 +369,30 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
     const float* __restrict__ S_in, const double* __restrict__ g,
     unsigned long long* __restrict__ hist_acc, double* __restrict__ grad,
     uint32_t* __restrict__ work, int cap_big) {
+    constexpr bool kGrad = kMode != kRbFwd;
     extern __shared__ double s_dyn[];
     __shared__ RbModel sR;
     unsigned long long* s_h = reinterpret_cast<unsigned long long*>(s_dyn);
     double* s_g = s_dyn;
     float* s_buf = reinterpret_cast<float*>(s_dyn + a.B);
+    const int B = a.B;
+    JacLds jl;
+    jl.JE = s_dyn + B + rb_pmf_floats(cap_big) / 2;
+    jl.JK = jl.JE + B * kRbMaxStates;
+    jl.hu = jl.JK + B * kRbMaxPairs;
+    jl.Hb = jl.hu + B * (kWave / kRbSmallLanes);
+    jl.Sk = jl.Hb + B;
     const uint32_t lane = threadIdx.x;
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(rbm);
         uint32_t* dst = reinterpret_cast<uint32_t*>(&sR);
         for (uint32_t i = lane; i < sizeof(RbModel) / 4; i += kWave) dst[i] = src[i];
         for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
-            if (kGrad) s_g[i] = g[i];
+            if (kMode == kRbGrad) s_g[i] = g[i];
             else s_h[i] = 0ull;
         }
+        if (kMode == kRbJac)
+            for (size_t i = lane; i < rb_jac_doubles(B); i += kWave) jl.JE[i] = 0.0;
     }
     __syncthreads();
     const uint32_t nb = a.nunits / a.R;
@@ -318,13 +404,24 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
         if (bl >= nb) break;
         const int bound = rb_total_bound(bursts[a.begin + bl].P, sR.coef);
         if (bound < kRbSmallCap)
-            rb_burst<kRbSmallLanes, kGrad>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h,
-                                           s_buf, kRbSmallCap, lane, gp_acc, gk_acc);
+            rb_burst<kRbSmallLanes, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h,
+                                           s_buf, kRbSmallCap, lane, gp_acc, gk_acc, jl);
         else
-            rb_burst<kWave, kGrad>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf,
-                                   cap_big, lane, gp_acc, gk_acc);
+            rb_burst<kWave, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf,
+                                   cap_big, lane, gp_acc, gk_acc, jl);
     }
-    if (!kGrad) {
+    if (kMode == kRbJac) {
+        // grad holds J[b][i], i < n + K (efficiencies, then rates in args
+        // order), B rows of kRbMaxStates + kRbMaxPairs
+        __syncthreads();
+        const int n = sR.n, K = sR.K, w = n + K;
+        const double f = a.R > 1 ? double(a.R) / double(a.R - 1) : 0.0;
+        for (int i = int(lane); i < B * w; i += kWave) {
+            const int b = i / w, c = i - b * w;
+            const double v = c < n ? jl.JE[b * kRbMaxStates + c] : f * jl.JK[b * kRbMaxPairs + (c - n)];
+            if (v != 0.0) atomicAdd(&grad[size_t(b) * (kRbMaxStates + kRbMaxPairs) + size_t(c)], v);
+        }
+    } else if (!kGrad) {
         __syncthreads();
         unsigned long long* row = hist_acc + size_t(blockIdx.x & 7u) * kMaxBins;
         for (uint32_t i = lane; i < uint32_t(a.B); i += kWave)
@@ -403,8 +500,9 @@ hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu) {
     switch (which) {
         case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_walk_kernel<false>, kWave, 0);
         case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_walk_kernel<true>, kWave, 0);
-        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<false>, kWave, size_t(dyn_lds));
-        default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<true>, kWave, size_t(dyn_lds));
+        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbFwd>, kWave, size_t(dyn_lds));
+        case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbGrad>, kWave, size_t(dyn_lds));
+        default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbJac>, kWave, size_t(dyn_lds));
     }
 }
 
@@ -421,10 +519,9 @@ hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, c
     return hipGetLastError();
 }
 
-size_t rb_pmf_lds(int B, int cap_big) {
-    const size_t floats = std::max<size_t>(size_t(kWave / kRbSmallLanes) * 3u * kRbSmallCap,
-                                           3u * size_t(cap_big));
-    return sizeof(double) * size_t(B) + sizeof(float) * floats;
+size_t rb_pmf_lds(int B, int cap_big, int mode) {
+    return sizeof(double) * size_t(B) + sizeof(float) * rb_pmf_floats(cap_big) +
+           (mode == 2 ? sizeof(double) * rb_jac_doubles(B) : 0u);
 }
 
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
@@ -435,17 +532,20 @@ hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab,
     return hipGetLastError();
 }
 
-hipError_t launch_rb_pmf(bool grad_mode, uint32_t grid, int cap_big, const RbModel* rbm,
+hipError_t launch_rb_pmf(int mode, uint32_t grid, int cap_big, const RbModel* rbm,
                          const double* lf, const float* pmf_tab, const RunArgs& a,
                          const BurstDesc* bursts, const uint2* cnt, const float* F,
                          const float* S, const double* g, unsigned long long* hist_acc,
                          double* grad, uint32_t* work, hipStream_t stream) {
-    const size_t lds = rb_pmf_lds(a.B, cap_big);
-    if (grad_mode)
-        hipLaunchKernelGGL(rb_pmf_kernel<true>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
+    const size_t lds = rb_pmf_lds(a.B, cap_big, mode);
+    if (mode == kRbGrad)
+        hipLaunchKernelGGL(rb_pmf_kernel<kRbGrad>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
+                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
+    else if (mode == kRbJac)
+        hipLaunchKernelGGL(rb_pmf_kernel<kRbJac>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
                            pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
     else
-        hipLaunchKernelGGL(rb_pmf_kernel<false>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
+        hipLaunchKernelGGL(rb_pmf_kernel<kRbFwd>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
                            pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
     return hipGetLastError();
 }

@@ -54,22 +54,23 @@ PDA_HD int64_t binomial_table(int n, int s, const Model& M, const BinomTable& ta
     if (n >= kBinomTable || !(n < kInversionMaxN || pp * double(n) <= 30.0)) return -1;
     const double q = M.qq[s];
     double px = tab.qn[s][n];
-    const int64_t bound = tab.bound[s][n];
+    const int bound = tab.bound[s][n];
     double U = uniform53(hi, lo);
-    int64_t X = 0;
+    int X = 0;  // < kBinomTable: 32-bit counters convert to double exactly
     while (U > px) {
         ++X;
         if (X > bound) return -1;
         U -= px;
         px = (double(n - X + 1) * pp * px) / (double(X) * q);
     }
-    return M.flip[s] ? int64_t(n) - X : X;
+    return M.flip[s] ? int64_t(n - X) : int64_t(X);
 }
 
-// background_count() for means <= 64 (one inversion chunk); -1 otherwise.
+// background_count() for means <= 64 (one inversion chunk); -1 otherwise
+// (also for caps of 2^30 photons or more, which a 32-bit counter could pass).
 PDA_HD float background_single(float lam, float cap, float u_first, float u_cap) {
-    if (lam > 64.0f) return -1.0f;
-    int64_t k = 0;
+    if (lam > 64.0f || !(cap < 0x1p30f)) return -1.0f;
+    int32_t k = 0;
     bool over = false;
     if (lam > 0.0f) {
         float pk = fexp(-lam);

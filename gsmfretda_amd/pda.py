@@ -39,7 +39,7 @@ EXPORTS = (
     "pda_get_tuning", "pda_set_launch_shape", "pda_get_launch_shape", "pda_autotune",
     "pda_set_population", "pda_run_population", "pda_get_population_result",
     "pda_population_stats", "pda_rb_forward", "pda_rb_get_result", "pda_rb_adjoint",
-    "pda_rb_stats", "pda_debug_primitive",
+    "pda_rb_jacobian", "pda_rb_stats", "pda_debug_primitive",
 )
 
 
@@ -116,6 +116,7 @@ def load_library(path=LIB_PATH):
                                  ctypes.c_uint32, ctypes.c_int]
     L.pda_rb_get_result.argtypes = [_vp, ctypes.c_int, _u64p, _f64p, ctypes.c_int, _f64p, _i32p]
     L.pda_rb_adjoint.argtypes = [_vp, ctypes.c_int, _f64p, ctypes.c_int, _f64p, ctypes.c_int]
+    L.pda_rb_jacobian.argtypes = [_vp, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int]
     L.pda_rb_stats.argtypes = [_vp, ctypes.c_int, _f32p, _f32p, _f32p]
     _lib = L
     return L
@@ -428,8 +429,15 @@ class MC:
                                       len(out)))
         return out
 
+    def rb_jacobian(self, n, nbins, sid=0):
+        """d M_b / d args of the last RB forward: array (nbins, n (n + 1)),
+        mc::set_params layout; g @ J equals rb_adjoint(g)."""
+        J = np.zeros((nbins, n * (n + 1)), np.float64)
+        _check(self._L.pda_rb_jacobian(self._h, sid, _p(J, _f64p), nbins, J.shape[1]))
+        return J
+
     def rb_stats(self, sid=0):
-        """GPU ms of the last RB walk, whole forward, adjoint."""
+        """GPU ms of the last RB walk, whole forward, last adjoint or Jacobian."""
         v = [ctypes.c_float() for _ in range(3)]
         _check(self._L.pda_rb_stats(self._h, sid, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)

@@ -233,9 +233,19 @@ int pda_rb_get_result(pda_ctx *ctx, int sid, uint64_t *M_fixed, double *M, int n
  * 0 < E < 1, rates > 0 and a fixed initial state. */
 int pda_rb_adjoint(pda_ctx *ctx, int sid, const double *g, int nbins, double *grad, int nargs);
 
+/* Jacobian of the slot's last RB forward run (blocking): J[b * nargs + i] =
+ * d M_b / d args_i for every bin at once, M_b the expected counts of
+ * pda_rb_get_result (summed over the R resamples), in the layout and with the
+ * estimators of pda_rb_adjoint, so sum_b g[b] J[b * nargs + i] is the
+ * adjoint's grad[i] for any cotangent g.  For Gauss-Newton / Levenberg-
+ * Marquardt fits of the binned chi2.  Same preconditions as pda_rb_adjoint;
+ * the per-bin scratch (about 170 B per bin) must fit the LDS next to the
+ * pmf buffers. */
+int pda_rb_jacobian(pda_ctx *ctx, int sid, double *J, int nbins, int nargs);
+
 /* GPU milliseconds of the slot's last RB runs: the walk, the whole forward
- * sequence, the adjoint kernel (0 before the first adjoint).  Outputs may be
- * NULL. */
+ * sequence, the last adjoint or Jacobian kernel (0 before the first).
+ * Outputs may be NULL. */
 int pda_rb_stats(pda_ctx *ctx, int sid, float *walk_ms, float *forward_ms, float *adjoint_ms);
 
 /* Diagnostics (no reference counterpart): evaluates one primitive of the

@@ -10,6 +10,9 @@ autograd Function, through the C ABI.
   1e-3 relative; the efficiency gradient equals central finite differences
   of the GPU forward (h = 1e-4 relative) within 1e-3; the rate gradient
   agrees with central differences over independent seeds.
+* Jacobian: g @ J equals the adjoint for random cotangents (both lane
+  layouts), and its rows equal the oracle's FP64 gradient for indicator
+  cotangents within 1e-3.
 * torch: the autograd Function's chi2 and gradient match the library's.
 """
 import numpy as np
@@ -115,6 +118,37 @@ def test_rb_adjoint_matches_fp64_oracle(gpu_mc, oracle):
     scale = np.abs(go).max()
     assert np.all(np.abs(grad[:9] - go) <= 1e-3 * np.maximum(np.abs(go), 1e-3 * scale)), (grad, go)
     assert np.all(grad[9:] == 0.0)
+
+
+@pytest.mark.parametrize("long_bursts", [False, True])
+def test_rb_jacobian_is_the_adjoint_of_every_cotangent(gpu_mc, oracle, long_bursts):
+    # long bursts (acceptor totals >= 256) take the one-unit-per-wave layout
+    if long_bursts:
+        d = synth.generate(60, synth.make_args([0.2, 0.55, 0.85], [900, 2500, 700, 3000, 1500, 5000]),
+                           3, base_photons=400, mean_geom=150, seed=57)
+    else:
+        d = _data(400, 53)
+    _load(gpu_mc, d)
+    R, seed = 8, 21
+    M, _, _, _ = _rb(gpu_mc, 3, ARGS3, R, seed, set_id=2)
+    J = gpu_mc.rb_jacobian(3, len(M))
+    assert J.shape == (50, 12) and np.all(J[:, 9:] == 0.0)
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        g = rng.normal(size=50)
+        adj = gpu_mc.rb_adjoint(g, 3)
+        scale = np.abs(adj).max()
+        # the adjoint rounds cotangent differences to float
+        assert np.all(np.abs(g @ J - adj) <= 1e-5 * np.maximum(np.abs(adj), 1e-2 * scale)), (g @ J, adj)
+    # rows against the oracle's FP64 forward-mode derivative (indicator cotangents)
+    od = oracle.Data.from_bursts(d)
+    p = oracle.make_params(3, ARGS3, *PH)
+    for b in np.argsort(-M)[:4]:
+        e = np.zeros(50)
+        e[b] = 1.0
+        _, go = oracle.rb_grad(od, p, R, seed, e, set_id=2)
+        sc = np.abs(go).max()
+        assert np.all(np.abs(J[b, :9] - go) <= 1e-3 * np.maximum(np.abs(go), 1e-3 * sc)), (b, J[b], go)
 
 
 def test_rb_efficiency_gradient_matches_central_differences(gpu_mc, oracle):
