@@ -11,7 +11,10 @@ C ABI in three forms:
 * `GeneticFitter(batched=True)` -- a generation per pda_run_population call
   (one graph replay);
 * `AdamFitter` -- Adam on (logit E, log K) with the gradient of the
-  Rao-Blackwellised chi2 from the adjoint kernel (gsmfretda_amd.rb_torch).
+  Rao-Blackwellised chi2 from the adjoint kernel (gsmfretda_amd.rb_torch);
+* `LMFitter` -- Levenberg-Marquardt on the same chi2 written as a sum of
+  squared Pearson residuals, with the Jacobian of the expected histogram
+  from one pass of the Jacobian kernel (pda_rb_jacobian).
 
 Every fitter reports progress in the same currency, the plain simulator's
 chi2 at the current best parameters with fixed random streams (seed, set id
@@ -252,4 +255,103 @@ class AdamFitter:
             if (step + 1) % self.check_every == 0:
                 th = theta.detach().numpy()
                 hist.record(time.perf_counter() - t0, prob.chi2(th), prob.evaluations, th)
+        return hist
+
+
+class LMFitter:
+    """Levenberg-Marquardt on (logit E, log K) for the Rao-Blackwellised chi2.
+
+    With m_b = M_b / R the expected counts of the kept bins (b >= left_pad,
+    M_b > 0), the chi2 is sum_b r_b^2 / dof with r_b = (O_b - m_b) / sqrt(m_b);
+    dr_b / dM_b = -(O_b + m_b) / (2 R m_b^1.5) and dM / dargs comes from
+    pda_rb_jacobian on the forward run at the current point, so one
+    accepted step costs one forward (with scores) and one Jacobian pass; a
+    rejected step costs one forward.  Marquardt damping: solve
+    (A + lam D) delta = -J^T r with A = J^T J and D = diag(A) floored at
+    1e-3 max diag(A) (rates the histogram barely sees would otherwise take
+    huge steps), lam / 3 after an accepted step, lam x 4 after a rejected
+    one; a step is also capped at `step_max` in every coordinate (logit E,
+    log K), doubled after an accepted step and halved after a rejected one
+    (within [step_min, 2]).  The RB chi2 with common random numbers is a
+    step function of the rates at fine scales, so the fit stops when the cap
+    falls below step_min.  Progress is reported as the plain chi2, like the
+    other fitters."""
+
+    def __init__(self, R_grad=None, lam0=1e-2, lam_max=1e8, step_max=0.5, step_min=1e-4,
+                 check_every=1, left_pad=2):
+        self.R_grad = R_grad
+        self.lam0 = lam0
+        self.lam_max = lam_max
+        self.step_max = step_max
+        self.step_min = step_min
+        self.check_every = check_every
+        self.left_pad = left_pad
+
+    def _forward(self, prob, theta, R):
+        """RB expected histogram (summed over R resamples) and its reduced chi2."""
+        args = theta_to_args(theta, prob.n)
+        if not prob.mc.set_params(prob.n, 0, args):
+            raise ValueError("set_params rejected the parameters")
+        prob.mc.rb_forward(R=R, seed=prob.seed, set_id=0, scores=True)
+        M, _, c2, _ = prob.mc.rb_result()
+        return M, c2
+
+    def _system(self, prob, theta, M, R):
+        """Normal equations of the residuals at the forward just run."""
+        n = prob.n
+        J = prob.mc.rb_jacobian(n, len(M))[:, :n * n]       # d M_b / d (E, K)
+        args = theta_to_args(theta, n).astype(np.float64)
+        E = args[:n]
+        dargs = np.concatenate([E * (1.0 - E), args[n:n * n]])   # d args / d theta
+        O = prob.observed
+        m = M / R
+        keep = (np.arange(len(M)) >= self.left_pad) & (M > 0)
+        r = np.zeros(len(M))
+        r[keep] = (O[keep] - m[keep]) / np.sqrt(m[keep])
+        dr = np.zeros(len(M))
+        dr[keep] = -(O[keep] + m[keep]) / (2.0 * R * m[keep] ** 1.5)
+        Jr = (dr[:, None] * J) * dargs[None, :]
+        return Jr.T @ Jr, Jr.T @ r
+
+    def run(self, prob, theta0, max_seconds, target=None, max_steps=100000):
+        n = prob.n
+        R = self.R_grad or prob.R
+        hist = History("lm-rb")
+        t0 = time.perf_counter()
+        theta = clip_theta(theta0, n)
+        hist.record(time.perf_counter() - t0, prob.chi2(theta), prob.evaluations, theta)
+        M, c2 = self._forward(prob, theta, R)
+        prob.evaluations += 1
+        A, g = self._system(prob, theta, M, R)
+        lam, cap = self.lam0, self.step_max
+        for step in range(max_steps):
+            if target is not None and hist.chi2[-1] <= target:
+                break
+            if time.perf_counter() - t0 > max_seconds or lam > self.lam_max or cap < self.step_min:
+                break
+            dA = np.diag(A)
+            D = np.diag(np.maximum(dA, 1e-3 * max(dA.max(), 1e-300)))
+            try:
+                delta = np.linalg.solve(A + lam * D, -g)
+            except np.linalg.LinAlgError:
+                lam *= 4.0
+                continue
+            big = np.abs(delta).max()
+            if big > cap:
+                delta *= cap / big
+            cand = clip_theta(theta + delta, n)
+            Mc, cc = self._forward(prob, cand, R)
+            prob.evaluations += 1
+            if np.isfinite(cc) and cc < c2:
+                theta, M, c2 = cand, Mc, cc
+                A, g = self._system(prob, theta, M, R)
+                lam = max(lam / 3.0, 1e-12)
+                cap = min(2.0 * cap, 2.0)
+                if (step + 1) % self.check_every == 0:
+                    hist.record(time.perf_counter() - t0, prob.chi2(theta), prob.evaluations, theta)
+            else:
+                lam *= 4.0
+                cap *= 0.5
+        if hist.best_theta is None or not np.array_equal(hist.best_theta, theta):
+            hist.record(time.perf_counter() - t0, prob.chi2(theta), prob.evaluations, theta)
         return hist

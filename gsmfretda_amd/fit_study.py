@@ -6,7 +6,7 @@ LogNormal(2 ms, 0.6) durations, 32 resamples per burst, 50 bins.  The
 observed histogram is the simulator at the true parameters with R = 1 and a
 held-out seed; chi2(truth) is the plain simulator's chi2 at the true
 parameters with the evaluation streams.  From the same initial population of
-64 random parameter sets (states from the fitted model's range), three
+64 random parameter sets (states from the fitted model's range), four
 fitters run under the same wall-clock limit:
 
   ga-per-set   genetic algorithm, one pda_run per individual (the reference's
@@ -15,11 +15,13 @@ fitters run under the same wall-clock limit:
   adam-rb      Adam on (logit E, log K) with the adjoint gradient of the
                Rao-Blackwellised chi2 (8 resamples per burst), started from
                the best initial set found by one batched evaluation
+  lm-rb        Levenberg-Marquardt on the same chi2 with the Jacobian of the
+               expected histogram (pda_rb_jacobian), same start
 
 Each is timed to reach chi2 <= 1.01 chi2(truth), and to reach within 1 % of
 the chi2 the per-set genetic algorithm ends with.
 
-    python -m gsmfretda_amd.fit_study [--bursts N] [--seconds S] [--out FILE]
+    python -m gsmfretda_amd.fit_study [--bursts N] [--seconds S] [--fitters a,b] [--out FILE]
 """
 import argparse
 import json
@@ -29,6 +31,28 @@ import time
 import numpy as np
 
 from . import fit, pda, synth
+
+
+def make_problem(m, cfg, nbins=50):
+    """Loads a synth.config data set into the MC context m and returns the
+    fit problem: the observed histogram is the simulator at the true
+    parameters with R = 1 and the held-out seed."""
+    d, n = cfg["data"], cfg["n"]
+    truth = np.asarray(cfg["args"], np.float32)
+    m.load(d, SgDivSr=None)
+    m.set_observed_hist(np.zeros(nbins, np.float32))
+    m.set_photophysics(0, *cfg["photophysics"])
+    assert m.set_params(n, 0, truth)
+    m.set_burst_range(0, -1, 0)
+    m.reSampleTimes = 1
+    m.run_kernel(sid=0, seed=cfg["seeds"]["observed"], samples=True)
+    m.synchronize(0)
+    obs_E, _, _ = m.get_samples(0, d.nbursts)
+    m.set_observed(obs_E, nbins, 2)
+    x = obs_E[(obs_E >= 0) & (obs_E < 1)].astype(np.float64)
+    observed = np.bincount(np.floor(x * nbins).astype(np.int64),
+                           minlength=nbins)[:nbins].astype(np.float64)
+    return fit.Problem(m, n, cfg["R"], cfg["seeds"]["mc"], observed)
 
 
 def main(argv=None):
@@ -42,7 +66,8 @@ def main(argv=None):
     ap.add_argument("--plateau", type=int, default=0,
                     help="Adam: halve the learning rate after this many steps without progress")
     ap.add_argument("--lr-decay", type=float, default=1.0, help="Adam: per-step decay")
-    ap.add_argument("--only-adam", action="store_true", help="skip the genetic fitters")
+    ap.add_argument("--fitters", default="ga-per-set,ga-batched,adam-rb,lm-rb",
+                    help="comma-separated subset of ga-per-set, ga-batched, adam-rb, lm-rb")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
 
@@ -55,31 +80,19 @@ def main(argv=None):
     truth = np.asarray(cfg["args"], np.float32)
     log(f"data: {d.nbursts} bursts, {d.nphotons} photons, {time.perf_counter() - t0:.1f} s")
     m = pda.MC(0, 1)
-    m.load(d, SgDivSr=None)
-    m.set_observed_hist(np.zeros(50, np.float32))
-    m.set_photophysics(0, *cfg["photophysics"])
-    assert m.set_params(n, 0, truth)
-    m.set_burst_range(0, -1, 0)
-    m.reSampleTimes = 1
-    m.run_kernel(sid=0, seed=cfg["seeds"]["observed"], samples=True)
-    m.synchronize(0)
-    obs_E, _, _ = m.get_samples(0, d.nbursts)
-    m.set_observed(obs_E, 50, 2)
-    x = obs_E[(obs_E >= 0) & (obs_E < 1)].astype(np.float64)
-    observed = np.bincount(np.floor(x * 50).astype(np.int64), minlength=50)[:50].astype(np.float64)
-
-    prob = fit.Problem(m, n, R, cfg["seeds"]["mc"], observed)
+    prob = make_problem(m, cfg)
     chi2_truth = prob.chi2(fit.args_to_theta(truth, n))
     target = 1.01 * chi2_truth
     log(f"chi2(truth) = {chi2_truth:.5f}, target {target:.5f}")
     pop0 = [fit.args_to_theta(args, n) for _, args in cfg["population"][:a.pop]]
 
     results = {}
-    ga_specs = [] if a.only_adam else [
+    fitters = a.fitters.split(",")
+    ga_specs = [s for s in [
         ("ga-per-set", lambda: fit.GeneticFitter(batched=False, pop_size=a.pop,
                                                  rng=np.random.default_rng(7))),
         ("ga-batched", lambda: fit.GeneticFitter(batched=True, pop_size=a.pop,
-                                                 rng=np.random.default_rng(7)))]
+                                                 rng=np.random.default_rng(7)))] if s[0] in fitters]
     for name, make in ga_specs:
         prob.evaluations = 0
         h = make().run(prob, pop0, a.seconds, target=target)
@@ -87,18 +100,23 @@ def main(argv=None):
         log(f"{name}: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s, "
             f"{prob.evaluations} evaluations, {len(h.times)} generations")
 
-    # Adam from the best initial set (found by one batched evaluation, timed)
-    prob.evaluations = 0
-    ts = time.perf_counter()
-    fit0 = prob.chi2_population(pop0)
-    start = pop0[int(np.argmin(fit0))]
-    t_init = time.perf_counter() - ts
-    h = fit.AdamFitter(lr=a.lr, R_grad=a.r_grad, check_every=5, plateau=a.plateau,
-                       lr_decay=a.lr_decay).run(prob, start, a.seconds - t_init, target=target)
-    h.times = [t + t_init for t in h.times]
-    results["adam-rb"] = h
-    log(f"adam-rb: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s, "
-        f"{prob.evaluations} evaluations")
+    # the gradient fitters start from the best initial set (found by one
+    # batched evaluation, timed)
+    grad_specs = [s for s in [
+        ("adam-rb", lambda: fit.AdamFitter(lr=a.lr, R_grad=a.r_grad, check_every=5,
+                                           plateau=a.plateau, lr_decay=a.lr_decay)),
+        ("lm-rb", lambda: fit.LMFitter(R_grad=a.r_grad))] if s[0] in fitters]
+    for name, make in grad_specs:
+        prob.evaluations = 0
+        ts = time.perf_counter()
+        fit0 = prob.chi2_population(pop0)
+        start = pop0[int(np.argmin(fit0))]
+        t_init = time.perf_counter() - ts
+        h = make().run(prob, start, a.seconds - t_init, target=target)
+        h.times = [t + t_init for t in h.times]
+        results[name] = h
+        log(f"{name}: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s, "
+            f"{prob.evaluations} evaluations")
 
     ga_final = results["ga-per-set"].chi2[-1] if "ga-per-set" in results else float("nan")
     out = {"workload": "4 states, rates 1e3-1e5, %d bursts x %d resamples, 50 bins" % (d.nbursts, R),

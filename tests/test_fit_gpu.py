@@ -1,0 +1,45 @@
+"""GPU: the fitters over the C ABI on a reduced fast-dynamics workload
+(4 states, rates 1e3-1e5 s^-1, 50,000 bursts x 32 resamples)."""
+import numpy as np
+import pytest
+
+from gsmfretda_amd import fit, fit_study, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def problem(gpu_mc):
+    cfg = synth.config(4, n_bursts=50000)
+    prob = fit_study.make_problem(gpu_mc, cfg)
+    truth = fit.args_to_theta(np.asarray(cfg["args"], np.float32), cfg["n"])
+    pop0 = [fit.args_to_theta(a, cfg["n"]) for _, a in cfg["population"][:16]]
+    return prob, truth, pop0
+
+
+def test_lm_reaches_the_truth_chi2(problem):
+    prob, truth, pop0 = problem
+    c_truth = prob.chi2(truth)
+    start = pop0[int(np.argmin(prob.chi2_population(pop0)))]
+    assert prob.chi2(start) > 2.0 * c_truth
+    h = fit.LMFitter(R_grad=8).run(prob, start, max_seconds=30.0, target=1.01 * c_truth)
+    assert h.chi2[-1] <= 1.01 * c_truth, (h.chi2[-5:], c_truth)
+    assert np.all(np.diff(h.chi2) <= 0)
+
+
+def test_lm_normal_equations_match_the_rb_chi2_in_e(problem):
+    # J^T r of the Jacobian kernel is half the gradient of sum r^2 = dof chi2
+    prob, truth, pop0 = problem
+    lm = fit.LMFitter(R_grad=8)
+    th = pop0[3]
+    M, c2 = lm._forward(prob, th, 8)
+    A, g = lm._system(prob, th, M, 8)
+    keep = (np.arange(len(M)) >= 2) & (M > 0)
+    dof = keep.sum() - prob.n * (prob.n + 1)
+    # the efficiency part of the gradient against central differences of the
+    # RB chi2 (smooth in E)
+    for i in range(prob.n):
+        e = np.zeros_like(th)
+        e[i] = 1e-3
+        fd = (lm._forward(prob, th + e, 8)[1] - lm._forward(prob, th - e, 8)[1]) / 2e-3
+        assert abs(2 * g[i] / dof - fd) <= 1e-2 * abs(fd), (i, 2 * g[i] / dof, fd)
