@@ -6,6 +6,13 @@ reference's full evaluation per step: 100 resamples per burst, the 50-bin E
 histogram and its chi2, all on the GPU, bursts already resident in HBM.
 One step = one parameter-set evaluation = 1e7 burst-resamples.
 
+Besides the headline number the line carries the VALU and HBM rooflines of
+the walk kernel, the batched-population leg (256 sets in one graph replay),
+the Rao-Blackwellised leg (forward, adjoint and Jacobian GPU times), the fit
+leg (wall seconds to 1.01 x chi2(truth) for the per-set genetic algorithm
+and the gradient fitters, 100,000 bursts) and the CPU baseline (the oracle
+on the host's cores).
+
 Prints one JSON line (rank 0).  --gpus above 1 is refused: this build runs
 on one GPU (the multi-GPU design is parameter-set or burst sharding with a
 histogram all-reduce).
@@ -128,7 +135,7 @@ def rb_leg(pda, synth, steps):
     x = obs_E[(obs_E >= 0) & (obs_E < 1)].astype(np.float64)
     O = np.bincount(np.floor(x * 50).astype(np.int64), minlength=50)[:50].astype(np.float64)
     seed = cfg["seeds"]["mc"]
-    fwd, adj, walk = [], [], []
+    fwd, adj, walk, jac = [], [], [], []
     for i in range(steps + 1):
         m.rb_forward(R=R, seed=seed, scores=True)
         M, _, c2, _ = m.rb_result()
@@ -138,19 +145,61 @@ def rb_leg(pda, synth, steps):
         g[keep] = (1.0 - O[keep] ** 2 / mm[keep] ** 2) / (R * (keep.sum() - n * (n + 1)))
         m.rb_adjoint(g, n)
         w, f, a = m.rb_stats()
+        m.rb_jacobian(n, 50)
+        j = m.rb_stats()[2]
         if i:
             fwd.append(f)
             adj.append(a)
             walk.append(w)
+            jac.append(j)
     m.close()
     units = d.nbursts * R
     return {"workload": "Rao-Blackwellised PDA, 3 states, 100000 bursts x 64 resamples, 50 bins: "
                         "expected histogram + chi2, then d chi2 / d(E, K)",
             "forward_units_per_s": units / (np.mean(fwd) * 1e-3),
             "forward_gpu_ms": float(np.mean(fwd)), "walk_gpu_ms": float(np.mean(walk)),
-            "adjoint_gpu_ms": float(np.mean(adj)),
+            "adjoint_gpu_ms": float(np.mean(adj)), "jacobian_gpu_ms": float(np.mean(jac)),
             "forward_plus_adjoint_units_per_s": units / ((np.mean(fwd) + np.mean(adj)) * 1e-3),
             "chi2": c2}
+
+
+def fit_leg(pda, synth, seconds):
+    """Fit wall time to 1.01 x chi2(truth) on the fast-dynamics workload at
+    100,000 bursts (4 states, rates 1e3-1e5 s^-1, 32 resamples): the
+    per-set genetic algorithm (one pda_run per individual, the reference's
+    pattern) against the gradient fitters on the RB form, same 64 starting
+    sets.  Each fitter stops at the target or after `seconds`."""
+    from gsmfretda_amd import fit, fit_study
+    cfg = synth.config(4, n_bursts=100_000)
+    n = cfg["n"]
+    m = pda.MC(0, 1)
+    prob = fit_study.make_problem(m, cfg)
+    c_truth = prob.chi2(fit.args_to_theta(np.asarray(cfg["args"], np.float32), n))
+    target = 1.01 * c_truth
+    pop0 = [fit.args_to_theta(a, n) for _, a in cfg["population"][:64]]
+    res = {}
+    h = fit.GeneticFitter(batched=False, pop_size=64, rng=np.random.default_rng(7)).run(
+        prob, pop0, seconds, target=target)
+    res["ga-per-set"] = h
+    for name, make in (("adam-rb", lambda: fit.AdamFitter(lr=0.1, R_grad=8, check_every=5)),
+                       ("lm-rb", lambda: fit.LMFitter(R_grad=8))):
+        t0 = time.perf_counter()
+        start = pop0[int(np.argmin(prob.chi2_population(pop0)))]
+        t_init = time.perf_counter() - t0
+        h = make().run(prob, start, seconds - t_init, target=target)
+        h.times = [t + t_init for t in h.times]
+        res[name] = h
+    m.close()
+    t = {k: v.time_to(target) for k, v in res.items()}
+    out = {"workload": "fit to 1.01 x chi2(truth): 4 states, rates 1e3-1e5 s^-1, 100000 bursts x 32 "
+                       "resamples, 50 bins, 64 starting sets; wall seconds per fitter",
+           "chi2_truth": c_truth, "seconds_limit": seconds, "seconds_to_target": t,
+           "final_chi2": {k: v.chi2[-1] for k, v in res.items()}}
+    if t["lm-rb"]:
+        ga = t["ga-per-set"]
+        out["speedup_lm_vs_ga_per_set"] = (ga if ga else seconds) / t["lm-rb"]
+        out["speedup_is_lower_bound"] = ga is None
+    return out
 
 
 def main():
@@ -161,6 +210,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-population", action="store_true",
                     help="skip the batched-population leg")
+    ap.add_argument("--no-fit", action="store_true", help="skip the fit wall-time leg")
+    ap.add_argument("--fit-seconds", type=float, default=30.0)
     ap.add_argument("--no-rb", action="store_true",
                     help="skip the Rao-Blackwellised (differentiable) leg")
     ap.add_argument("--cpu-bursts", type=int, default=100_000)
@@ -243,6 +294,8 @@ def main():
         out["population"] = population_leg(pda, synth, max(1, args.steps // 4), 1)
     if not args.no_rb:
         out["rao_blackwellised"] = rb_leg(pda, synth, max(2, args.steps // 4))
+    if not args.no_fit:
+        out["fit"] = fit_leg(pda, synth, args.fit_seconds)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, threads, args.cpu_bursts)
