@@ -74,14 +74,16 @@ Launch plan_launch(const pda_ctx* ctx, int pass, bool population, uint64_t units
     const uint64_t max_grid = uint64_t(ctx->num_cu) * uint64_t(per_cu);
     uint64_t batch = uint64_t(kWave) * ctx->shape.batch_tiles;
     if (ctx->shape.batch_tiles == 0) {
-        batch = (units / (max_grid * 2u) + kWave - 1) / kWave * kWave;
+        // about eight grabs per wave: small batches keep the grid's tail
+        // short (a wave that grabs last still walks a whole batch)
+        batch = (units / (max_grid * 8u) + kWave - 1) / kWave * kWave;
         batch = std::min<uint64_t>(std::max<uint64_t>(batch, kWave), 4096);
     }
     const uint64_t batches = (units + batch - 1) / batch;
     Launch L;
     L.grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, batches)));
     L.batch = uint32_t(batch);
-    const uint32_t q = ctx->shape.queues ? ctx->shape.queues : kMaxQueues;
+    const uint32_t q = ctx->shape.queues ? ctx->shape.queues : kDefaultQueues;
     L.queues = std::min<uint32_t>(q, L.grid);
     return L;
 }
@@ -202,8 +204,8 @@ int pda_create(int device, int nstreams, pda_ctx** out) {
             hipMalloc(&s.d_blob, sizeof(ModelBlob)) != hipSuccess ||
             hipMalloc(&s.d_replay_count, sizeof(uint32_t)) != hipSuccess ||
             hipMemset(s.d_replay_count, 0, sizeof(uint32_t)) != hipSuccess ||
-            hipMalloc(&s.d_work, sizeof(uint32_t) * kMaxQueues) != hipSuccess ||
-            hipMemset(s.d_work, 0, sizeof(uint32_t) * kMaxQueues) != hipSuccess ||
+            hipMalloc(&s.d_work, sizeof(uint32_t) * kWorkWords) != hipSuccess ||
+            hipMemset(s.d_work, 0, sizeof(uint32_t) * kWorkWords) != hipSuccess ||
             hipEventCreateWithFlags(&s.model_copied, hipEventDisableTiming) != hipSuccess ||
             hipMalloc(&s.d_hist_acc, sizeof(int32_t) * kMaxBins * 8) != hipSuccess ||
 

@@ -65,7 +65,7 @@ struct Population {
     uint64_t* d_replay = nullptr;
     uint32_t* d_replay_count = nullptr;
     size_t replay_cap = 0;
-    uint32_t* d_work = nullptr;      // kMaxQueues counters
+    uint32_t* d_work = nullptr;      // kWorkWords: queue counters, one per line
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     PopGraphKey key;
@@ -89,7 +89,7 @@ struct RbState {
     float* d_S = nullptr;              // per unit: scores of the rates
     size_t cap_units = 0, cap_scores = 0;
     unsigned long long* d_hist = nullptr;  // [8][kMaxBins]
-    uint32_t* d_work = nullptr;        // 9 counters
+    uint32_t* d_work = nullptr;        // kWorkWords: walk queues + pmf burst counter
     uint8_t* d_res = nullptr;          // header + uint64 M[kMaxBins]
     uint8_t* h_res = nullptr;          // pinned
     double* d_g = nullptr;             // cotangent per bin
@@ -115,7 +115,7 @@ struct Slot {
     bool any_v = false;           // broadening: the general pass walks every unit
     uint32_t* d_replay = nullptr;  // units the fast pass abandoned
     uint32_t* d_replay_count = nullptr;
-    uint32_t* d_work = nullptr;    // work counters of the run (one per XCD)
+    uint32_t* d_work = nullptr;    // kWorkWords: work counters of the run, one per line
     size_t replay_cap = 0;
     hipEvent_t model_copied = nullptr;
     bool model_pending = false;

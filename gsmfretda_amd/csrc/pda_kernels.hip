@@ -3,7 +3,7 @@
 // walk_kernel + finish_kernel replace gSMFRETda's mc_kernel (src/mc.cu:83-197)
 // together with the host histogram and chi2 of src/streamWorker.cpp:184-206:
 //   * one-wave workgroups; every lane streams through (burst, resample)
-//     units taken from XCD-local work counters, advancing one dwell window
+//     units taken from per-XCD work counters, advancing one dwell window
 //     per loop pass, so lanes and waves never wait for the longest unit;
 //   * each window end is located by a binary search over burst-relative
 //     uint32 ticks and the window's sticky-flag count read from per-photon
@@ -78,8 +78,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     // whose unit ends starts another in the same pass, so lanes do not wait
     // for the wave's longest unit and waves do not wait for the grid's
     // slowest batch.  The units are split into a.queues contiguous ranges
-    // with a counter each; workgroup g draws from queue g % a.queues (its own
-    // XCD's counter when a.queues == 8).  Photons are read through L1/L2.
+    // with a counter each, every counter on its own 256-byte line (device-
+    // scope atomics to one line serialise); workgroup g draws from queue
+    // g % a.queues, so with a multiple of 8 queues each queue is served by
+    // one XCD and its bursts stay in that XCD's L2.  Batches are small
+    // (about eight grabs per wave): the wave that grabs last still walks a
+    // whole batch, so the batch sets the grid's tail.  Photons are read
+    // through L1/L2.
     constexpr bool kFastForm = kPass == kPassFast;
     const BinomTable* tab = &blob->tab;
     const uint32_t queue = blockIdx.x % a.queues;
@@ -99,7 +104,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
             if (want == 0) break;
             if (pool_next >= pool_end) {
                 uint32_t off = 0;
-                if (lane == 0) off = atomicAdd(&work[queue], a.batch);
+                if (lane == 0) off = atomicAdd(&work[queue * kCounterStride], a.batch);
                 off = __shfl(off, 0);
                 if (off >= q_end - q_begin) {
                     drained = true;
@@ -252,7 +257,7 @@ __global__ __launch_bounds__(kWave * kFinishWaves) void finish_kernel(
         result->pad = 0;
         atomicExch(replay_count, 0u);
     }
-    if (tid < kMaxQueues) atomicExch(&work[tid], 0u);
+    if (tid < kMaxQueues) atomicExch(&work[tid * kCounterStride], 0u);
 }
 
 // ---- population runs ---------------------------------------------------------
@@ -329,7 +334,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
             if (want == 0) break;
             if (pool_next >= pool_end) {
                 uint32_t k = 0;
-                if (lane == 0) k = atomicAdd(&work[queue], 1u);
+                if (lane == 0) k = atomicAdd(&work[queue * kCounterStride], 1u);
                 k = __shfl(k, 0);
                 if (k >= q_end - q_begin) {
                     drained = true;
@@ -489,7 +494,7 @@ __global__ __launch_bounds__(kWave) void pop_finish_kernel(
             st.pad[0] = st.pad[1] = 0;
             *status = st;
         }
-        if (lane < kMaxQueues) atomicExch(&work[lane], 0u);
+        for (uint32_t q = lane; q < kMaxQueues; q += kWave) atomicExch(&work[q * kCounterStride], 0u);
     }
 }
 

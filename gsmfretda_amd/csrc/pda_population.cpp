@@ -80,8 +80,8 @@ size_t out_bytes(size_t nsets, int B) {
 int ensure_pop_buffers(pda_ctx* ctx, Slot& s, int B) {
     Population& p = s.pop;
     if (!p.d_work) {
-        HIP_TRY(hipMalloc(&p.d_work, sizeof(uint32_t) * kMaxQueues));
-        HIP_TRY(hipMemset(p.d_work, 0, sizeof(uint32_t) * kMaxQueues));
+        HIP_TRY(hipMalloc(&p.d_work, sizeof(uint32_t) * kWorkWords));
+        HIP_TRY(hipMemset(p.d_work, 0, sizeof(uint32_t) * kWorkWords));
         HIP_TRY(hipMalloc(&p.d_replay_count, sizeof(uint32_t)));
         HIP_TRY(hipMemset(p.d_replay_count, 0, sizeof(uint32_t)));
         HIP_TRY(hipEventCreate(&p.ev0));
@@ -455,7 +455,7 @@ int pda_autotune(pda_ctx* ctx, int sid, int population, int R, int reps, int* ba
         ctx->shape = saved;
         return rc;
     }
-    const uint32_t tiles[] = {2, 4, 8, 16, 32, 64, 128};
+    const uint32_t tiles[] = {1, 2, 3, 4, 6, 8, 16, 32, 64};
     for (uint32_t bt : tiles) {
         LaunchShape c = best;
         c.batch_tiles = bt;
@@ -478,7 +478,7 @@ int pda_autotune(pda_ctx* ctx, int sid, int population, int R, int reps, int* ba
             best = c;
         }
     }
-    const uint32_t qs[] = {1, 2, 4};
+    const uint32_t qs[] = {8, 16, 64};
     for (uint32_t q : qs) {
         if (rc) break;
         LaunchShape c = best;

@@ -25,7 +25,8 @@
 
 namespace pda {
 
-constexpr uint32_t kRbPmfCounter = 8;  // work[8]: bursts taken by rb_pmf_kernel
+// counter kMaxQueues (after the walk's queues): bursts taken by rb_pmf_kernel
+constexpr uint32_t kRbPmfCounter = kMaxQueues * kCounterStride;
 
 // Jacobian scratch in LDS (doubles, after the float buffers): per bin the
 // efficiency and rate derivatives, the current unit's histogram per lane
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(kWave) void rb_walk_kernel(
             if (want == 0) break;
             if (pool_next >= pool_end) {
                 uint32_t off = 0;
-                if (lane == 0) off = atomicAdd(&work[queue], a.batch);
+                if (lane == 0) off = atomicAdd(&work[queue * kCounterStride], a.batch);
                 off = __shfl(off, 0);
                 if (off >= q_end - q_begin) {
                     drained = true;
@@ -493,7 +494,7 @@ __global__ __launch_bounds__(kWave) void rb_finish_kernel(
         res->effN = effN;
         res->B = a.B;
     }
-    if (lane <= kRbPmfCounter) atomicExch(&work[lane], 0u);
+    for (uint32_t q = lane; q <= kMaxQueues; q += kWave) atomicExch(&work[q * kCounterStride], 0u);
 }
 
 hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu) {

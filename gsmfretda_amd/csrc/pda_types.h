@@ -10,7 +10,14 @@ constexpr int kWave = 64;            // lanes per workgroup of the walk kernel
 constexpr uint32_t kTickSentinel = 0x80000000u;  // "never closes a window"
 constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
 constexpr int kBinomTable = 128;     // photon counts with tabulated inversion constants
-constexpr uint32_t kMaxQueues = 8;   // work counters of a run (one per XCD)
+constexpr uint32_t kMaxQueues = 64;     // work counters of a run
+constexpr uint32_t kDefaultQueues = 32;  // four per XCD unless tuned
+// Each work counter sits on its own 256-byte line: device-scope atomics on
+// one line serialise, and counters sharing a line would serialise together.
+constexpr uint32_t kCounterStride = 64;  // uint32 words between counters
+// Counter words of a run: kMaxQueues queue counters and one spare (the RB
+// pmf pass's burst counter), each on its own line.
+constexpr size_t kWorkWords = size_t(kMaxQueues + 1) * kCounterStride;
 
 // One burst, original order (the range [begin, end) of mc::setBurstBd
 // addresses these directly).  16 bytes, one dwordx4 load.

@@ -154,9 +154,10 @@ int pda_get_tuning(const pda_ctx *ctx, int *tiles_per_chunk, int *lds_photons);
 /* Launch shape of the walk kernels, single-set and population runs (no
  * reference counterpart; the reference sizes its grid with
  * cudaOccupancyMaxPotentialBlockSize, src/mc.cu:309): batch_tiles x 64 units
- * per work-counter grab (0 = sized from the run), resident one-wave
- * workgroups per CU (0 = as many as fit), work counters (0 = 8, one per
- * XCD).  pda_set_tuning's tiles_per_chunk is the same knob as batch_tiles.
+ * per work-counter grab (0 = sized from the run, about eight grabs per
+ * wave), resident one-wave workgroups per CU (0 = as many as fit), work
+ * counters, each on its own 256-byte line (0 = 32, four per XCD; at most
+ * 64).  pda_set_tuning's tiles_per_chunk is the same knob as batch_tiles.
  * Results do not depend on the shape. */
 int pda_set_launch_shape(pda_ctx *ctx, int batch_tiles, int waves_per_cu, int queues);
 int pda_get_launch_shape(const pda_ctx *ctx, int *batch_tiles, int *waves_per_cu, int *queues);
