@@ -74,9 +74,9 @@ Launch plan_launch(const pda_ctx* ctx, int pass, bool population, uint64_t units
     const uint64_t max_grid = uint64_t(ctx->num_cu) * uint64_t(per_cu);
     uint64_t batch = uint64_t(kWave) * ctx->shape.batch_tiles;
     if (ctx->shape.batch_tiles == 0) {
-        // about eight grabs per wave: small batches keep the grid's tail
+        // about six grabs per wave: small batches keep the grid's tail
         // short (a wave that grabs last still walks a whole batch)
-        batch = (units / (max_grid * 8u) + kWave - 1) / kWave * kWave;
+        batch = (units / (max_grid * 6u) + kWave - 1) / kWave * kWave;
         batch = std::min<uint64_t>(std::max<uint64_t>(batch, kWave), 4096);
     }
     const uint64_t batches = (units + batch - 1) / batch;

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     // scope atomics to one line serialise); workgroup g draws from queue
     // g % a.queues, so with a multiple of 8 queues each queue is served by
     // one XCD and its bursts stay in that XCD's L2.  Batches are small
-    // (about eight grabs per wave): the wave that grabs last still walks a
+    // (about six grabs per wave): the wave that grabs last still walks a
     // whole batch, so the batch sets the grid's tail.  Photons are read
     // through L1/L2.
     constexpr bool kFastForm = kPass == kPassFast;

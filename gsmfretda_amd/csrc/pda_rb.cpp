@@ -208,7 +208,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     a.cap = ctx->cap;
     a.nstates = s->n;
     a.nsets = 1;
-    // walk grid: persistent, batches of about eight grabs per wave
+    // walk grid: persistent, batches of about six grabs per wave
     int per_cu = 0;
     HIP_TRY(rb_occupancy(scores ? 1 : 0, 0, &per_cu));
     per_cu = std::max(per_cu, 1);
@@ -216,7 +216,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     const uint64_t max_grid = uint64_t(ctx->num_cu) * uint64_t(per_cu);
     uint64_t batch = uint64_t(kWave) * ctx->shape.batch_tiles;
     if (ctx->shape.batch_tiles == 0) {
-        batch = (uint64_t(units) / (max_grid * 8u) + kWave - 1) / kWave * kWave;
+        batch = (uint64_t(units) / (max_grid * 6u) + kWave - 1) / kWave * kWave;
         batch = std::min<uint64_t>(std::max<uint64_t>(batch, kWave), 4096);
     }
     const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, (uint64_t(units) + batch - 1) / batch)));

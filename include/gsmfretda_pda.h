@@ -154,7 +154,7 @@ int pda_get_tuning(const pda_ctx *ctx, int *tiles_per_chunk, int *lds_photons);
 /* Launch shape of the walk kernels, single-set and population runs (no
  * reference counterpart; the reference sizes its grid with
  * cudaOccupancyMaxPotentialBlockSize, src/mc.cu:309): batch_tiles x 64 units
- * per work-counter grab (0 = sized from the run, about eight grabs per
+ * per work-counter grab (0 = sized from the run, about six grabs per
  * wave), resident one-wave workgroups per CU (0 = as many as fit), work
  * counters, each on its own 256-byte line (0 = 32, four per XCD; at most
  * 64).  pda_set_tuning's tiles_per_chunk is the same knob as batch_tiles.
