@@ -240,6 +240,25 @@ def test_launch_shape_does_not_change_results(gpu_mc, oracle):
         gpu_mc.set_tuning(*saved)
 
 
+def test_work_counters_do_not_change_results(gpu_mc, oracle):
+    """Every queue count up to 64 (each counter on its own line) and batch
+    size gives the oracle's units, twice in a row (the finish kernel leaves
+    all counters zero for the next run)."""
+    cfg = synth.config(0, n_bursts=900)
+    d = cfg["data"]
+    _load(gpu_mc, d, np.zeros(900, np.float32))
+    o = _oracle_run(oracle, d, cfg["n"], cfg["args"], cfg["photophysics"], 11, 23)
+    saved = gpu_mc.get_launch_shape()
+    try:
+        for tiles, q in ((1, 64), (2, 3), (4, 32), (0, 1), (16, 64), (0, 0)):
+            gpu_mc.set_launch_shape(batch_tiles=tiles, queues=q)
+            for _ in range(2):
+                g = _gpu_run(gpu_mc, cfg["n"], cfg["args"], 11, 23, ph=cfg["photophysics"])
+                _compare_units(g, o)
+    finally:
+        gpu_mc.set_launch_shape(*saved)
+
+
 def test_mc_shim_dropin(oracle, tmp_path):
     """The reference's call sequence through the C++ `class mc` shim
     (tests/host/mc_dropin.cpp): hmcE bit-exact with the oracle, and the

@@ -76,6 +76,21 @@ def test_rb_forward_bit_exact(gpu_mc, oracle, n, bg, rng_):
     gpu_mc.set_burst_range(0, -1, 0)
 
 
+def test_rb_forward_does_not_depend_on_the_launch_shape(gpu_mc):
+    d = _data(300, 37)
+    _load(gpu_mc, d)
+    saved = gpu_mc.get_launch_shape()
+    try:
+        got = []
+        for tiles, q in ((0, 0), (1, 64), (3, 5), (16, 1)):
+            gpu_mc.set_launch_shape(batch_tiles=tiles, queues=q)
+            got.append(_rb(gpu_mc, 3, ARGS3, 10, 91, set_id=1)[1])
+        for g in got[1:]:
+            assert np.array_equal(g, got[0])
+    finally:
+        gpu_mc.set_launch_shape(*saved)
+
+
 def test_rb_is_the_expectation_of_the_plain_histogram(gpu_mc):
     d = _data(3000, 41)
     _load(gpu_mc, d)
