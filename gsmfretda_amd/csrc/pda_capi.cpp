@@ -61,9 +61,10 @@ void range_of(const pda_ctx* ctx, const Slot& s, int64_t* b0, int64_t* b1) {
 
 // Persistent grid of one-wave workgroups, as many as fit on the device (or
 // the tuned number per CU); batches of units per work-counter grab: the tuned
-// size, or about two grabs per wave (a grab is a round trip to the counter
-// that stalls the whole wave; measured slower below ~512 units on 1e7-unit
-// runs).
+// size, or about six grabs per wave.  A grab is a round trip to the counter
+// that stalls the whole wave, but with every counter on its own line it is
+// short, and smaller batches shorten the grid's tail (on 1e7-unit runs
+// 256-unit batches beat both 64 and 700).
 Launch plan_launch(const pda_ctx* ctx, int pass, bool population, uint64_t units) {
     int per_cu = 0;
     hipError_t e = population ? pop_walk_occupancy(pass, ctx->B, &per_cu)
