@@ -98,6 +98,17 @@ struct RbState {
     float* d_pmf_tab = nullptr;        // binomial pmfs of the states, counts < 256
     double* d_jac = nullptr;           // [kMaxBins][n + K] rows of 16
     double* h_jac = nullptr;           // pinned
+    // each unit's pmf from the last forward with scores, for the derivative
+    // passes: unit r of burst b0 + i at d_pmf + poff[i] + r * (bound_i + 1)
+    float* d_pmf = nullptr;
+    size_t cap_pmf = 0;                // floats allocated
+    uint64_t* d_poff = nullptr;        // nbursts + 1 offsets of the run
+    size_t cap_poff = 0;
+    std::vector<uint64_t> h_poff;
+    int64_t poff_b0 = -1, poff_b1 = -1;
+    int poff_R = 0;
+    float poff_coef = -1.0f;
+    bool pmf_valid = false;
     double* h_grad = nullptr;          // pinned
     hipEvent_t ev0 = nullptr, ev_walk = nullptr, ev1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;
     bool ran = false, scores = false, adj_ran = false;
@@ -218,7 +229,8 @@ hipError_t launch_rb_pmf(int mode, uint32_t grid, int cap_big, const RbModel* rb
                          const double* lf, const float* pmf_tab, const RunArgs& a,
                          const BurstDesc* bursts, const uint2* cnt, const float* F,
                          const float* S, const double* g, unsigned long long* hist_acc,
-                         double* grad, uint32_t* work, hipStream_t stream);
+                         double* grad, uint32_t* work, float* pmf_io, const uint64_t* poff,
+                         hipStream_t stream);
 hipError_t launch_rb_finish(const RunArgs& a, const float* obs, unsigned long long* hist_acc,
                             uint32_t* work, void* res, hipStream_t stream);
 

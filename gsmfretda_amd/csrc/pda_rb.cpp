@@ -40,6 +40,8 @@ void free_rb(RbState& r) {
     (void)hipFree(r.d_pmf_tab);
     (void)hipFree(r.d_jac);
     (void)hipHostFree(r.h_jac);
+    (void)hipFree(r.d_pmf);
+    (void)hipFree(r.d_poff);
     (void)hipHostFree(r.h_grad);
     for (hipEvent_t e : {r.ev0, r.ev_walk, r.ev1, r.ev_a0, r.ev_a1})
         if (e) (void)hipEventDestroy(e);
@@ -60,6 +62,49 @@ struct RbResultHeaderH {  // device RbResultHeader
 constexpr size_t kRbResBytes = sizeof(RbResultHeaderH) + sizeof(uint64_t) * kMaxBins;
 constexpr int kGradLen = kRbMaxStates + kRbMaxPairs;
 constexpr size_t kRbCounters = kWorkWords;  // queue counters + the pmf pass's
+
+// Per-burst offsets of the unit pmf store for bursts [b0, b1) with R
+// resamples (each unit of burst b gets rb_total_bound(P_b) + 1 floats), and
+// the store itself.  When the device cannot hold the store, d_pmf stays
+// null and the derivative passes rebuild each pmf (same values).
+int ensure_pmf_store(pda_ctx* ctx, RbState& r, int64_t b0, int64_t b1, int R, float coef,
+                     hipStream_t stream) {
+    if (r.poff_b0 != b0 || r.poff_b1 != b1 || r.poff_R != R || r.poff_coef != coef) {
+        const size_t nb = size_t(b1 - b0);
+        r.h_poff.assign(nb + 1, 0u);
+        for (size_t i = 0; i < nb; ++i)
+            r.h_poff[i + 1] = r.h_poff[i] + uint64_t(R) *
+                uint64_t(rb_total_bound(ctx->h_P[size_t(b0) + i], coef) + 1);
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (nb + 1 > r.cap_poff) {
+            (void)hipFree(r.d_poff);
+            r.d_poff = nullptr;
+            r.cap_poff = 0;
+            HIP_TRY(hipMalloc(&r.d_poff, sizeof(uint64_t) * (nb + 1)));
+            r.cap_poff = nb + 1;
+        }
+        HIP_TRY(hipMemcpy(r.d_poff, r.h_poff.data(), sizeof(uint64_t) * (nb + 1),
+                          hipMemcpyHostToDevice));
+        r.poff_b0 = b0;
+        r.poff_b1 = b1;
+        r.poff_R = R;
+        r.poff_coef = coef;
+    }
+    const size_t need = size_t(r.h_poff.back());
+    if (need > r.cap_pmf || !r.d_pmf) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        (void)hipFree(r.d_pmf);
+        r.d_pmf = nullptr;
+        r.cap_pmf = 0;
+        if (hipMalloc(&r.d_pmf, sizeof(float) * std::max<size_t>(need, 1)) != hipSuccess) {
+            (void)hipGetLastError();  // out of memory: no store
+            r.d_pmf = nullptr;
+        } else {
+            r.cap_pmf = need;
+        }
+    }
+    return PDA_OK;
+}
 
 int ensure_rb(pda_ctx* ctx, RbState& r) {
     if (!r.d_work) {
@@ -228,6 +273,12 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     const uint32_t pgrid = uint32_t(std::max<int64_t>(
         1, std::min<int64_t>(int64_t(ctx->num_cu) * per_cu, b1 - b0)));
 
+    float* pmf_io = nullptr;
+    if (scores) {
+        int rc = ensure_pmf_store(ctx, r, b0, b1, R, hm->coef, s->stream);
+        if (rc) return rc;
+        pmf_io = r.d_pmf;  // null when the store did not fit: derivatives rebuild
+    }
     HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
     HIP_TRY(hipMemcpyAsync(r.d_rbm, r.h_rbm, sizeof(RbModel), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipEventRecord(r.ev0, s->stream));
@@ -239,7 +290,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
                             s->stream));
     HIP_TRY(launch_rb_pmf(0, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
                           r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, r.d_hist, nullptr,
-                          r.d_work, s->stream));
+                          r.d_work, pmf_io, r.d_poff, s->stream));
     HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s->stream));
     HIP_TRY(hipEventRecord(r.ev1, s->stream));
     HIP_TRY(hipMemcpyAsync(r.h_res, r.d_res, sizeof(RbResultHeaderH) + sizeof(uint64_t) * size_t(a.B),
@@ -247,6 +298,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     r.ran = true;
     r.adj_ran = false;
     r.scores = scores;
+    r.pmf_valid = pmf_io != nullptr;
     r.R = R;
     r.B = a.B;
     r.left_pad = a.left_pad;
@@ -347,7 +399,7 @@ int pda_rb_adjoint(pda_ctx* ctx, int sid, const double* g, int nbins, double* gr
     HIP_TRY(hipEventRecord(r.ev_a0, s->stream));
     HIP_TRY(launch_rb_pmf(1, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
                           r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, r.d_g, nullptr, r.d_grad,
-                          r.d_work, s->stream));
+                          r.d_work, r.pmf_valid ? r.d_pmf : nullptr, r.d_poff, s->stream));
     HIP_TRY(hipEventRecord(r.ev_a1, s->stream));
     HIP_TRY(hipMemcpyAsync(r.h_grad, r.d_grad, sizeof(double) * kGradLen, hipMemcpyDeviceToHost,
                            s->stream));
@@ -378,7 +430,7 @@ int pda_rb_jacobian(pda_ctx* ctx, int sid, double* J, int nbins, int nargs) {
     HIP_TRY(hipEventRecord(r.ev_a0, s->stream));
     HIP_TRY(launch_rb_pmf(2, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
                           r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, nullptr,
-                          r.d_jac, r.d_work, s->stream));
+                          r.d_jac, r.d_work, r.pmf_valid ? r.d_pmf : nullptr, r.d_poff, s->stream));
     HIP_TRY(hipEventRecord(r.ev_a1, s->stream));
     HIP_TRY(hipMemcpyAsync(r.h_jac, r.d_jac, bytes, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));

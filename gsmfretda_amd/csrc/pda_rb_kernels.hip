@@ -189,12 +189,17 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
                                          const float* __restrict__ S_in, const double* s_g,
                                          unsigned long long* s_h, float* s_buf, int cap,
                                          uint32_t lane, double& gp_acc, double& gk_acc,
-                                         const JacLds& jl) {
+                                         const JacLds& jl, float* __restrict__ pmf_io,
+                                         const uint64_t* __restrict__ poff) {
     constexpr bool kGrad = kMode != kRbFwd;
     constexpr int G = kWave / L;
     const int g = int(lane) / L, lg = int(lane) % L;
     const int n = sR.n, K = sR.K, B = a.B;
     float* base = s_buf + g * 3 * cap;
+    // pmf_io: the forward stores each unit's pmf at poff[bl] + r * stride,
+    // the derivative passes read it back instead of rebuilding it
+    const uint64_t slot0 = pmf_io ? poff[bl] : 0u;
+    const uint64_t stride = pmf_io ? (poff[bl + 1] - slot0) / a.R : 0u;
     double H = 0.0, Sg = 0.0, Ac = 0.0;  // lane lg of each group: rate pair lg
     for (uint32_t r0 = 0; r0 < a.R; r0 += G) {
         const uint32_t r = r0 + uint32_t(g);
@@ -205,38 +210,48 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
         int bX = 0, bZ = 1, bY = 2;
         int xl = 1;
         if (lg == 0) base[0] = 1.0f;
+        if (kGrad && pmf_io) {
+            // the pmf of the forward run: length 1 + sum of the counts
+            for (int s = 0; s < kRbMaxStates; ++s) xl += (act && s < n) ? int(rb_count(c, s)) : 0;
+            const float* src = pmf_io + slot0 + uint64_t(r) * stride;
+            for (int q = lg; act && q < xl; q += L) base[q] = src[q];
+            __syncthreads();
+        } else {
 #pragma unroll
-        for (int s = 0; s < kRbMaxStates; ++s) {
-            const int m = (act && s < n) ? int(rb_count(c, s)) : 0;
-            float* Y = base + bY * cap;
-            for (int k = lg; m > 0 && k <= m; k += L)
-                Y[k] = m < kRbPmfTable
-                           ? pmf_tab[rb_pmf_index(s, m, k)]
-                           : float(rb_binom_pmf(m, k, lf, sR.p[s], sR.lp[s], sR.lq[s]));
-            __syncthreads();
-            if (m > 0) {
-                if (xl == 1) {  // X = [1]: the product is Y itself, bit for bit
-                    const int t = bX;
-                    bX = bY;
-                    bY = t;
-                    xl = m + 1;
-                } else {
-                    const float* X = base + bX * cap;
-                    float* Z = base + bZ * cap;
-                    for (int q = lg; q < xl + m; q += L) Z[q] = rb_conv_at(Y, m + 1, X, xl, q);
-                    const int t = bX;
-                    bX = bZ;
-                    bZ = t;
-                    xl += m;
+            for (int s = 0; s < kRbMaxStates; ++s) {
+                const int m = (act && s < n) ? int(rb_count(c, s)) : 0;
+                float* Y = base + bY * cap;
+                for (int k = lg; m > 0 && k <= m; k += L)
+                    Y[k] = m < kRbPmfTable
+                               ? pmf_tab[rb_pmf_index(s, m, k)]
+                               : float(rb_binom_pmf(m, k, lf, sR.p[s], sR.lp[s], sR.lq[s]));
+                __syncthreads();
+                if (m > 0) {
+                    if (xl == 1) {  // X = [1]: the product is Y itself, bit for bit
+                        const int t = bX;
+                        bX = bY;
+                        bY = t;
+                        xl = m + 1;
+                    } else {
+                        const float* X = base + bX * cap;
+                        float* Z = base + bZ * cap;
+                        for (int q = lg; q < xl + m; q += L) Z[q] = rb_conv_at(Y, m + 1, X, xl, q);
+                        const int t = bX;
+                        bX = bZ;
+                        bZ = t;
+                        xl += m;
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
         const float* P = base + bX * cap;
         if (!kGrad) {
+            float* dst = pmf_io ? pmf_io + slot0 + uint64_t(r) * stride : nullptr;
             for (int q = lg; act && q < xl; q += L) {
                 const int bin = rb_bin(q, F, B);
                 if (bin >= 0) atomicAdd(&s_h[bin], (unsigned long long)rb_fixed(P[q]));
+                if (dst) dst[q] = P[q];
             }
         } else {
             float* dG = base + bZ * cap;  // G(c + 1) - G(c)
@@ -369,7 +384,8 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
     const uint2* __restrict__ cnt_in, const float* __restrict__ F_in,
     const float* __restrict__ S_in, const double* __restrict__ g,
     unsigned long long* __restrict__ hist_acc, double* __restrict__ grad,
-    uint32_t* __restrict__ work, int cap_big) {
+    uint32_t* __restrict__ work, int cap_big, float* __restrict__ pmf_io,
+    const uint64_t* __restrict__ poff) {
     constexpr bool kGrad = kMode != kRbFwd;
     extern __shared__ double s_dyn[];
     __shared__ RbModel sR;
@@ -406,10 +422,11 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
         const int bound = rb_total_bound(bursts[a.begin + bl].P, sR.coef);
         if (bound < kRbSmallCap)
             rb_burst<kRbSmallLanes, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h,
-                                           s_buf, kRbSmallCap, lane, gp_acc, gk_acc, jl);
+                                           s_buf, kRbSmallCap, lane, gp_acc, gk_acc, jl, pmf_io,
+                                           poff);
         else
             rb_burst<kWave, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf,
-                                   cap_big, lane, gp_acc, gk_acc, jl);
+                                   cap_big, lane, gp_acc, gk_acc, jl, pmf_io, poff);
     }
     if (kMode == kRbJac) {
         // grad holds J[b][i], i < n + K (efficiencies, then rates in args
@@ -537,17 +554,21 @@ hipError_t launch_rb_pmf(int mode, uint32_t grid, int cap_big, const RbModel* rb
                          const double* lf, const float* pmf_tab, const RunArgs& a,
                          const BurstDesc* bursts, const uint2* cnt, const float* F,
                          const float* S, const double* g, unsigned long long* hist_acc,
-                         double* grad, uint32_t* work, hipStream_t stream) {
+                         double* grad, uint32_t* work, float* pmf_io, const uint64_t* poff,
+                         hipStream_t stream) {
     const size_t lds = rb_pmf_lds(a.B, cap_big, mode);
     if (mode == kRbGrad)
         hipLaunchKernelGGL(rb_pmf_kernel<kRbGrad>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
-                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
+                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big,
+                           pmf_io, poff);
     else if (mode == kRbJac)
         hipLaunchKernelGGL(rb_pmf_kernel<kRbJac>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
-                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
+                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big,
+                           pmf_io, poff);
     else
         hipLaunchKernelGGL(rb_pmf_kernel<kRbFwd>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
-                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big);
+                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big,
+                           pmf_io, poff);
     return hipGetLastError();
 }
 
