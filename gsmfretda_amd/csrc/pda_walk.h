@@ -24,16 +24,17 @@ namespace pda {
 constexpr uint32_t kMaxWindows = 1u << 22;
 
 // drawDisIdx (src/gen_rand.cuh:35-50) on precomputed running sums: the first
-// i with cum[i] > u, else n - 1.
+// i with cum[i] > u, else n - 1.  n is the same for every lane (the model's),
+// so the loop runs n passes with no early exit; unrolled to kMaxStates it
+// would keep one compare mask per bound live in scalar registers.
 PDA_HD int first_above(const float* cum, int n, float u) {
     int idx = n - 1;
     bool found = false;
-#pragma unroll
-    for (int i = 0; i < kMaxStates; ++i) {
-        if (i < n && !found && cum[i] > u) {
-            idx = i;
-            found = true;
-        }
+#pragma unroll 1
+    for (int i = 0; i < n; ++i) {
+        const bool hit = !found && cum[i] > u;
+        idx = hit ? i : idx;
+        found = found || hit;
     }
     return idx;
 }
