@@ -30,6 +30,8 @@ enum Pass : int {
     kPassGeneral = 2,  // general form for every unit (models with broadening)
 };
 
+// The per-unit outputs outE / outAcc / outF are requested together (all
+// null or all set), so one test covers the three stores.
 __device__ __forceinline__ void tally(int32_t* s_hist, const RunArgs& a, const WalkOut& o,
                                       uint32_t u, float* outE, int32_t* outAcc, float* outF) {
     const float E = o.F > 0.0f ? o.acc / o.F : 0.0f;
@@ -37,9 +39,11 @@ __device__ __forceinline__ void tally(int32_t* s_hist, const RunArgs& a, const W
         const int bin = int(floor(double(E) * double(a.B)));
         if (bin < a.B) atomicAdd(&s_hist[bin], 1);
     }
-    if (outE) outE[u] = E;
-    if (outAcc) outAcc[u] = int32_t(o.acc);
-    if (outF) outF[u] = o.F;
+    if (outE) {
+        outE[u] = E;
+        outAcc[u] = int32_t(o.acc);
+        outF[u] = o.F;
+    }
 }
 
 // The walk over every unit of the run.  Each one-wave workgroup keeps an LDS
