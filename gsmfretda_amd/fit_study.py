@@ -60,6 +60,8 @@ def main(argv=None):
     ap.add_argument("--bursts", type=int, default=1_000_000)
     ap.add_argument("--seconds", type=float, default=60.0, help="wall limit per fitter")
     ap.add_argument("--pop", type=int, default=64)
+    ap.add_argument("--pop-seed", type=int, default=None,
+                    help="draw the 64 starting sets with this seed instead of the workload's")
     ap.add_argument("--lr", type=float, default=0.1)
     ap.add_argument("--r-grad", type=int, default=8,
                     help="resamples per burst of the RB gradient (the plain chi2 keeps R = 32)")
@@ -84,7 +86,11 @@ def main(argv=None):
     chi2_truth = prob.chi2(fit.args_to_theta(truth, n))
     target = 1.01 * chi2_truth
     log(f"chi2(truth) = {chi2_truth:.5f}, target {target:.5f}")
-    pop0 = [fit.args_to_theta(args, n) for _, args in cfg["population"][:a.pop]]
+    start_sets = cfg["population"]
+    if a.pop_seed is not None:
+        start_sets = synth.population(a.pop, np.random.default_rng(a.pop_seed), lo=1e3, hi=1e5,
+                                      states=(n,))
+    pop0 = [fit.args_to_theta(args, n) for _, args in start_sets[:a.pop]]
 
     results = {}
     fitters = a.fitters.split(",")
@@ -120,6 +126,7 @@ def main(argv=None):
 
     ga_final = results["ga-per-set"].chi2[-1] if "ga-per-set" in results else float("nan")
     out = {"workload": "4 states, rates 1e3-1e5, %d bursts x %d resamples, 50 bins" % (d.nbursts, R),
+           "pop_seed": a.pop_seed,
            "chi2_truth": chi2_truth, "target_truth": target, "ga_per_set_final": ga_final,
            "seconds_limit": a.seconds,
            "time_to_truth_target": {k: v.time_to(target) for k, v in results.items()},
