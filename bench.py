@@ -184,9 +184,11 @@ def fit_leg(pda, synth, seconds):
     for name, make in (("adam-rb", lambda: fit.AdamFitter(lr=0.1, R_grad=8, check_every=5)),
                        ("lm-rb", lambda: fit.LMFitter(R_grad=8))):
         t0 = time.perf_counter()
-        start = pop0[int(np.argmin(prob.chi2_population(pop0)))]
+        fit0 = prob.chi2_population(pop0)
+        ranked = [pop0[i] for i in np.argsort(fit0, kind="stable")]
         t_init = time.perf_counter() - t0
-        h = make().run(prob, start, seconds - t_init, target=target)
+        extra = {"restarts": ranked[1:]} if name == "lm-rb" else {}
+        h = make().run(prob, ranked[0], seconds - t_init, target=target, **extra)
         h.times = [t + t_init for t in h.times]
         res[name] = h
     m.close()

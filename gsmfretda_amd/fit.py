@@ -313,17 +313,32 @@ class LMFitter:
         Jr = (dr[:, None] * J) * dargs[None, :]
         return Jr.T @ Jr, Jr.T @ r
 
-    def run(self, prob, theta0, max_seconds, target=None, max_steps=100000):
+    def run(self, prob, theta0, max_seconds, target=None, max_steps=100000, restarts=()):
+        """Descends from theta0; when a descent converges (the step cap
+        falls below step_min) short of the target, starts again from the next
+        point of `restarts` (e.g. the next best sets of an initial
+        population) while time is left.  The history keeps the best plain
+        chi2 over all descents."""
         n = prob.n
         R = self.R_grad or prob.R
         hist = History("lm-rb")
         t0 = time.perf_counter()
-        theta = clip_theta(theta0, n)
+        for start in [theta0, *restarts]:
+            self._descend(prob, clip_theta(start, n), R, hist, t0, max_seconds, target, max_steps)
+            if target is not None and hist.chi2 and hist.chi2[-1] <= target:
+                break
+            if time.perf_counter() - t0 > max_seconds:
+                break
+        return hist
+
+    def _descend(self, prob, theta, R, hist, t0, max_seconds, target, max_steps):
+        n = prob.n
         hist.record(time.perf_counter() - t0, prob.chi2(theta), prob.evaluations, theta)
         M, c2 = self._forward(prob, theta, R)
         prob.evaluations += 1
         A, g = self._system(prob, theta, M, R)
         lam, cap = self.lam0, self.step_max
+        last_recorded = theta
         for step in range(max_steps):
             if target is not None and hist.chi2[-1] <= target:
                 break
@@ -349,9 +364,9 @@ class LMFitter:
                 cap = min(2.0 * cap, 2.0)
                 if (step + 1) % self.check_every == 0:
                     hist.record(time.perf_counter() - t0, prob.chi2(theta), prob.evaluations, theta)
+                    last_recorded = theta
             else:
                 lam *= 4.0
                 cap *= 0.5
-        if hist.best_theta is None or not np.array_equal(hist.best_theta, theta):
+        if last_recorded is not theta:
             hist.record(time.perf_counter() - t0, prob.chi2(theta), prob.evaluations, theta)
-        return hist

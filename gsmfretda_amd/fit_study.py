@@ -108,6 +108,8 @@ def main(argv=None):
 
     # the gradient fitters start from the best initial set (found by one
     # batched evaluation, timed)
+    # (Levenberg-Marquardt restarts from the next best sets when a descent
+    # converges short of the target)
     grad_specs = [s for s in [
         ("adam-rb", lambda: fit.AdamFitter(lr=a.lr, R_grad=a.r_grad, check_every=5,
                                            plateau=a.plateau, lr_decay=a.lr_decay)),
@@ -116,9 +118,10 @@ def main(argv=None):
         prob.evaluations = 0
         ts = time.perf_counter()
         fit0 = prob.chi2_population(pop0)
-        start = pop0[int(np.argmin(fit0))]
+        ranked = [pop0[i] for i in np.argsort(fit0, kind="stable")]
         t_init = time.perf_counter() - ts
-        h = make().run(prob, start, a.seconds - t_init, target=target)
+        extra = {"restarts": ranked[1:]} if name == "lm-rb" else {}
+        h = make().run(prob, ranked[0], a.seconds - t_init, target=target, **extra)
         h.times = [t + t_init for t in h.times]
         results[name] = h
         log(f"{name}: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s, "

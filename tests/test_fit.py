@@ -146,3 +146,17 @@ def test_lm_converges_on_a_smooth_problem():
     h = fit.LMFitter(R_grad=4).run(prob, start, max_seconds=30, max_steps=200)
     assert h.chi2[-1] < 1e-6, h.chi2[-5:]
     assert h.chi2[0] > 1.0
+
+
+def test_lm_restarts_after_a_converged_descent():
+    prob, th_true = _fake()
+    starts = [th_true + 0.3, th_true - 0.2, th_true + np.linspace(-0.2, 0.2, len(th_true))]
+    # an unreachable target: every start is descended in turn
+    h = fit.LMFitter(R_grad=4).run(prob, starts[0], max_seconds=60, target=-1.0, max_steps=3,
+                                   restarts=starts[1:])
+    assert len(h.chi2) >= 3 * 2          # a start record and at least one step per descent
+    # a reachable one: the first descent gets there, no restart is started
+    prob2, _ = _fake()
+    h2 = fit.LMFitter(R_grad=4).run(prob2, starts[0], max_seconds=60, target=1e-3,
+                                    restarts=starts[1:])
+    assert h2.chi2[-1] <= 1e-3
