@@ -104,10 +104,19 @@ def population_leg(pda, synth, steps, warmup):
     units = d.nbursts * R * len(pop)
     st = m.population_stats()
     m.close()
+    # VALU roofline of the whole generation (graph replay timed with HIP
+    # events) at the frozen lane-operations per unit of this workload
+    cm = json.load(open(COST_MODEL))
+    ops = cm["population_valu_lane_ops_per_unit"] * units
+    achieved = ops / (float(np.mean(gpu_ms)) * 1e-3) / 1e12
+    peak = cm["valu_peak_lane_ops_per_s"] / 1e12
     return {"workload": "batched population: 256 parameter sets (2-4 states, rates 1e2-1e4), "
                         "50000 bursts x 32 resamples, 50 bins, per-set chi2, one hipGraph replay "
                         "per generation",
             "value": units * steps / dt, "unit": "burst-resamples/s",
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
+                         "frac": achieved / peak, "kernel": "pop_walk_kernel<0> (whole generation)",
+                         "ops_per_unit": cm["population_valu_lane_ops_per_unit"]},
             "ms_per_generation": dt / steps * 1e3, "gpu_ms_per_generation": float(np.mean(gpu_ms)),
             "units_per_generation": units, "graph_captures": st[2], "replayed_units": st[1],
             "autotuned_shape": {"batch_tiles": tuned[0], "waves_per_cu": tuned[1],
