@@ -147,6 +147,9 @@ int load_common(pda_ctx* ctx, const int64_t* start, const int64_t* stop, const i
     for (auto& s : ctx->slots) {
         s.begin = 0;
         s.end = -1;
+        s.rb.poff_b0 = -1;  // the RB pmf offsets follow the photon counts
+        s.rb.pmf_valid = false;
+        s.rb.ran = false;    // no derivative of a run over the previous data
     }
     return PDA_OK;
 }
