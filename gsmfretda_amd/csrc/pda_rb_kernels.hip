@@ -25,8 +25,13 @@
 
 namespace pda {
 
-// counter kMaxQueues (after the walk's queues): bursts taken by rb_pmf_kernel
+// rb_pmf_kernel takes bursts in batches (1-8, about 16 grabs per
+// workgroup) from up to kAuxQueues counters after the walk's (workgroup g
+// from counter g % kAuxQueues, one per XCD, each over a contiguous eighth of
+// the bursts): one counter and one grab per burst made the pass wait on a
+// million serialised atomics at 10^6 bursts.
 constexpr uint32_t kRbPmfCounter = kMaxQueues * kCounterStride;
+constexpr uint32_t kRbMaxBurstBatch = 8;
 
 // Jacobian scratch in LDS (doubles, after the float buffers): per bin the
 // efficiency and rate derivatives, the current unit's histogram per lane
@@ -413,12 +418,26 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
     }
     __syncthreads();
     const uint32_t nb = a.nunits / a.R;
+    // every counter must have a workgroup: fewer queues on a small grid
+    const uint32_t nq = gridDim.x < kAuxQueues ? gridDim.x : kAuxQueues;
+    const uint32_t queue = blockIdx.x % nq;
+    const uint32_t qb = uint32_t(uint64_t(nb) * queue / nq);
+    const uint32_t qn = uint32_t(uint64_t(nb) * (queue + 1u) / nq) - qb;
+    uint32_t* counter = &work[kRbPmfCounter + queue * kCounterStride];
+    uint32_t batch = nb / (gridDim.x * 16u);
+    batch = batch < 1u ? 1u : (batch > kRbMaxBurstBatch ? kRbMaxBurstBatch : batch);
     double gp_acc = 0.0, gk_acc = 0.0;  // lane s: d/dp_s; lane i: d/dk_i
+    uint32_t next = 0, end = 0;  // the wave's current batch of bursts
     while (true) {
-        uint32_t bl = 0;
-        if (lane == 0) bl = atomicAdd(&work[kRbPmfCounter], 1u);
-        bl = __shfl(bl, 0);
-        if (bl >= nb) break;
+        if (next >= end) {
+            uint32_t off = 0;
+            if (lane == 0) off = atomicAdd(counter, batch);
+            off = __shfl(off, 0);
+            if (off >= qn) break;
+            next = qb + off;
+            end = off + batch < qn ? next + batch : qb + qn;
+        }
+        const uint32_t bl = next++;
         const int bound = rb_total_bound(bursts[a.begin + bl].P, sR.coef);
         if (bound < kRbSmallCap)
             rb_burst<kRbSmallLanes, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h,
@@ -511,7 +530,8 @@ __global__ __launch_bounds__(kWave) void rb_finish_kernel(
         res->effN = effN;
         res->B = a.B;
     }
-    for (uint32_t q = lane; q <= kMaxQueues; q += kWave) atomicExch(&work[q * kCounterStride], 0u);
+    for (uint32_t q = lane; q < kMaxQueues + kAuxQueues; q += kWave)
+        atomicExch(&work[q * kCounterStride], 0u);
 }
 
 hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu) {

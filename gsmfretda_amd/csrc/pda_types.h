@@ -15,9 +15,10 @@ constexpr uint32_t kDefaultQueues = 32;  // four per XCD unless tuned
 // Each work counter sits on its own 256-byte line: device-scope atomics on
 // one line serialise, and counters sharing a line would serialise together.
 constexpr uint32_t kCounterStride = 64;  // uint32 words between counters
-// Counter words of a run: kMaxQueues queue counters and one spare (the RB
-// pmf pass's burst counter), each on its own line.
-constexpr size_t kWorkWords = size_t(kMaxQueues + 1) * kCounterStride;
+// Counter words of a run: kMaxQueues queue counters and kAuxQueues more
+// (the RB pmf pass's burst counters), each on its own line.
+constexpr uint32_t kAuxQueues = 8;
+constexpr size_t kWorkWords = size_t(kMaxQueues + kAuxQueues) * kCounterStride;
 
 // One burst, original order (the range [begin, end) of mc::setBurstBd
 // addresses these directly).  16 bytes, one dwordx4 load.

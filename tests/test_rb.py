@@ -56,7 +56,7 @@ def _rb(m, n, args, R, seed, set_id=0, scores=True):
 
 
 @pytest.mark.parametrize("n,bg,rng_", [(2, True, (0, -1)), (3, True, (5, 170)), (3, False, (0, -1)),
-                                       (4, True, (0, -1))])
+                                       (4, True, (0, -1)), (3, True, (10, 13))])
 def test_rb_forward_bit_exact(gpu_mc, oracle, n, bg, rng_):
     d = _data(200, 31 + n, bg)
     _load(gpu_mc, d)
