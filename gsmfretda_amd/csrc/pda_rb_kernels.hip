@@ -25,11 +25,14 @@
 
 namespace pda {
 
-// rb_pmf_kernel takes bursts in batches (1-8, about 16 grabs per
-// workgroup) from up to kAuxQueues counters after the walk's (workgroup g
-// from counter g % kAuxQueues, one per XCD, each over a contiguous eighth of
-// the bursts): one counter and one grab per burst made the pass wait on a
-// million serialised atomics at 10^6 bursts.
+// rb_pmf_kernel takes bursts in batches from up to kAuxQueues counters
+// after the walk's (workgroup g from counter g % kAuxQueues, one per XCD,
+// each over a contiguous eighth of the bursts): one counter and one grab per
+// burst made the pass wait on a million serialised atomics at 10^6 bursts.
+// In the derivative passes batches shrink to 1 as a queue empties and a
+// workgroup whose queue is empty takes single bursts from the others, so the
+// pass ends with every wave within about one burst of the last (the forward,
+// whose bursts are cheap, keeps fixed batches: fewer grabs measured faster).
 constexpr uint32_t kRbPmfCounter = kMaxQueues * kCounterStride;
 constexpr uint32_t kRbMaxBurstBatch = 8;
 
@@ -420,11 +423,17 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
     const uint32_t nb = a.nunits / a.R;
     // every counter must have a workgroup: fewer queues on a small grid
     const uint32_t nq = gridDim.x < kAuxQueues ? gridDim.x : kAuxQueues;
-    const uint32_t queue = blockIdx.x % nq;
-    const uint32_t qb = uint32_t(uint64_t(nb) * queue / nq);
-    const uint32_t qn = uint32_t(uint64_t(nb) * (queue + 1u) / nq) - qb;
+    const uint32_t waves_q = gridDim.x / nq;  // >= 1 workgroups per queue
+    uint32_t queue = blockIdx.x % nq, visited = 1;
+    uint32_t qb = uint32_t(uint64_t(nb) * queue / nq);
+    uint32_t qn = uint32_t(uint64_t(nb) * (queue + 1u) / nq) - qb;
     uint32_t* counter = &work[kRbPmfCounter + queue * kCounterStride];
-    uint32_t batch = nb / (gridDim.x * 16u);
+    // The forward takes fixed batches (about 16 grabs per workgroup).  The
+    // derivative passes, several times dearer per burst, take guided batches
+    // (a quarter of the wave's share of what its queue had left at its last
+    // grab) and then single bursts from the other queues.
+    constexpr bool kGuided = kMode != kRbFwd;
+    uint32_t batch = kGuided ? qn / (waves_q * 4u) : nb / (gridDim.x * 16u);
     batch = batch < 1u ? 1u : (batch > kRbMaxBurstBatch ? kRbMaxBurstBatch : batch);
     double gp_acc = 0.0, gk_acc = 0.0;  // lane s: d/dp_s; lane i: d/dk_i
     uint32_t next = 0, end = 0;  // the wave's current batch of bursts
@@ -433,9 +442,25 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
             uint32_t off = 0;
             if (lane == 0) off = atomicAdd(counter, batch);
             off = __shfl(off, 0);
-            if (off >= qn) break;
+            if (off >= qn) {
+                // queue drained: help the next one until every queue has
+                // been seen drained
+                if (!kGuided || visited == nq) break;
+                ++visited;
+                queue = queue + 1u == nq ? 0u : queue + 1u;
+                qb = uint32_t(uint64_t(nb) * queue / nq);
+                qn = uint32_t(uint64_t(nb) * (queue + 1u) / nq) - qb;
+                counter = &work[kRbPmfCounter + queue * kCounterStride];
+                batch = 1u;
+                continue;
+            }
             next = qb + off;
             end = off + batch < qn ? next + batch : qb + qn;
+            if (kGuided) {
+                const uint32_t left = off + batch < qn ? qn - off - batch : 0u;
+                batch = left / (waves_q * 4u);
+                batch = batch < 1u ? 1u : (batch > kRbMaxBurstBatch ? kRbMaxBurstBatch : batch);
+            }
         }
         const uint32_t bl = next++;
         const int bound = rb_total_bound(bursts[a.begin + bl].P, sR.coef);
