@@ -177,7 +177,9 @@ def fit_leg(pda, synth, seconds):
     100,000 bursts (4 states, rates 1e3-1e5 s^-1, 32 resamples): the
     per-set genetic algorithm (one pda_run per individual, the reference's
     pattern) against the gradient fitters on the RB form, same 64 starting
-    sets.  Each fitter stops at the target or after `seconds`."""
+    sets (the gradient fitters start from the best of them, ranked by one
+    timed batched evaluation with 4 resamples per burst).  Each fitter stops
+    at the target or after `seconds`."""
     from gsmfretda_amd import fit, fit_study
     cfg = synth.config(4, n_bursts=100_000)
     n = cfg["n"]
@@ -192,13 +194,8 @@ def fit_leg(pda, synth, seconds):
     res["ga-per-set"] = h
     for name, make in (("adam-rb", lambda: fit.AdamFitter(lr=0.1, R_grad=8, check_every=5)),
                        ("lm-rb", lambda: fit.LMFitter(R_grad=8))):
-        t0 = time.perf_counter()
-        fit0 = prob.chi2_population(pop0)
-        ranked = [pop0[i] for i in np.argsort(fit0, kind="stable")]
-        t_init = time.perf_counter() - t0
-        extra = {"restarts": ranked[1:]} if name == "lm-rb" else {}
-        h = make().run(prob, ranked[0], seconds - t_init, target=target, **extra)
-        h.times = [t + t_init for t in h.times]
+        h, _ = fit.fit_from_population(make(), prob, pop0, seconds, target=target,
+                                       rank_resamples=4, restarts=name == "lm-rb")
         res[name] = h
     m.close()
     t = {k: v.time_to(target) for k, v in res.items()}

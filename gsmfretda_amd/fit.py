@@ -94,15 +94,38 @@ class Problem:
                                     self.seed, set_id=0)
         return c64
 
-    def chi2_population(self, thetas):
+    def chi2_population(self, thetas, R=None):
         """Plain-simulator chi2 of a generation in one population run; every
-        set uses set id 0, the streams of `chi2`."""
+        set uses set id 0, the streams of `chi2`.  R: resamples per burst
+        (default the problem's)."""
         self.evaluations += len(thetas)
         self.mc.set_population([self.n] * len(thetas),
                                [theta_to_args(t, self.n) for t in thetas])
-        self.mc.run_population(R=self.R, seed=self.seed, set_base=0, set_stride=0)
+        self.mc.run_population(R=self.R if R is None else R, seed=self.seed, set_base=0,
+                               set_stride=0)
         _, c64, _, _ = self.mc.get_population_result()
         return c64
+
+
+def fit_from_population(fitter, prob, pop0, seconds, target=None, rank_resamples=4,
+                        restarts=False):
+    """Runs `fitter` from the best of the starting sets `pop0`.
+
+    The sets are ranked by one batched plain evaluation with `rank_resamples`
+    resamples per burst: random starting sets differ in chi2 by orders of
+    magnitude, which a few resamples of every burst resolve, so the ranking
+    costs a fraction of an evaluation at the problem's R.  Its wall time is
+    part of the returned history (every chi2 in which is the plain chi2 at
+    the problem's R).  restarts: hand the next best sets to the fitter
+    (LMFitter).  Returns (History, seconds spent ranking)."""
+    t0 = time.perf_counter()
+    fit0 = prob.chi2_population(pop0, R=rank_resamples)
+    ranked = [pop0[i] for i in np.argsort(fit0, kind="stable")]
+    t_init = time.perf_counter() - t0
+    extra = {"restarts": ranked[1:]} if restarts else {}
+    h = fitter.run(prob, ranked[0], seconds - t_init, target=target, **extra)
+    h.times = [t + t_init for t in h.times]
+    return h, t_init
 
 
 @dataclass

@@ -14,7 +14,8 @@ fitters run under the same wall-clock limit:
   ga-batched   the same algorithm, one pda_run_population per generation
   adam-rb      Adam on (logit E, log K) with the adjoint gradient of the
                Rao-Blackwellised chi2 (8 resamples per burst), started from
-               the best initial set found by one batched evaluation
+               the best initial set, found by one batched evaluation with 4
+               resamples per burst (timed)
   lm-rb        Levenberg-Marquardt on the same chi2 with the Jacobian of the
                expected histogram (pda_rb_jacobian), same start
 
@@ -68,6 +69,9 @@ def main(argv=None):
     ap.add_argument("--plateau", type=int, default=0,
                     help="Adam: halve the learning rate after this many steps without progress")
     ap.add_argument("--lr-decay", type=float, default=1.0, help="Adam: per-step decay")
+    ap.add_argument("--rank-resamples", type=int, default=4,
+                    help="resamples per burst of the batched evaluation that ranks the starting "
+                         "sets for the gradient fitters")
     ap.add_argument("--fitters", default="ga-per-set,ga-batched,adam-rb,lm-rb",
                     help="comma-separated subset of ga-per-set, ga-batched, adam-rb, lm-rb")
     ap.add_argument("--out", default=None)
@@ -106,36 +110,35 @@ def main(argv=None):
         log(f"{name}: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s, "
             f"{prob.evaluations} evaluations, {len(h.times)} generations")
 
-    # the gradient fitters start from the best initial set (found by one
-    # batched evaluation, timed)
+    # the gradient fitters start from the best initial set, found by one
+    # timed batched evaluation with rank_resamples resamples per burst
     # (Levenberg-Marquardt restarts from the next best sets when a descent
     # converges short of the target)
     grad_specs = [s for s in [
         ("adam-rb", lambda: fit.AdamFitter(lr=a.lr, R_grad=a.r_grad, check_every=5,
                                            plateau=a.plateau, lr_decay=a.lr_decay)),
         ("lm-rb", lambda: fit.LMFitter(R_grad=a.r_grad))] if s[0] in fitters]
+    ranking_seconds = {}
     for name, make in grad_specs:
         prob.evaluations = 0
-        ts = time.perf_counter()
-        fit0 = prob.chi2_population(pop0)
-        ranked = [pop0[i] for i in np.argsort(fit0, kind="stable")]
-        t_init = time.perf_counter() - ts
-        extra = {"restarts": ranked[1:]} if name == "lm-rb" else {}
-        h = make().run(prob, ranked[0], a.seconds - t_init, target=target, **extra)
-        h.times = [t + t_init for t in h.times]
+        h, t_init = fit.fit_from_population(make(), prob, pop0, a.seconds, target=target,
+                                            rank_resamples=a.rank_resamples,
+                                            restarts=name == "lm-rb")
         results[name] = h
-        log(f"{name}: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s, "
-            f"{prob.evaluations} evaluations")
+        ranking_seconds[name] = t_init
+        log(f"{name}: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s "
+            f"(ranking {t_init:.3f} s), {prob.evaluations} evaluations")
 
     ga_final = results["ga-per-set"].chi2[-1] if "ga-per-set" in results else float("nan")
     out = {"workload": "4 states, rates 1e3-1e5, %d bursts x %d resamples, 50 bins" % (d.nbursts, R),
-           "pop_seed": a.pop_seed,
+           "pop_seed": a.pop_seed, "rank_resamples": a.rank_resamples,
+           "ranking_seconds": ranking_seconds,
            "chi2_truth": chi2_truth, "target_truth": target, "ga_per_set_final": ga_final,
            "seconds_limit": a.seconds,
            "time_to_truth_target": {k: v.time_to(target) for k, v in results.items()},
            "time_to_ga_final": {k: v.time_to(1.01 * ga_final) for k, v in results.items()},
            "final_chi2": {k: v.chi2[-1] for k, v in results.items()},
-           "rb_ms": dict(zip(("walk", "forward", "adjoint"), m.rb_stats())),
+           "rb_ms": dict(zip(("walk", "forward", "last_derivative_pass"), m.rb_stats())),
            "truth": truth.tolist(),
            "histories": {k: v.as_dict() for k, v in results.items()}}
     if a.out:

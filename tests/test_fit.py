@@ -160,3 +160,34 @@ def test_lm_restarts_after_a_converged_descent():
     h2 = fit.LMFitter(R_grad=4).run(prob2, starts[0], max_seconds=60, target=1e-3,
                                     restarts=starts[1:])
     assert h2.chi2[-1] <= 1e-3
+
+
+def test_fit_from_population_ranks_then_runs_from_the_best():
+    class Prob:
+        evaluations = 0
+
+        def __init__(self):
+            self.calls = []
+
+        def chi2_population(self, thetas, R=None):
+            self.calls.append(R)
+            return [float(np.sum(t ** 2)) for t in thetas]
+
+    class Fitter:
+        def run(self, prob, theta0, max_seconds, target=None, restarts=()):
+            self.got = (theta0, list(restarts), max_seconds, target)
+            h = fit.History("stub")
+            h.record(0.5, 1.0, 1, theta0)
+            return h
+
+    pop = [np.full(3, v) for v in (3.0, -1.0, 2.0, 0.5)]
+    prob, f = Prob(), Fitter()
+    h, t = fit.fit_from_population(f, prob, pop, 10.0, target=2.0, rank_resamples=4,
+                                   restarts=True)
+    assert prob.calls == [4]
+    assert np.array_equal(f.got[0], pop[3])
+    assert [r[0] for r in f.got[1]] == [-1.0, 2.0, 3.0]
+    assert f.got[2] == 10.0 - t and f.got[3] == 2.0 and t >= 0.0
+    assert h.times == [0.5 + t]
+    fit.fit_from_population(f, prob, pop, 10.0)
+    assert f.got[1] == [] and prob.calls == [4, 4]
