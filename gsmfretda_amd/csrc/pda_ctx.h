@@ -221,8 +221,9 @@ hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, c
                           const uint2* links, const ModelBlob* blob, const RbModel* rbm,
                           const RunArgs& a, uint2* cnt, float* F, float* S, uint32_t* work,
                           hipStream_t stream);
-// mode: 0 forward, 1 adjoint, 2 Jacobian
-size_t rb_pmf_lds(int B, int cap_big, int mode);
+// mode: 0 forward, 1 adjoint, 2 Jacobian; stored: the pass reads the
+// forward's stored pmfs (fewer LDS buffers)
+size_t rb_pmf_lds(int B, int cap_big, int mode, bool stored);
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
                            hipStream_t stream);
 hipError_t launch_rb_pmf(int mode, uint32_t grid, int cap_big, const RbModel* rbm,

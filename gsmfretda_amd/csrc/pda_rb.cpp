@@ -209,7 +209,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
                     "%u photons need %d)", kRbMaxCount, maxP, bound);
     const int cap = bound + 1;
     const bool scores = (flags & PDA_RB_SCORES) != 0;
-    if (rb_pmf_lds(ctx->B, cap, 0) + sizeof(RbModel) > ctx->lds_per_block)
+    if (rb_pmf_lds(ctx->B, cap, 0, false) + sizeof(RbModel) > ctx->lds_per_block)
         return fail(PDA_E_DATA, "bursts of %u photons exceed the LDS of the RB kernels", maxP);
 
     // buffers of the per-unit summaries
@@ -267,7 +267,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, (uint64_t(units) + batch - 1) / batch)));
     a.batch = uint32_t(batch);
     a.queues = std::min<uint32_t>(ctx->shape.queues ? ctx->shape.queues : kDefaultQueues, grid);
-    const int dyn = int(rb_pmf_lds(ctx->B, cap, 0));
+    const int dyn = int(rb_pmf_lds(ctx->B, cap, 0, false));
     HIP_TRY(rb_occupancy(2, dyn, &per_cu));
     per_cu = std::max(per_cu, 1);
     const uint32_t pgrid = uint32_t(std::max<int64_t>(
@@ -362,7 +362,8 @@ int derivative_pass(pda_ctx* ctx, const RbState& r, int mode, RunArgs& a, uint32
     a.B = r.B;
     a.left_pad = r.left_pad;
     a.nstates = r.n;
-    const size_t lds = rb_pmf_lds(r.B, r.cap, mode);
+    // the launch gets the stored pmfs when the forward kept them
+    const size_t lds = rb_pmf_lds(r.B, r.cap, mode, r.pmf_valid && r.d_pmf);
     if (lds + sizeof(RbModel) > ctx->lds_per_block)
         return fail(PDA_E_DATA, "%d bins and bursts needing %d acceptor outcomes exceed the LDS of "
                     "the %s kernel", r.B, r.cap, mode == 2 ? "Jacobian" : "adjoint");

@@ -195,7 +195,7 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
                                          const uint2* __restrict__ cnt_in,
                                          const float* __restrict__ F_in,
                                          const float* __restrict__ S_in, const double* s_g,
-                                         unsigned long long* s_h, float* s_buf, int cap,
+                                         unsigned long long* s_h, float* s_buf, int nbuf, int cap,
                                          uint32_t lane, double& gp_acc, double& gk_acc,
                                          const JacLds& jl, float* __restrict__ pmf_io,
                                          const uint64_t* __restrict__ poff) {
@@ -203,7 +203,7 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
     constexpr int G = kWave / L;
     const int g = int(lane) / L, lg = int(lane) % L;
     const int n = sR.n, K = sR.K, B = a.B;
-    float* base = s_buf + g * 3 * cap;
+    float* base = s_buf + g * nbuf * cap;
     // pmf_io: the forward stores each unit's pmf at poff[bl] + r * stride,
     // the derivative passes read it back instead of rebuilding it
     const uint64_t slot0 = pmf_io ? poff[bl] : 0u;
@@ -375,10 +375,18 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
     }
 }
 
-// Float entries of the pmf buffers: 4 groups x 3 x 256 or 3 x cap_big.
-__host__ __device__ inline size_t rb_pmf_floats(int cap_big) {
-    const size_t small = size_t(kWave / kRbSmallLanes) * 3u * kRbSmallCap;
-    const size_t big = 3u * size_t(cap_big);
+// pmf buffers per lane group: the forward (and a derivative pass without
+// stored pmfs) convolves in X, Y and Z; the adjoint reading stored pmfs needs
+// X and Z (the cotangent differences), the Jacobian X alone.  Fewer buffers
+// leave LDS for more resident waves.
+__host__ __device__ inline int rb_pmf_buffers(int mode, bool stored) {
+    return !stored || mode == kRbFwd ? 3 : (mode == kRbGrad ? 2 : 1);
+}
+
+// Float entries of the pmf buffers: 4 groups x nbuf x 256 or nbuf x cap_big.
+__host__ __device__ inline size_t rb_pmf_floats(int cap_big, int nbuf) {
+    const size_t small = size_t(kWave / kRbSmallLanes) * size_t(nbuf) * kRbSmallCap;
+    const size_t big = size_t(nbuf) * size_t(cap_big);
     return ((small > big ? small : big) + 1u) & ~size_t(1);  // even: doubles follow
 }
 
@@ -402,7 +410,8 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
     float* s_buf = reinterpret_cast<float*>(s_dyn + a.B);
     const int B = a.B;
     JacLds jl;
-    jl.JE = s_dyn + B + rb_pmf_floats(cap_big) / 2;
+    const int nbuf = rb_pmf_buffers(kMode, pmf_io != nullptr);
+    jl.JE = s_dyn + B + rb_pmf_floats(cap_big, nbuf) / 2;
     jl.JK = jl.JE + B * kRbMaxStates;
     jl.hu = jl.JK + B * kRbMaxPairs;
     jl.Hb = jl.hu + B * (kWave / kRbSmallLanes);
@@ -466,11 +475,11 @@ __global__ __launch_bounds__(kWave) void rb_pmf_kernel(
         const int bound = rb_total_bound(bursts[a.begin + bl].P, sR.coef);
         if (bound < kRbSmallCap)
             rb_burst<kRbSmallLanes, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h,
-                                           s_buf, kRbSmallCap, lane, gp_acc, gk_acc, jl, pmf_io,
-                                           poff);
+                                           s_buf, nbuf, kRbSmallCap, lane, gp_acc, gk_acc, jl,
+                                           pmf_io, poff);
         else
             rb_burst<kWave, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf,
-                                   cap_big, lane, gp_acc, gk_acc, jl, pmf_io, poff);
+                                   nbuf, cap_big, lane, gp_acc, gk_acc, jl, pmf_io, poff);
     }
     if (kMode == kRbJac) {
         // grad holds J[b][i], i < n + K (efficiencies, then rates in args
@@ -582,8 +591,9 @@ hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, c
     return hipGetLastError();
 }
 
-size_t rb_pmf_lds(int B, int cap_big, int mode) {
-    return sizeof(double) * size_t(B) + sizeof(float) * rb_pmf_floats(cap_big) +
+size_t rb_pmf_lds(int B, int cap_big, int mode, bool stored) {
+    return sizeof(double) * size_t(B) +
+           sizeof(float) * rb_pmf_floats(cap_big, rb_pmf_buffers(mode, stored)) +
            (mode == 2 ? sizeof(double) * rb_jac_doubles(B) : 0u);
 }
 
@@ -601,7 +611,7 @@ hipError_t launch_rb_pmf(int mode, uint32_t grid, int cap_big, const RbModel* rb
                          const float* S, const double* g, unsigned long long* hist_acc,
                          double* grad, uint32_t* work, float* pmf_io, const uint64_t* poff,
                          hipStream_t stream) {
-    const size_t lds = rb_pmf_lds(a.B, cap_big, mode);
+    const size_t lds = rb_pmf_lds(a.B, cap_big, mode, pmf_io != nullptr);
     if (mode == kRbGrad)
         hipLaunchKernelGGL(rb_pmf_kernel<kRbGrad>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
                            pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big,
