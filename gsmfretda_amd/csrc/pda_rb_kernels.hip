@@ -390,11 +390,16 @@ __host__ __device__ inline size_t rb_pmf_floats(int cap_big, int nbuf) {
     return ((small > big ? small : big) + 1u) & ~size_t(1);  // even: doubles follow
 }
 
-// One wave per burst (taken from work[8]).  Dynamic LDS: B 8-byte entries
-// (the fixed-point histogram, or the cotangent) followed by the float
-// buffers, then (Jacobian) the JacLds arrays.
+// One wave per burst at a time (bursts from the counters after the walk's).
+// Dynamic LDS: B 8-byte entries (the fixed-point histogram, or the cotangent)
+// followed by the float buffers, then (Jacobian) the JacLds arrays.  The
+// adjoint is held to 128 VGPRs (4 waves per SIMD instead of 3, a few values
+// spilled to scratch): its deconvolution scans wait on shuffles and LDS, and
+// the extra wave measured 8-14 % faster.  The Jacobian's LDS allows 3 waves
+// per SIMD whatever its registers.
 template <int kMode>
-__global__ __launch_bounds__(kWave) void rb_pmf_kernel(
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
+    kMode == kRbGrad ? 4 : 1))) void rb_pmf_kernel(
     const RbModel* __restrict__ rbm, const double* __restrict__ lf,
     const float* __restrict__ pmf_tab, const RunArgs a, const BurstDesc* __restrict__ bursts,
     const uint2* __restrict__ cnt_in, const float* __restrict__ F_in,
