@@ -179,15 +179,15 @@ def rb_leg(pda, synth, steps):
 def fit_leg(pda, synth, seconds):
     """Fit wall time to 1.01 x chi2(truth) on the fast-dynamics workload at
     100,000 bursts (4 states, rates 1e3-1e5 s^-1, 32 resamples): the
-    per-set genetic algorithm (one pda_run per individual, the reference's
-    pattern) against the gradient fitters on the RB form, same 64 starting
+    per-set genetic algorithm (one pda_run per individual on three stream
+    slots in flight, the reference's pattern) against the gradient fitters on the RB form, same 64 starting
     sets (the gradient fitters start from the best of them, ranked by one
     timed batched evaluation with 4 resamples per burst).  Each fitter stops
     at the target or after `seconds`."""
     from gsmfretda_amd import fit, fit_study
     cfg = synth.config(4, n_bursts=100_000)
     n = cfg["n"]
-    m = pda.MC(0, 1)
+    m = pda.MC(0, 3)
     prob = fit_study.make_problem(m, cfg)
     c_truth = prob.chi2(fit.args_to_theta(np.asarray(cfg["args"], np.float32), n))
     target = 1.01 * c_truth

@@ -94,6 +94,29 @@ class Problem:
                                     self.seed, set_id=0)
         return c64
 
+    def chi2_sets(self, thetas):
+        """Plain-simulator chi2 of each set, one pda_run per set (the
+        reference's per-set evaluation) with the context's stream slots in
+        flight in turn, as the reference's worker keeps them
+        (src/gpuWorker.cpp:38-88); set id 0 everywhere, the streams of
+        `chi2`.  Every slot needs the photophysics and burst range set."""
+        m, k = self.mc, max(1, self.mc.streamNum)
+        self.evaluations += len(thetas)
+        out = [0.0] * len(thetas)
+        m.reSampleTimes = self.R
+        for i, t in enumerate(thetas):
+            sid = i % k
+            if i >= k:
+                m.synchronize(sid)
+                out[i - k] = m.get_hist_chi2(sid)[1]
+            if not m.set_params(self.n, sid, theta_to_args(t, self.n)):
+                raise pda.PdaError("bad parameter set")
+            m.run_kernel(sid=sid, seed=self.seed, set_id=0)
+        for i in range(max(0, len(thetas) - k), len(thetas)):
+            m.synchronize(i % k)
+            out[i] = m.get_hist_chi2(i % k)[1]
+        return out
+
     def chi2_population(self, thetas, R=None):
         """Plain-simulator chi2 of a generation in one population run; every
         set uses set id 0, the streams of `chi2`.  R: resamples per burst
@@ -175,7 +198,7 @@ class GeneticFitter:
     def _evaluate(self, prob, pop):
         if self.batched:
             return np.asarray(prob.chi2_population(pop))
-        return np.array([prob.chi2(t) for t in pop])
+        return np.asarray(prob.chi2_sets(pop))
 
     def _child(self, pop, fit, n):
         rng = self.rng

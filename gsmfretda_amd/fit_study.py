@@ -10,7 +10,8 @@ parameters with the evaluation streams.  From the same initial population of
 fitters run under the same wall-clock limit:
 
   ga-per-set   genetic algorithm, one pda_run per individual (the reference's
-               one-set-per-call evaluation)
+               one-set-per-call evaluation), three stream slots in flight as
+               the reference's worker keeps several
   ga-batched   the same algorithm, one pda_run_population per generation
   adam-rb      Adam on (logit E, log K) with the adjoint gradient of the
                Rao-Blackwellised chi2 (8 resamples per burst), started from
@@ -42,9 +43,10 @@ def make_problem(m, cfg, nbins=50):
     truth = np.asarray(cfg["args"], np.float32)
     m.load(d, SgDivSr=None)
     m.set_observed_hist(np.zeros(nbins, np.float32))
-    m.set_photophysics(0, *cfg["photophysics"])
+    for sid in range(m.streamNum):
+        m.set_photophysics(sid, *cfg["photophysics"])
+        m.set_burst_range(0, -1, sid)
     assert m.set_params(n, 0, truth)
-    m.set_burst_range(0, -1, 0)
     m.reSampleTimes = 1
     m.run_kernel(sid=0, seed=cfg["seeds"]["observed"], samples=True)
     m.synchronize(0)
@@ -69,6 +71,8 @@ def main(argv=None):
     ap.add_argument("--plateau", type=int, default=0,
                     help="Adam: halve the learning rate after this many steps without progress")
     ap.add_argument("--lr-decay", type=float, default=1.0, help="Adam: per-step decay")
+    ap.add_argument("--streams", type=int, default=3,
+                    help="stream slots the per-set genetic algorithm keeps in flight")
     ap.add_argument("--rank-resamples", type=int, default=4,
                     help="resamples per burst of the batched evaluation that ranks the starting "
                          "sets for the gradient fitters")
@@ -85,7 +89,7 @@ def main(argv=None):
     d, n, R = cfg["data"], cfg["n"], cfg["R"]
     truth = np.asarray(cfg["args"], np.float32)
     log(f"data: {d.nbursts} bursts, {d.nphotons} photons, {time.perf_counter() - t0:.1f} s")
-    m = pda.MC(0, 1)
+    m = pda.MC(0, a.streams)
     prob = make_problem(m, cfg)
     chi2_truth = prob.chi2(fit.args_to_theta(truth, n))
     target = 1.01 * chi2_truth
@@ -131,7 +135,7 @@ def main(argv=None):
 
     ga_final = results["ga-per-set"].chi2[-1] if "ga-per-set" in results else float("nan")
     out = {"workload": "4 states, rates 1e3-1e5, %d bursts x %d resamples, 50 bins" % (d.nbursts, R),
-           "pop_seed": a.pop_seed, "rank_resamples": a.rank_resamples,
+           "pop_seed": a.pop_seed, "rank_resamples": a.rank_resamples, "streams": a.streams,
            "ranking_seconds": ranking_seconds,
            "chi2_truth": chi2_truth, "target_truth": target, "ga_per_set_final": ga_final,
            "seconds_limit": a.seconds,

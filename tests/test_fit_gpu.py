@@ -43,3 +43,11 @@ def test_lm_normal_equations_match_the_rb_chi2_in_e(problem):
         e[i] = 1e-3
         fd = (lm._forward(prob, th + e, 8)[1] - lm._forward(prob, th - e, 8)[1]) / 2e-3
         assert abs(2 * g[i] / dof - fd) <= 1e-2 * abs(fd), (i, 2 * g[i] / dof, fd)
+
+
+def test_per_set_evaluations_in_flight_equal_one_at_a_time(problem):
+    prob, truth, pop0 = problem
+    assert prob.mc.streamNum >= 2
+    sets = pop0[:5] + [truth]
+    want = [prob.chi2(t) for t in sets]
+    assert prob.chi2_sets(sets) == want
