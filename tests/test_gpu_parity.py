@@ -259,6 +259,33 @@ def test_work_counters_do_not_change_results(gpu_mc, oracle):
         gpu_mc.set_launch_shape(*saved)
 
 
+def test_stream_slots_in_flight(gpu_mc, oracle):
+    """Both stream slots running at once, each with its own parameter set,
+    burst range, resample count and seed: every unit equals the oracle's,
+    on two rounds in a row."""
+    cfg = synth.config(0, n_bursts=1200)
+    d = cfg["data"]
+    _load(gpu_mc, d, np.zeros(1200, np.float32))
+    ph = cfg["photophysics"]
+    args3 = synth.make_args([0.15, 0.5, 0.85], [300.0, 2000.0, 800.0, 5000.0, 150.0, 1200.0])
+    runs = [(cfg["n"], cfg["args"], 0, 1200, 9, 31, 4), (3, args3, 200, 1100, 14, 32, 5)]
+    for sid, (n, args, b0, b1, R, seed, set_id) in enumerate(runs):
+        gpu_mc.set_photophysics(sid, *ph)
+        assert gpu_mc.set_params(n, sid, args)
+        assert gpu_mc.set_burst_range(b0, b1, sid) == b1 - b0
+    want = [_oracle_run(oracle, d, n, args, ph, R, seed, set_id=set_id, b0=b0, b1=b1)
+            for n, args, b0, b1, R, seed, set_id in runs]
+    for _ in range(2):
+        for sid, (n, args, b0, b1, R, seed, set_id) in enumerate(runs):
+            gpu_mc.reSampleTimes = R
+            gpu_mc.run_kernel(sid=sid, seed=seed, set_id=set_id, samples=True)
+        for sid, (n, args, b0, b1, R, seed, set_id) in enumerate(runs):
+            gpu_mc.synchronize(sid)
+            hist = gpu_mc.get_hist_chi2(sid)[0]
+            E, acc, F = gpu_mc.get_samples(sid, (b1 - b0) * R)
+            _compare_units(dict(hist=hist, E=E, acc=acc, F=F), want[sid])
+
+
 def test_mc_shim_dropin(oracle, tmp_path):
     """The reference's call sequence through the C++ `class mc` shim
     (tests/host/mc_dropin.cpp): hmcE bit-exact with the oracle, and the
