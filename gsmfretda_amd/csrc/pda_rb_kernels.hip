@@ -38,16 +38,19 @@ constexpr uint32_t kRbMaxBurstBatch = 8;
 
 // Jacobian scratch in LDS (doubles, after the float buffers): per bin the
 // efficiency and rate derivatives, the current unit's histogram per lane
-// group, the burst's histogram, the burst's score sums.
+// group, the burst's histogram, the burst's score sums, the current units'
+// scores per lane group.
 struct JacLds {
     double* JE;  // [B][kRbMaxStates]
     double* JK;  // [B][kRbMaxPairs]
     double* hu;  // [kWave / kRbSmallLanes][B]
     double* Hb;  // [B]
     double* Sk;  // [kRbMaxPairs]
+    double* Su;  // [kWave / kRbSmallLanes][kRbMaxPairs]
 };
 __host__ __device__ inline size_t rb_jac_doubles(int B) {
-    return size_t(B) * (kRbMaxStates + kRbMaxPairs + kWave / 16 + 1) + kRbMaxPairs;
+    return size_t(B) * (kRbMaxStates + kRbMaxPairs + kWave / 16 + 1) +
+           size_t(kRbMaxPairs) * (1 + kWave / 16);
 }
 
 template <bool kScores>
@@ -324,17 +327,30 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
                 if (kMode == kRbGrad && lg == s) gp_acc += part;
             }
             if (kMode == kRbJac) {
-                // rates: sum_r h_rb S_r per (bin, rate); the burst's histogram
-                if (act && lg < K) {
-                    const double si = double(S_in[size_t(u) * size_t(K) + size_t(lg)]);
+                // rates: JK[b][k] += sum_g hu_g[b] S_g[k] over the wave's G
+                // units, and the burst's histogram Hb[b] += sum_g hu_g[b];
+                // every entry owned by one lane (lanes over k = 0..K, the
+                // column K being Hb), in a fixed order: no atomics
+                if (lg < K) {
+                    const double si = act ? double(S_in[size_t(u) * size_t(K) + size_t(lg)]) : 0.0;
                     Sg += si;
-                    for (int b = 0; b < B; ++b) {
-                        const double hb = hu[b];
-                        if (hb != 0.0) atomicAdd(&jl.JK[b * kRbMaxPairs + lg], hb * si);
+                    jl.Su[g * kRbMaxPairs + lg] = si;
+                }
+                __syncthreads();
+                const int w = K + 1, per = kWave / w;
+                const int k = int(lane) % w, bb = int(lane) / w;
+                if (bb < per) {
+                    double sk[G];
+#pragma unroll
+                    for (int q = 0; q < G; ++q) sk[q] = k < K ? jl.Su[q * kRbMaxPairs + k] : 1.0;
+                    for (int b = bb; b < B; b += per) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int q = 0; q < G; ++q) acc += jl.hu[q * B + b] * sk[q];
+                        if (k < K) jl.JK[b * kRbMaxPairs + k] += acc;
+                        else jl.Hb[b] += acc;
                     }
                 }
-                for (int b = lg; act && b < B; b += L)
-                    if (hu[b] != 0.0) atomicAdd(&jl.Hb[b], hu[b]);
                 __syncthreads();
                 for (int b = lg; b < B; b += L) hu[b] = 0.0;
             } else if (act) {
@@ -391,8 +407,9 @@ __host__ __device__ inline size_t rb_pmf_floats(int cap_big, int nbuf) {
 }
 
 // One wave per burst at a time (bursts from the counters after the walk's).
-// Dynamic LDS: B 8-byte entries (the fixed-point histogram, or the cotangent)
-// followed by the float buffers, then (Jacobian) the JacLds arrays.  The
+// Dynamic LDS: B 8-byte entries (the forward's fixed-point histogram or the
+// adjoint's cotangent; none in the Jacobian) followed by the float buffers,
+// then (Jacobian) the JacLds arrays.  The
 // adjoint is held to 128 VGPRs (4 waves per SIMD instead of 3, a few values
 // spilled to scratch): its deconvolution scans wait on shuffles and LDS, and
 // the extra wave measured 8-14 % faster.  The Jacobian's LDS allows 3 waves
@@ -410,23 +427,27 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     constexpr bool kGrad = kMode != kRbFwd;
     extern __shared__ double s_dyn[];
     __shared__ RbModel sR;
+    // B 8-byte entries first (the forward's histogram, the adjoint's
+    // cotangent; none for the Jacobian)
+    const int head = kMode == kRbJac ? 0 : a.B;
     unsigned long long* s_h = reinterpret_cast<unsigned long long*>(s_dyn);
     double* s_g = s_dyn;
-    float* s_buf = reinterpret_cast<float*>(s_dyn + a.B);
+    float* s_buf = reinterpret_cast<float*>(s_dyn + head);
     const int B = a.B;
     JacLds jl;
     const int nbuf = rb_pmf_buffers(kMode, pmf_io != nullptr);
-    jl.JE = s_dyn + B + rb_pmf_floats(cap_big, nbuf) / 2;
+    jl.JE = s_dyn + head + rb_pmf_floats(cap_big, nbuf) / 2;
     jl.JK = jl.JE + B * kRbMaxStates;
     jl.hu = jl.JK + B * kRbMaxPairs;
     jl.Hb = jl.hu + B * (kWave / kRbSmallLanes);
     jl.Sk = jl.Hb + B;
+    jl.Su = jl.Sk + kRbMaxPairs;
     const uint32_t lane = threadIdx.x;
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(rbm);
         uint32_t* dst = reinterpret_cast<uint32_t*>(&sR);
         for (uint32_t i = lane; i < sizeof(RbModel) / 4; i += kWave) dst[i] = src[i];
-        for (uint32_t i = lane; i < uint32_t(a.B); i += kWave) {
+        for (uint32_t i = lane; i < uint32_t(head); i += kWave) {
             if (kMode == kRbGrad) s_g[i] = g[i];
             else s_h[i] = 0ull;
         }
@@ -597,7 +618,7 @@ hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, c
 }
 
 size_t rb_pmf_lds(int B, int cap_big, int mode, bool stored) {
-    return sizeof(double) * size_t(B) +
+    return sizeof(double) * size_t(mode == kRbJac ? 0 : B) +
            sizeof(float) * rb_pmf_floats(cap_big, rb_pmf_buffers(mode, stored)) +
            (mode == 2 ? sizeof(double) * rb_jac_doubles(B) : 0u);
 }
