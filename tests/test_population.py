@@ -209,3 +209,26 @@ def test_population_errors(gpu_mc, data):
         a = np.zeros(7, np.float32)
         pda._check(m._L.pda_set_population(m._h, 0, 2, nst.ctypes.data_as(pda._i32p),
                                            a.ctypes.data_as(pda._f32p), 7))
+
+
+def test_full_population_workload_against_oracle(gpu_mc, oracle):
+    """The batched-population workload at full size (256 sets of 2-4 states x
+    50,000 bursts x 32 resamples in one launch sequence): four sets spread
+    over the population equal the oracle bit for bit."""
+    cfg = synth.config(2)
+    d, R, pop = cfg["data"], cfg["R"], cfg["population"]
+    assert len(pop) == 256 and d.nbursts == 50_000
+    m = gpu_mc
+    m.load(d, SgDivSr=np.zeros(d.nbursts, np.float32), nbins=50, left_pad=2)
+    m.set_photophysics(0, *cfg["photophysics"])
+    m.set_burst_range(0, -1, 0)
+    m.set_population([n for n, _ in pop], [a for _, a in pop])
+    seed = cfg["seeds"]["mc"]
+    m.run_population(R=R, seed=seed, set_base=0, set_stride=1)
+    hist, _, _, _ = m.get_population_result()
+    od = oracle.Data.from_bursts(d)
+    for i in (0, 85, 170, 255):
+        n, a = pop[i]
+        p = oracle.make_params(n, a, *cfg["photophysics"])
+        want = oracle.run_threaded(od, p, R=R, seed=seed, set_id=i, nthreads=16)
+        assert np.array_equal(hist[i].astype(np.int64), want), f"set {i} (n={n})"

@@ -76,6 +76,22 @@ def test_rb_forward_bit_exact(gpu_mc, oracle, n, bg, rng_):
     gpu_mc.set_burst_range(0, -1, 0)
 
 
+def test_rb_forward_full_workload_bit_exact(gpu_mc, oracle):
+    """The Rao-Blackwellised workload at full size (3 states, 100,000 bursts
+    x 64 resamples, with scores and the pmf store): the fixed-point
+    histogram equals the oracle's."""
+    cfg = synth.config(3)
+    d = cfg["data"]
+    gpu_mc.load(d, SgDivSr=np.zeros(d.nbursts, np.float32), nbins=50, left_pad=2)
+    gpu_mc.set_photophysics(0, *cfg["photophysics"])
+    gpu_mc.set_burst_range(0, -1, 0)
+    M, fixed, c2, eff = _rb(gpu_mc, cfg["n"], cfg["args"], cfg["R"], 3003)
+    od = oracle.Data.from_bursts(d)
+    p = oracle.make_params(cfg["n"], np.asarray(cfg["args"], np.float32), *cfg["photophysics"])
+    want = oracle.rb_run(od, p, cfg["R"], 3003, 0)
+    assert np.array_equal(fixed, want), np.nonzero(fixed != want)
+
+
 def test_rb_forward_does_not_depend_on_the_launch_shape(gpu_mc):
     d = _data(300, 37)
     _load(gpu_mc, d)
