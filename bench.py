@@ -273,11 +273,13 @@ def main():
         m.run_kernel(sid=0, seed=seed, set_id=0)
     m.synchronize(0)
     kernel_ms, walk_ms = [], []
+    t_one = time.perf_counter()
     for _ in range(args.steps):
         m.run_kernel(sid=0, seed=seed, set_id=0)
         m.synchronize(0)
         kernel_ms.append(m.last_run_ms(0))
         walk_ms.append(m.last_pass_ms(0)[0])
+    t_one = time.perf_counter() - t_one
     hist, chi2_64, chi2_ref, effN = m.get_hist_chi2(0)
 
     def evaluations(k):
@@ -324,6 +326,8 @@ def main():
                    "bursts": d.nbursts, "photons": int(d.nphotons), "resamples": R,
                    "bins": 50, "states": n, "parallelism": "single GPU", "streams": S},
         "gpu_ms_per_eval_avg": float(np.mean(kernel_ms)),
+        "one_at_a_time": {"value": units * args.steps / t_one,
+                          "ms_per_step": t_one / args.steps * 1e3},
         "chi2": chi2_64,
     }
     out["roofline"], out["roofline_hbm"] = roofline(units, float(np.mean(walk_ms)),
