@@ -12,7 +12,7 @@
 //     (seed, set, window, sub-stream, resample, burst) -- no RNG state in
 //     memory, nothing to initialise (replaces setup_kernel,
 //     src/gen_rand.cuh:7-25);
-//   * the common case runs in a lean form (about 80 VGPRs); the rare units
+//   * the common case runs in a lean form (at most 72 VGPRs); the rare units
 //     that need BTPE draws, inversion restarts or large background means are
 //     handed to finish_kernel, which replays them in full, reduces the LDS
 //     histograms of all workgroups and computes chi2 on the device.
