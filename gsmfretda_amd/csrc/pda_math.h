@@ -397,8 +397,12 @@ PDA_HD float background_count(float lam, float cap, float u_first, float u_cap,
             float u = u_first;
             if (c > 0) {
                 const uint32_t q = uint32_t(c - 1);
-                if ((q & 3u) == 0u) extra = draw_block(key, w, sub_base + (q >> 2));
-                u = uniform24(extra.w[q & 3u]);
+                const uint32_t i = q & 3u;
+                if (i == 0u) extra = draw_block(key, w, sub_base + (q >> 2));
+                // selects, not extra.w[i]: an indexed word would put the block
+                // in scratch memory, stored on every call
+                u = uniform24(i == 0u ? extra.w[0]
+                                      : (i == 1u ? extra.w[1] : (i == 2u ? extra.w[2] : extra.w[3])));
             }
             int64_t k = 0;
             float pk = p0, cdf = p0;
