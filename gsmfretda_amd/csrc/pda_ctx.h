@@ -77,6 +77,7 @@ struct Population {
     uint32_t last_seed = 0, last_base = 0, last_stride = 0;
     bool last_graph = false;
     hipEvent_t ev0 = nullptr, ev_mid = nullptr, ev1 = nullptr;
+    bool ready = false;  // counters and events allocated
 };
 
 // Rao-Blackwellised runs of a slot: per-unit walk summaries of the last
@@ -111,6 +112,7 @@ struct RbState {
     bool pmf_valid = false;
     double* h_grad = nullptr;          // pinned
     hipEvent_t ev0 = nullptr, ev_walk = nullptr, ev1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;
+    bool ready = false;                // every fixed-size buffer above allocated
     bool ran = false, scores = false, adj_ran = false;
     int R = 0, B = 0, n = 0, K = 0, cap = 0, left_pad = 0;
     int64_t b0 = 0, b1 = 0;

@@ -79,14 +79,15 @@ size_t out_bytes(size_t nsets, int B) {
 // Buffers that depend only on the population size and the bin count.
 int ensure_pop_buffers(pda_ctx* ctx, Slot& s, int B) {
     Population& p = s.pop;
-    if (!p.d_work) {
-        HIP_TRY(hipMalloc(&p.d_work, sizeof(uint32_t) * kWorkWords));
+    if (!p.ready) {
+        // each once: a call that fails part way leaves the rest for the next
+        if (!p.d_work) HIP_TRY(hipMalloc(&p.d_work, sizeof(uint32_t) * kWorkWords));
         HIP_TRY(hipMemset(p.d_work, 0, sizeof(uint32_t) * kWorkWords));
-        HIP_TRY(hipMalloc(&p.d_replay_count, sizeof(uint32_t)));
+        if (!p.d_replay_count) HIP_TRY(hipMalloc(&p.d_replay_count, sizeof(uint32_t)));
         HIP_TRY(hipMemset(p.d_replay_count, 0, sizeof(uint32_t)));
-        HIP_TRY(hipEventCreate(&p.ev0));
-        HIP_TRY(hipEventCreate(&p.ev_mid));
-        HIP_TRY(hipEventCreate(&p.ev1));
+        for (hipEvent_t* e : {&p.ev0, &p.ev_mid, &p.ev1})
+            if (!*e) HIP_TRY(hipEventCreate(e));
+        p.ready = true;
     }
     // histogram rows: [set][XCD][B] for the run's B; pda_set_population frees
     // them when the population outgrows them

@@ -107,28 +107,37 @@ int ensure_pmf_store(pda_ctx* ctx, RbState& r, int64_t b0, int64_t b1, int R, fl
 }
 
 int ensure_rb(pda_ctx* ctx, RbState& r) {
-    if (!r.d_work) {
-        HIP_TRY(hipMalloc(&r.d_rbm, sizeof(RbModel)));
-        HIP_TRY(hipHostMalloc(&r.h_rbm, sizeof(RbModel), hipHostMallocDefault));
-        HIP_TRY(hipMalloc(&r.d_hist, sizeof(unsigned long long) * 8u * kMaxBins));
+    if (!r.ready) {
+        // each buffer once: a call that fails part way (out of memory) leaves
+        // what it got for the next call, which allocates the rest
+        if (!r.d_rbm) HIP_TRY(hipMalloc(&r.d_rbm, sizeof(RbModel)));
+        if (!r.h_rbm) HIP_TRY(hipHostMalloc(&r.h_rbm, sizeof(RbModel), hipHostMallocDefault));
+        if (!r.d_hist) HIP_TRY(hipMalloc(&r.d_hist, sizeof(unsigned long long) * 8u * kMaxBins));
         HIP_TRY(hipMemset(r.d_hist, 0, sizeof(unsigned long long) * 8u * kMaxBins));
-        HIP_TRY(hipMalloc(&r.d_work, sizeof(uint32_t) * kRbCounters));
+        if (!r.d_work) HIP_TRY(hipMalloc(&r.d_work, sizeof(uint32_t) * kRbCounters));
         HIP_TRY(hipMemset(r.d_work, 0, sizeof(uint32_t) * kRbCounters));
-        HIP_TRY(hipMalloc(&r.d_res, kRbResBytes));
-        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_res), kRbResBytes, hipHostMallocDefault));
-        HIP_TRY(hipMalloc(&r.d_g, sizeof(double) * kMaxBins));
-        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_g), sizeof(double) * kMaxBins,
-                              hipHostMallocDefault));
-        HIP_TRY(hipMalloc(&r.d_grad, sizeof(double) * kGradLen));
-        HIP_TRY(hipMalloc(&r.d_pmf_tab, sizeof(float) * size_t(kRbMaxStates) * kRbPmfStride));
-        HIP_TRY(hipMalloc(&r.d_jac, sizeof(double) * kGradLen * kMaxBins));
-        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_jac), sizeof(double) * kGradLen * kMaxBins,
-                              hipHostMallocDefault));
-        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_grad), sizeof(double) * kGradLen,
-                              hipHostMallocDefault));
+        if (!r.d_res) HIP_TRY(hipMalloc(&r.d_res, kRbResBytes));
+        if (!r.h_res)
+            HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_res), kRbResBytes,
+                                  hipHostMallocDefault));
+        if (!r.d_g) HIP_TRY(hipMalloc(&r.d_g, sizeof(double) * kMaxBins));
+        if (!r.h_g)
+            HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_g), sizeof(double) * kMaxBins,
+                                  hipHostMallocDefault));
+        if (!r.d_grad) HIP_TRY(hipMalloc(&r.d_grad, sizeof(double) * kGradLen));
+        if (!r.d_pmf_tab)
+            HIP_TRY(hipMalloc(&r.d_pmf_tab, sizeof(float) * size_t(kRbMaxStates) * kRbPmfStride));
+        if (!r.d_jac) HIP_TRY(hipMalloc(&r.d_jac, sizeof(double) * kGradLen * kMaxBins));
+        if (!r.h_jac)
+            HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_jac),
+                                  sizeof(double) * kGradLen * kMaxBins, hipHostMallocDefault));
+        if (!r.h_grad)
+            HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_grad), sizeof(double) * kGradLen,
+                                  hipHostMallocDefault));
         for (hipEvent_t* e : {&r.ev0, &r.ev_walk, &r.ev1, &r.ev_a0, &r.ev_a1})
-            HIP_TRY(hipEventCreate(e));
+            if (!*e) HIP_TRY(hipEventCreate(e));
         HIP_TRY(hipDeviceSynchronize());
+        r.ready = true;
     }
     if (!ctx->d_lf) {
         std::vector<double> lf(kRbMaxCount + 1);
