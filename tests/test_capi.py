@@ -71,3 +71,12 @@ def test_missing_library_raises(tmp_path):
             pda.load_library(str(tmp_path / "nope.so"))
     finally:
         pda._lib = saved
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus above 1 exits with a message before
+    touching a device."""
+    r = subprocess.run(["python", os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
