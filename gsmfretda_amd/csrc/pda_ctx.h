@@ -34,7 +34,7 @@ const std::string& last_error();
 struct LaunchShape {
     uint32_t batch_tiles = 0;   // 64-unit tiles per work-counter grab
     uint32_t waves_per_cu = 0;  // resident one-wave workgroups per CU (grid size)
-    uint32_t queues = 0;        // work counters (1..8)
+    uint32_t queues = 0;        // work counters (1..kMaxQueues)
 };
 
 // Key of a captured population graph: everything baked into its nodes.
@@ -218,13 +218,15 @@ hipError_t launch_pop_finish(const ModelBlob* blobs, const RunArgs& a, const flo
                              hipStream_t stream);
 hipError_t launch_primitive(int op, const void* in, void* out, int64_t n, hipStream_t stream);
 struct RbModel;
+// which: 0/1 RB walk without/with scores; 2 forward, 3 adjoint, 4 Jacobian,
+// 5 forward in place (rb_pmf_kernel)
 hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu);
 hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
                           const uint2* links, const ModelBlob* blob, const RbModel* rbm,
                           const RunArgs& a, uint2* cnt, float* F, float* S, uint32_t* work,
                           hipStream_t stream);
-// mode: 0 forward, 1 adjoint, 2 Jacobian; stored: the pass reads the
-// forward's stored pmfs (fewer LDS buffers)
+// mode: 0 forward, 1 adjoint, 2 Jacobian, 3 forward in place; stored: the
+// pass reads the forward's stored pmfs (fewer LDS buffers)
 size_t rb_pmf_lds(int B, int cap_big, int mode, bool stored);
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
                            hipStream_t stream);

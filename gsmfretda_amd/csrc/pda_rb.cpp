@@ -276,8 +276,19 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, (uint64_t(units) + batch - 1) / batch)));
     a.batch = uint32_t(batch);
     a.queues = std::min<uint32_t>(ctx->shape.queues ? ctx->shape.queues : kDefaultQueues, grid);
-    const int dyn = int(rb_pmf_lds(ctx->B, cap, 0, false));
-    HIP_TRY(rb_occupancy(2, dyn, &per_cu));
+    // the in-place forward (two LDS buffers, more registers) when it lets
+    // more waves share a CU; otherwise (bursts large enough that the
+    // whole-wave layout sets the LDS) the three-buffer one
+    int fwd_mode = 0;
+    {
+        int per_ip = 0;
+        HIP_TRY(rb_occupancy(2, int(rb_pmf_lds(ctx->B, cap, 0, false)), &per_cu));
+        HIP_TRY(rb_occupancy(5, int(rb_pmf_lds(ctx->B, cap, 3, false)), &per_ip));
+        if (per_ip > per_cu) {
+            fwd_mode = 3;
+            per_cu = per_ip;
+        }
+    }
     per_cu = std::max(per_cu, 1);
     const uint32_t pgrid = uint32_t(std::max<int64_t>(
         1, std::min<int64_t>(int64_t(ctx->num_cu) * per_cu, b1 - b0)));
@@ -297,7 +308,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     HIP_TRY(hipEventRecord(r.ev_walk, s->stream));
     HIP_TRY(launch_rb_table(static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, r.d_pmf_tab, s->n,
                             s->stream));
-    HIP_TRY(launch_rb_pmf(0, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
+    HIP_TRY(launch_rb_pmf(fwd_mode, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
                           r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, r.d_hist, nullptr,
                           r.d_work, pmf_io, r.d_poff, s->stream));
     HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s->stream));

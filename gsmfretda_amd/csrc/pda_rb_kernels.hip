@@ -168,8 +168,10 @@ constexpr int kRbSmallCap = 256;   // acceptor totals below this use the small l
 
 // All R units of one burst, kWave / L at a time (one unit per group of L
 // lanes, lanes over acceptor totals).  Per group, LDS buffers X (the pmf so
-// far), Z (a free buffer) and Y (the next state's binomial pmf) of `cap`
-// floats each; the group's buffer roles are swapped instead of copied.
+// far), Y (the next state's binomial pmf) and Z (a free buffer) of `cap`
+// floats each; the group's buffer roles are swapped instead of copied.  The
+// in-place forward (kRbFwdInPlace) writes each 16-lane product back over X
+// and has no Z.
 //
 // Forward (kGrad = false): the unit's pmf P(a), a = 0..Mt, goes into the LDS
 // histogram at E = a / F in 2^-32 fixed point.
@@ -189,7 +191,10 @@ constexpr int kRbSmallCap = 256;   // acceptor totals below this use the small l
 // c and c + 1 fall in different bins, so d M_b / d p_s gathers m_s P_s(c) at
 // the unit's bin boundaries; the rate columns use the unit's histogram h_rb
 // in place of h_r (sum_r h_rb S_r here, the mean term per burst).
-constexpr int kRbFwd = 0, kRbGrad = 1, kRbJac = 2;
+// kRbFwdInPlace: the forward with the 16-lane products written back over X
+// from registers (two buffers per group instead of three, more registers):
+// chosen when that lets more waves share a CU.
+constexpr int kRbFwd = 0, kRbGrad = 1, kRbJac = 2, kRbFwdInPlace = 3;
 
 template <int L, int kMode>
 __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __restrict__ lf,
@@ -202,7 +207,7 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
                                          uint32_t lane, double& gp_acc, double& gk_acc,
                                          const JacLds& jl, float* __restrict__ pmf_io,
                                          const uint64_t* __restrict__ poff) {
-    constexpr bool kGrad = kMode != kRbFwd;
+    constexpr bool kGrad = kMode == kRbGrad || kMode == kRbJac;
     constexpr int G = kWave / L;
     const int g = int(lane) / L, lg = int(lane) % L;
     const int n = sR.n, K = sR.K, B = a.B;
@@ -218,7 +223,10 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
         const uint32_t u = bl * a.R + (act ? r : 0u);
         const uint2 c = act ? cnt_in[u] : make_uint2(0u, 0u);
         const float F = act ? F_in[u] : 0.0f;
-        int bX = 0, bZ = 1, bY = 2;
+        // the forward in the 16-lane layout convolves in place (X and Y
+        // only); every other case writes products to Z
+        constexpr bool kInPlace = kMode == kRbFwdInPlace && L == kRbSmallLanes;
+        int bX = 0, bZ = 1, bY = kInPlace ? 1 : 2;
         int xl = 1;
         if (lg == 0) base[0] = 1.0f;
         if (kGrad && pmf_io) {
@@ -243,6 +251,26 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
                         bX = bY;
                         bY = t;
                         xl = m + 1;
+                    } else if (kInPlace) {
+                        // at most kRbSmallCap / L outputs per lane, held in
+                        // registers until every read of X is done: the
+                        // group's lanes are lanes of one wave, so all the
+                        // loads below are issued before any store, and each
+                        // store waits on the loads its value came from
+                        constexpr int kOut = kRbSmallCap / kRbSmallLanes;
+                        float* X = base + bX * cap;
+                        float zr[kOut];
+#pragma unroll
+                        for (int i = 0; i < kOut; ++i) {
+                            const int q = lg + i * L;
+                            zr[i] = q < xl + m ? rb_conv_at(Y, m + 1, X, xl, q) : 0.0f;
+                        }
+#pragma unroll
+                        for (int i = 0; i < kOut; ++i) {
+                            const int q = lg + i * L;
+                            if (q < xl + m) X[q] = zr[i];
+                        }
+                        xl += m;
                     } else {
                         const float* X = base + bX * cap;
                         float* Z = base + bZ * cap;
@@ -391,32 +419,39 @@ __device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __rest
     }
 }
 
-// pmf buffers per lane group: the forward (and a derivative pass without
-// stored pmfs) convolves in X, Y and Z; the adjoint reading stored pmfs needs
-// X and Z (the cotangent differences), the Jacobian X alone.  Fewer buffers
-// leave LDS for more resident waves.
-__host__ __device__ inline int rb_pmf_buffers(int mode, bool stored) {
-    return !stored || mode == kRbFwd ? 3 : (mode == kRbGrad ? 2 : 1);
+// pmf buffers per lane group: the forward convolves in X and Y (in place)
+// in the 16-lane layout and in X, Y and Z in the whole-wave one; a
+// derivative pass without stored pmfs rebuilds them in X, Y and Z; with
+// stored pmfs the adjoint needs X and Z (the cotangent differences), the
+// Jacobian X alone.  Fewer buffers leave LDS for more resident waves.
+__host__ __device__ inline int rb_pmf_buffers(int mode, bool stored, bool small) {
+    if (stored && mode == kRbJac) return 1;
+    if (stored && mode == kRbGrad) return 2;
+    return mode == kRbFwdInPlace && small ? 2 : 3;
 }
 
-// Float entries of the pmf buffers: 4 groups x nbuf x 256 or nbuf x cap_big.
-__host__ __device__ inline size_t rb_pmf_floats(int cap_big, int nbuf) {
-    const size_t small = size_t(kWave / kRbSmallLanes) * size_t(nbuf) * kRbSmallCap;
-    const size_t big = size_t(nbuf) * size_t(cap_big);
+// Float entries of the pmf buffers: 4 groups x nbuf x 256 (16-lane layout)
+// or nbuf x cap_big (whole-wave layout), whichever is larger.
+__host__ __device__ inline size_t rb_pmf_floats(int cap_big, int mode, bool stored) {
+    const size_t small = size_t(kWave / kRbSmallLanes) *
+                         size_t(rb_pmf_buffers(mode, stored, true)) * kRbSmallCap;
+    const size_t big = size_t(rb_pmf_buffers(mode, stored, false)) * size_t(cap_big);
     return ((small > big ? small : big) + 1u) & ~size_t(1);  // even: doubles follow
 }
 
 // One wave per burst at a time (bursts from the counters after the walk's).
 // Dynamic LDS: B 8-byte entries (the forward's fixed-point histogram or the
 // adjoint's cotangent; none in the Jacobian) followed by the float buffers,
-// then (Jacobian) the JacLds arrays.  The
+// then (Jacobian) the JacLds arrays.  The in-place forward, whose products
+// sit in registers, is held to 128 VGPRs (4 waves per SIMD) so that its
+// smaller LDS can let more waves in; the
 // adjoint is held to 128 VGPRs (4 waves per SIMD instead of 3, a few values
 // spilled to scratch): its deconvolution scans wait on shuffles and LDS, and
 // the extra wave measured 8-14 % faster.  The Jacobian's LDS allows 3 waves
 // per SIMD whatever its registers.
 template <int kMode>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
-    kMode == kRbGrad ? 4 : 1))) void rb_pmf_kernel(
+    kMode == kRbGrad || kMode == kRbFwdInPlace ? 4 : 1))) void rb_pmf_kernel(
     const RbModel* __restrict__ rbm, const double* __restrict__ lf,
     const float* __restrict__ pmf_tab, const RunArgs a, const BurstDesc* __restrict__ bursts,
     const uint2* __restrict__ cnt_in, const float* __restrict__ F_in,
@@ -424,7 +459,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     unsigned long long* __restrict__ hist_acc, double* __restrict__ grad,
     uint32_t* __restrict__ work, int cap_big, float* __restrict__ pmf_io,
     const uint64_t* __restrict__ poff) {
-    constexpr bool kGrad = kMode != kRbFwd;
+    constexpr bool kGrad = kMode == kRbGrad || kMode == kRbJac;
     extern __shared__ double s_dyn[];
     __shared__ RbModel sR;
     // B 8-byte entries first (the forward's histogram, the adjoint's
@@ -435,8 +470,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     float* s_buf = reinterpret_cast<float*>(s_dyn + head);
     const int B = a.B;
     JacLds jl;
-    const int nbuf = rb_pmf_buffers(kMode, pmf_io != nullptr);
-    jl.JE = s_dyn + head + rb_pmf_floats(cap_big, nbuf) / 2;
+    const bool stored = pmf_io != nullptr;
+    jl.JE = s_dyn + head + rb_pmf_floats(cap_big, kMode, stored) / 2;
     jl.JK = jl.JE + B * kRbMaxStates;
     jl.hu = jl.JK + B * kRbMaxPairs;
     jl.Hb = jl.hu + B * (kWave / kRbSmallLanes);
@@ -467,7 +502,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     // derivative passes, several times dearer per burst, take guided batches
     // (a quarter of the wave's share of what its queue had left at its last
     // grab) and then single bursts from the other queues.
-    constexpr bool kGuided = kMode != kRbFwd;
+    constexpr bool kGuided = kGrad;
     uint32_t batch = kGuided ? qn / (waves_q * 4u) : nb / (gridDim.x * 16u);
     batch = batch < 1u ? 1u : (batch > kRbMaxBurstBatch ? kRbMaxBurstBatch : batch);
     double gp_acc = 0.0, gk_acc = 0.0;  // lane s: d/dp_s; lane i: d/dk_i
@@ -501,11 +536,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
         const int bound = rb_total_bound(bursts[a.begin + bl].P, sR.coef);
         if (bound < kRbSmallCap)
             rb_burst<kRbSmallLanes, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h,
-                                           s_buf, nbuf, kRbSmallCap, lane, gp_acc, gk_acc, jl,
+                                           s_buf, rb_pmf_buffers(kMode, stored, true),
+                                           kRbSmallCap, lane, gp_acc, gk_acc, jl,
                                            pmf_io, poff);
         else
             rb_burst<kWave, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf,
-                                   nbuf, cap_big, lane, gp_acc, gk_acc, jl, pmf_io, poff);
+                                   rb_pmf_buffers(kMode, stored, false), cap_big, lane,
+                                   gp_acc, gk_acc, jl, pmf_io, poff);
     }
     if (kMode == kRbJac) {
         // grad holds J[b][i], i < n + K (efficiencies, then rates in args
@@ -599,6 +636,7 @@ hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu) {
         case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_walk_kernel<false>, kWave, 0);
         case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_walk_kernel<true>, kWave, 0);
         case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbFwd>, kWave, size_t(dyn_lds));
+        case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbFwdInPlace>, kWave, size_t(dyn_lds));
         case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbGrad>, kWave, size_t(dyn_lds));
         default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbJac>, kWave, size_t(dyn_lds));
     }
@@ -619,7 +657,7 @@ hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, c
 
 size_t rb_pmf_lds(int B, int cap_big, int mode, bool stored) {
     return sizeof(double) * size_t(mode == kRbJac ? 0 : B) +
-           sizeof(float) * rb_pmf_floats(cap_big, rb_pmf_buffers(mode, stored)) +
+           sizeof(float) * rb_pmf_floats(cap_big, mode, stored) +
            (mode == 2 ? sizeof(double) * rb_jac_doubles(B) : 0u);
 }
 
@@ -644,6 +682,10 @@ hipError_t launch_rb_pmf(int mode, uint32_t grid, int cap_big, const RbModel* rb
                            pmf_io, poff);
     else if (mode == kRbJac)
         hipLaunchKernelGGL(rb_pmf_kernel<kRbJac>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
+                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big,
+                           pmf_io, poff);
+    else if (mode == kRbFwdInPlace)
+        hipLaunchKernelGGL(rb_pmf_kernel<kRbFwdInPlace>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
                            pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big,
                            pmf_io, poff);
     else

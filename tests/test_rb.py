@@ -76,6 +76,22 @@ def test_rb_forward_bit_exact(gpu_mc, oracle, n, bg, rng_):
     gpu_mc.set_burst_range(0, -1, 0)
 
 
+@pytest.mark.parametrize("base,geom", [(200, 60), (400, 150)])
+def test_rb_forward_bit_exact_long_bursts(gpu_mc, oracle, base, geom):
+    """Bursts with acceptor totals of 256 and more take the whole-wave layout:
+    mixed with shorter ones (the in-place forward, both layouts in one
+    launch) and all long (the three-buffer forward, chosen when the long
+    bursts set the LDS)."""
+    d = synth.generate(120, synth.make_args([0.2, 0.55, 0.85], [900, 2500, 700, 3000, 1500, 5000]),
+                       3, base_photons=base, mean_geom=geom, seed=61)
+    _load(gpu_mc, d)
+    M, fixed, c2, eff = _rb(gpu_mc, 3, ARGS3, 6, 91, set_id=1)
+    od = oracle.Data.from_bursts(d)
+    p = oracle.make_params(3, np.asarray(ARGS3, np.float32), *PH)
+    want = oracle.rb_run(od, p, 6, 91, 1)
+    assert np.array_equal(fixed, want), np.nonzero(fixed != want)
+
+
 def test_rb_forward_full_workload_bit_exact(gpu_mc, oracle):
     """The Rao-Blackwellised workload at full size (3 states, 100,000 bursts
     x 64 resamples, with scores and the pmf store): the fixed-point
