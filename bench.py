@@ -36,26 +36,47 @@ METRIC = "burst-resamples/sec on 1 MI355X; fit wall-time to chi2 tol (grad+batch
 COST_MODEL = os.path.join(ROOT, "gsmfretda_amd", "cost_model.json")
 
 
-def cpu_baseline(cfg, threads, sample_bursts, repeats=3):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, threads, sample_bursts, repeats=5, single_bursts=2000):
     """The oracle (REF-semantics CPU MC on the same streams) on host cores:
-    median of `repeats` evaluations of the first `sample_bursts` bursts."""
+    median of `repeats` evaluations of the first `sample_bursts` bursts on
+    `threads` threads, and the single-thread oracle on the first
+    `single_bursts` bursts."""
     from oracle import oracle as O
     d = cfg["data"]
     od = O.Data.from_bursts(d)
     p = O.make_params(cfg["n"], cfg["args"], *cfg["photophysics"])
     R = cfg["R"]
+    seed = cfg["seeds"]["mc"]
     b1 = min(sample_bursts, d.nbursts)
-    O.run_threaded(od, p, R=R, seed=cfg["seeds"]["mc"], b1=min(b1, 2000), nthreads=threads)
+    O.run_threaded(od, p, R=R, seed=seed, b1=min(b1, 2000), nthreads=threads)
     times = []
     for _ in range(repeats):
         t0 = time.perf_counter()
-        O.run_threaded(od, p, R=R, seed=cfg["seeds"]["mc"], b1=b1, nthreads=threads)
+        O.run_threaded(od, p, R=R, seed=seed, b1=b1, nthreads=threads)
         times.append(time.perf_counter() - t0)
     dt = float(np.median(times))
+    s1 = min(single_bursts, d.nbursts)
+    t0 = time.perf_counter()
+    O.run_threaded(od, p, R=R, seed=seed, b1=s1, nthreads=1)
+    dt1 = time.perf_counter() - t0
     return dict(value=b1 * R / dt, unit="burst-resamples/s", cores=threads, kind="port",
                 sample=f"{b1} of {d.nbursts} bursts x {R} resamples (the whole evaluation), "
                        f"oracle/pda_oracle.c literal photon scan, median of {repeats} runs, "
-                       f"{threads} threads, {dt:.2f} s wall each")
+                       f"{threads} threads, {dt:.2f} s wall each",
+                cpu=cpu_model(), host_cpus=os.cpu_count(),
+                single_thread={"value": s1 * R / dt1, "sample": f"{s1} bursts x {R} resamples, "
+                               f"1 thread, {dt1:.2f} s"})
 
 
 def roofline(units, walk_ms, total_ms):
