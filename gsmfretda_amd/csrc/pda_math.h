@@ -40,6 +40,16 @@ struct UnitKey {
     uint32_t seed, set, r, b;
 };
 
+// a ^ b ^ c.  On gfx950 one v_bitop3_b32 (truth table 0x96 = 0xF0 ^ 0xCC ^
+// 0xAA), which the compiler does not form from two XORs by itself.
+PDA_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 PDA_HD Block philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                       uint32_t k1) {
     // Philox4x32-10, Salmon et al., "Parallel random numbers: as easy as
@@ -48,8 +58,8 @@ PDA_HD Block philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32
     for (int i = 0; i < 10; ++i) {
         const uint64_t m0 = uint64_t(0xD2511F53u) * c0;
         const uint64_t m1 = uint64_t(0xCD9E8D57u) * c2;
-        const uint32_t y0 = uint32_t(m1 >> 32) ^ c1 ^ k0;
-        const uint32_t y2 = uint32_t(m0 >> 32) ^ c3 ^ k1;
+        const uint32_t y0 = xor3(uint32_t(m1 >> 32), c1, k0);
+        const uint32_t y2 = xor3(uint32_t(m0 >> 32), c3, k1);
         c1 = uint32_t(m1);
         c3 = uint32_t(m0);
         c0 = y0;
