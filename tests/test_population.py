@@ -169,6 +169,26 @@ def test_population_with_broadening(gpu_mc, oracle, data):
         assert c32[i] == np.float32(c_o)
 
 
+def test_population_up_to_eight_states(gpu_mc, oracle, data):
+    """Sets of 5 to 8 states (the largest model the tables hold) mixed with a
+    2-state set in one population: every set equal to the oracle."""
+    d, obs = data
+    m = gpu_mc
+    _load(m, d, obs)
+    sets = _population(6, 13, states=(5, 6, 7, 8)) + _population(1, 14, states=(2,))
+    assert {n for n, _ in sets} == {2, 5, 6, 7, 8}
+    m.set_population([n for n, _ in sets], [a for _, a in sets])
+    R, seed = 6, 808
+    m.run_population(R=R, seed=seed, set_base=50, set_stride=2)
+    hist, _, c32, _ = m.get_population_result()
+    od = oracle.Data.from_bursts(d)
+    oh = _obs_hist(oracle, obs)
+    for i, (n, a) in enumerate(sets):
+        h_o, c_o = _oracle_set(oracle, od, oh, n, a, R, seed, 50 + 2 * i)
+        assert np.array_equal(hist[i].astype(np.int64), h_o), f"set {i} (n={n})"
+        assert c32[i] == np.float32(c_o), f"set {i}"
+
+
 def test_launch_shapes_and_autotune_do_not_change_results(gpu_mc, data):
     d, obs = data
     m = gpu_mc
