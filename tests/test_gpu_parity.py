@@ -170,6 +170,25 @@ def test_ranges_neither_photons_and_no_background(gpu_mc, oracle):
         _compare_units(g, o)
 
 
+def test_empty_burst_range(gpu_mc, oracle):
+    """An empty [begin, end) runs nothing: a zero histogram and no bins kept;
+    the next run on the slot equals the oracle."""
+    cfg = synth.config(0, n_bursts=300)
+    d = cfg["data"]
+    _load(gpu_mc, d, np.zeros(300, np.float32))
+    gpu_mc.set_photophysics(0, *cfg["photophysics"])
+    assert gpu_mc.set_params(cfg["n"], 0, cfg["args"])
+    assert gpu_mc.set_burst_range(120, 120, 0) == 0
+    gpu_mc.reSampleTimes = 9
+    gpu_mc.run_kernel(sid=0, seed=5, set_id=0)
+    gpu_mc.synchronize(0)
+    hist, _, _, eff = gpu_mc.get_hist_chi2(0)
+    assert not hist.any() and eff == 0
+    g = _gpu_run(gpu_mc, cfg["n"], cfg["args"], 9, 5, ph=cfg["photophysics"])
+    o = _oracle_run(oracle, d, cfg["n"], cfg["args"], cfg["photophysics"], 9, 5)
+    _compare_units(g, o)
+
+
 def test_broadening_and_four_states(gpu_mc, oracle):
     args = synth.make_args([0.1, 0.4, 0.6, 0.95], np.linspace(200, 3000, 12), v=[2.0, 0.0, 5.0, 1.0])
     d = synth.generate(800, args, 4, seed=5)
