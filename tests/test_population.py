@@ -189,6 +189,26 @@ def test_population_up_to_eight_states(gpu_mc, oracle, data):
         assert c32[i] == np.float32(c_o), f"set {i}"
 
 
+def test_population_at_200_bins(gpu_mc, oracle, data):
+    """The reference's default of 200 E bins: every set equal to the oracle."""
+    d, obs = data
+    m = gpu_mc
+    _load(m, d, obs, B=200)
+    sets = _population(5, 15)
+    m.set_population([n for n, _ in sets], [a for _, a in sets])
+    R, seed = 8, 77
+    m.run_population(R=R, seed=seed, set_base=0, set_stride=1)
+    hist, _, c32, _ = m.get_population_result()
+    assert hist.shape == (5, 200)
+    od = oracle.Data.from_bursts(d)
+    oh = _obs_hist(oracle, obs, 200)
+    for i, (n, a) in enumerate(sets):
+        h_o, c_o = _oracle_set(oracle, od, oh, n, a, R, seed, i, B=200)
+        assert np.array_equal(hist[i].astype(np.int64), h_o), f"set {i}"
+        assert c32[i] == np.float32(c_o), f"set {i}"
+    _load(m, d, obs)
+
+
 def test_launch_shapes_and_autotune_do_not_change_results(gpu_mc, data):
     d, obs = data
     m = gpu_mc

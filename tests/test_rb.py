@@ -167,6 +167,28 @@ def test_rb_adjoint_matches_fp64_oracle(gpu_mc, oracle):
     assert np.all(grad[9:] == 0.0)
 
 
+def test_rb_at_200_bins(gpu_mc, oracle):
+    """The reference's default of 200 E bins: forward bit-exact, adjoint
+    within 1e-3 of the FP64 oracle, Jacobian equal to the adjoint."""
+    d = _data(300, 71)
+    obs = _load(gpu_mc, d, B=200)
+    R, seed = 8, 12
+    M, fixed, _, _ = _rb(gpu_mc, 3, ARGS3, R, seed, set_id=4)
+    assert len(M) == 200
+    od = oracle.Data.from_bursts(d)
+    p = oracle.make_params(3, np.asarray(ARGS3, np.float32), *PH)
+    assert np.array_equal(fixed, oracle.rb_run(od, p, R, seed, 4, B=200))
+    g = _chi2_cotangent(M, _obs_hist(obs, 200), R, 3)
+    grad = gpu_mc.rb_adjoint(g, 3)
+    _, go = oracle.rb_grad(od, p, R, seed, g, set_id=4)
+    scale = np.abs(go).max()
+    assert np.all(np.abs(grad[:9] - go) <= 1e-3 * np.maximum(np.abs(go), 1e-3 * scale)), (grad, go)
+    J = gpu_mc.rb_jacobian(3, 200)
+    assert J.shape == (200, 12)
+    assert np.allclose(g @ J[:, :9], grad[:9], rtol=1e-5, atol=1e-5 * np.abs(grad).max())
+    _load(gpu_mc, d)
+
+
 @pytest.mark.parametrize("long_bursts", [False, True])
 def test_rb_jacobian_is_the_adjoint_of_every_cotangent(gpu_mc, oracle, long_bursts):
     # long bursts (acceptor totals >= 256) take the one-unit-per-wave layout
