@@ -22,6 +22,32 @@ void check(int rc, const char* what) {
 
 }  // namespace
 
+int showGPUsInfo(int dn, char* gpuuid, int* streamCount) {
+    int ndev = 0;
+    check(hipGetDeviceCount(&ndev) == hipSuccess ? 0 : -1, "hipGetDeviceCount");
+    if (dn >= ndev) return ndev;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    const int first = dn >= 0 ? dn : 0;
+    const int last = dn >= 0 ? dn + 1 : ndev;
+    for (int i = first; i < last; ++i) {
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, i) == hipSuccess ? 0 : -1, "hipGetDeviceProperties");
+        std::printf("Device Index: %d\n  Device UUID: ", i);
+        for (int k = 0; k < 16; ++k) std::printf("%02x", (unsigned char)prop.uuid.bytes[k]);
+        if (gpuuid) std::memcpy(gpuuid, prop.uuid.bytes, 16);
+        std::printf("\n  Concurrent copy and kernel execution: \t %s with %d copy engine(s)\n",
+                    prop.asyncEngineCount > 0 ? "Yes" : "No", prop.asyncEngineCount);
+        if (streamCount) *streamCount = prop.asyncEngineCount > 0 ? prop.asyncEngineCount * 2 : 1;
+        std::printf("  Device name: %s\n", prop.name);
+        size_t free_b = 0, total_b = 0;
+        if (hipSetDevice(i) == hipSuccess && hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+            std::printf("GPU %d memory: free=%zu MB, total=%zu MB\n", i, free_b >> 20, total_b >> 20);
+    }
+    (void)hipSetDevice(prev);
+    return ndev;
+}
+
 mc::mc(int id, int _streamNum, unsigned char de, std::uintmax_t, bool _profiler)
     : devid(id), workerNum(0), hmcE(nullptr), s_n(nullptr), sz_burst(0), reSampleTimes(4),
       ctx(nullptr), streamNum(_streamNum == 0 ? 4 : _streamNum), debug(de), profiler(_profiler),

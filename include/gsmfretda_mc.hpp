@@ -28,6 +28,13 @@
 
 #include "gsmfretda_pda.h"
 
+// Replaces showGPUsInfo (src/mc.cu:26-79, declared at src/mc.hpp:31): prints
+// index, UUID, copy engines, name and memory of device dn (every device when
+// dn < 0), writes device dn's 16 UUID bytes to gpuuid and 2 x its copy-engine
+// count (1 without overlap) to streamCount when they are not null, and
+// returns the number of devices.
+int showGPUsInfo(int dn = -1, char* gpuuid = nullptr, int* streamCount = nullptr);
+
 class mc {
    public:
     mc(int devid, int _streamNum = 16, unsigned char debug = 0, std::uintmax_t hdf5size = 0,
