@@ -123,6 +123,9 @@ int load_common(pda_ctx* ctx, const int64_t* start, const int64_t* stop, const i
     const int64_t total = int64_t(ticks.size());
     int rc = sync_all(ctx);
     if (rc) return rc;
+    // captured population graphs hold the old buffers' addresses
+    for (auto& s : ctx->slots) drop_graph(s.pop);
+    ++ctx->data_gen;
     free_data(ctx);
     if (nbursts > 0) {
         HIP_TRY(hipMalloc(&ctx->d_bursts, sizeof(BurstDesc) * size_t(nbursts)));
@@ -345,6 +348,14 @@ int pda_set_photophysics(pda_ctx* ctx, int sid, float gamma, float dexdir, float
     if (!std::isfinite(gamma) || !std::isfinite(dexdir) || !std::isfinite(crosstalk) ||
         !(r0 > 0.0f) || !std::isfinite(r0))
         return fail(PDA_E_ARG, "photophysics constants must be finite, r0 > 0");
+    // The corrected count (gamma - Dch2Ach) f_id + (1 - DexDirAem) f_ia
+    // (src/mc.cu:143) is the binomial's trial count; with a negative
+    // coefficient it goes negative, where rk_binomial (src/binom.cuh:172-188)
+    // has no meaning (it returns n < 0 acceptors for p > 1/2).
+    if (!(gamma - crosstalk >= 0.0f) || !(1.0f - dexdir >= 0.0f))
+        return fail(PDA_E_ARG,
+                    "gamma %g < crosstalk %g or DexDirAem %g > 1: the corrected photon count "
+                    "would be negative", double(gamma), double(crosstalk), double(dexdir));
     s->ph.gamma = gamma;
     s->ph.dexdir = dexdir;
     s->ph.crosstalk = crosstalk;
@@ -390,9 +401,9 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
     a.clk_p = ctx->clk_p;
     a.bg_ad = ctx->bg_ad;
     a.bg_dd = ctx->bg_dd;
-    a.cap = ctx->cap;
     a.nstates = s->n;
-    const int pass = s->any_v ? 2 : 0;  // general pass for broadening, else fast + replay
+    // general pass for broadening, else the fast pass and a replay pass
+    const int pass = s->any_v ? kPassGeneral : kPassFast;
     const Launch L = plan_launch(ctx, pass, false, uint64_t(nunits));
     const uint32_t grid = L.grid;
     a.batch = L.batch;
@@ -427,7 +438,7 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
         outAcc = s->d_acc;
         outF = s->d_F;
     }
-    if (pass == 0 && size_t(nunits) > s->replay_cap) {
+    if (pass == kPassFast && size_t(nunits) > s->replay_cap) {
         HIP_TRY(hipStreamSynchronize(s->stream));
         (void)hipFree(s->d_replay);
         s->d_replay = nullptr;
@@ -441,9 +452,21 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
                         s->d_hist_acc, s->d_replay, s->d_replay_count, s->d_work, outE, outAcc,
                         outF, s->stream));
     HIP_TRY(hipEventRecord(s->ev_mid, s->stream));
-    HIP_TRY(launch_finish(ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, a, ctx->d_obs,
-                          s->d_hist_acc, s->d_replay, s->d_replay_count, s->d_work, s->d_result,
-                          outE, outAcc, outF, s->stream));
+    if (pass == kPassFast && nunits > 0) {
+        // the abandoned units, in the general form over the whole GPU; the
+        // grid is sized for the worst case and its unneeded workgroups exit
+        // at once
+        int per_cu = 0;
+        if (walk_occupancy(kPassReplay, &per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
+        const uint64_t want = (uint64_t(nunits) + kWave - 1) / kWave;
+        const uint32_t rgrid =
+            uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(want, uint64_t(ctx->num_cu) * per_cu)));
+        HIP_TRY(launch_walk(kPassReplay, rgrid, ctx->d_bursts, ctx->d_ticks, ctx->d_links,
+                            s->d_blob, a, s->d_hist_acc, s->d_replay, s->d_replay_count, s->d_work,
+                            outE, outAcc, outF, s->stream));
+    }
+    HIP_TRY(launch_finish(a, ctx->d_obs, s->d_hist_acc, s->d_replay_count, s->d_work, s->d_result,
+                          s->stream));
     HIP_TRY(hipEventRecord(s->ev1, s->stream));
     HIP_TRY(hipMemcpyAsync(s->h_result, s->d_result, sizeof(ResultHeader) + sizeof(int32_t) * size_t(a.B),
                            hipMemcpyDeviceToHost, s->stream));
@@ -494,6 +517,17 @@ int pda_get_result(pda_ctx* ctx, int sid, int32_t* hist, int nbins, double* chi2
     return PDA_OK;
 }
 
+int pda_run_stats(pda_ctx* ctx, int sid, int64_t* units, int32_t* replays) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    if (!s->ran) return fail(PDA_E_STATE, "stream %d has not run", sid);
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (units) *units = s->last_units;
+    if (replays) *replays = s->last_units > 0 ? s->h_result->replays : 0;
+    return PDA_OK;
+}
+
 int pda_get_samples(pda_ctx* ctx, int sid, float* E, int32_t* acc, float* F, int64_t n) {
     Slot* s = slot_of(ctx, sid);
     if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
@@ -535,23 +569,62 @@ int pda_last_pass_ms(pda_ctx* ctx, int sid, float* walk_ms, float* finish_ms) {
     return PDA_OK;
 }
 
-int pda_set_tuning(pda_ctx* ctx, int tiles_per_chunk, int lds_photons) {
-    if (!ctx) return fail(PDA_E_ARG, "null context");
-    if (tiles_per_chunk < 0 || tiles_per_chunk > 64)
-        return fail(PDA_E_ARG, "tiles_per_chunk %d outside [0, 64]", tiles_per_chunk);
-    if (lds_photons < 0 || lds_photons > 256)
-        return fail(PDA_E_ARG, "lds_photons %d outside [0, 256]", lds_photons);
-    int rc = sync_all(ctx);
-    if (rc) return rc;
-    ctx->shape.batch_tiles = uint32_t(tiles_per_chunk);
-    ctx->cap = uint32_t(lds_photons);
-    return PDA_OK;
-}
-
-int pda_get_tuning(const pda_ctx* ctx, int* tiles_per_chunk, int* lds_photons) {
-    if (!ctx) return fail(PDA_E_ARG, "null context");
-    if (tiles_per_chunk) *tiles_per_chunk = int(ctx->shape.batch_tiles);
-    if (lds_photons) *lds_photons = int(ctx->cap);
+int pda_debug_microbench(pda_ctx* ctx, int op, int iters, float param, float* ms,
+                         int64_t* events, int64_t* inner) {
+    if (!ctx || op < 0 || op > 5 || iters < 1 || iters > (1 << 24))
+        return fail(PDA_E_ARG, "bad argument");
+    if ((op == 4) && !(param >= 0.0f && param < float(kBinomTable)))
+        return fail(PDA_E_ARG, "binomial microbenchmark: n = %g outside [0, %d)", double(param),
+                    kBinomTable);
+    if ((op == 3) && !(param >= 0.0f && param <= 64.0f))
+        return fail(PDA_E_ARG, "Poisson microbenchmark: mean %g outside [0, 64]", double(param));
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint32_t grid = 0, threads = 0;
+    HIP_TRY(microbench_shape(op, &grid, &threads));
+    const size_t lanes = size_t(grid) * threads;
+    // q^m and the inversion bound of Binomial(m, 0.3), m < kBinomTable
+    std::vector<double> qn(kBinomTable);
+    std::vector<int32_t> bnd(kBinomTable);
+    for (int m = 0; m < kBinomTable; ++m) {
+        qn[size_t(m)] = std::pow(0.7, double(m));
+        const double mp = 0.3 * m;
+        bnd[size_t(m)] = int32_t(std::min(double(m), std::floor(mp + 10.0 * std::sqrt(mp * 0.7 + 1.0))));
+    }
+    uint64_t* d_out = nullptr;
+    double* d_qn = nullptr;
+    int32_t* d_bnd = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<uint64_t> h(lanes * 3);
+    hipError_t e = hipMalloc(&d_out, sizeof(uint64_t) * 3 * lanes);
+    if (e == hipSuccess) e = hipMalloc(&d_qn, sizeof(double) * kBinomTable);
+    if (e == hipSuccess) e = hipMalloc(&d_bnd, sizeof(int32_t) * kBinomTable);
+    if (e == hipSuccess) e = hipMemcpy(d_qn, qn.data(), sizeof(double) * kBinomTable, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_bnd, bnd.data(), sizeof(int32_t) * kBinomTable, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // one untimed launch, then the timed one
+    if (e == hipSuccess) e = launch_microbench(op, grid, iters, param, d_qn, d_bnd, d_out, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = launch_microbench(op, grid, iters, param, d_qn, d_bnd, d_out, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), d_out, sizeof(uint64_t) * 3 * lanes, hipMemcpyDeviceToHost);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d_out);
+    (void)hipFree(d_qn);
+    (void)hipFree(d_bnd);
+    if (e != hipSuccess) return fail(PDA_E_HIP, "microbenchmark %d: %s", op, hipGetErrorString(e));
+    uint64_t ev = 0, in = 0;
+    for (size_t i = 0; i < lanes; ++i) {
+        ev += h[3 * i + 1];
+        in += h[3 * i + 2];
+    }
+    if (ms) *ms = t;
+    if (events) *events = int64_t(ev);
+    if (inner) *inner = int64_t(in);
     return PDA_OK;
 }
 

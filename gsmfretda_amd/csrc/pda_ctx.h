@@ -43,7 +43,12 @@ struct PopGraphKey {
     int64_t b0 = 0, b1 = 0;
     uint32_t grid = 0, batch = 0, queues = 0, replay_cap = 0;
     float clk_p = 0.f, bg_ad = 0.f, bg_dd = 0.f;
-    const void* data = nullptr;
+    // the resident bursts the nodes read: their buffers and the load that
+    // produced them (a reload may return a freed buffer's address)
+    const void* bursts = nullptr;
+    const void* ticks = nullptr;
+    const void* links = nullptr;
+    uint64_t data_gen = 0;
     const void* obs = nullptr;
     bool operator==(const PopGraphKey& o) const;
 };
@@ -72,7 +77,8 @@ struct Population {
     int captures = 0;
     bool ran = false;
     bool in_flight = false;
-    int last_B = 0, last_R = 0;
+    int last_B = 0, last_R = 0, last_left_pad = 0;
+    int64_t last_b0 = 0, last_b1 = 0;  // burst range of the last run
     int64_t last_units = 0;          // per set
     uint32_t last_seed = 0, last_base = 0, last_stride = 0;
     bool last_graph = false;
@@ -154,8 +160,6 @@ struct Slot {
     RbState rb;
 };
 
-constexpr uint32_t kDefaultCap = 256;  // LDS photon capacity of a replayed burst
-
 }  // namespace pda
 
 struct pda_ctx {
@@ -163,13 +167,13 @@ struct pda_ctx {
     int num_cu = 256;
     std::vector<pda::Slot> slots;
     int64_t nbursts = 0, nphotons = 0;
+    uint64_t data_gen = 0;          // bumped by every burst load
     pda::BurstDesc* d_bursts = nullptr;
     uint32_t* d_ticks = nullptr;
     uint2* d_links = nullptr;
     float clk_p = 0.f, bg_ad = 0.f, bg_dd = 0.f;
     int B = 0, left_pad = 2;
     float* d_obs = nullptr;
-    uint32_t cap = pda::kDefaultCap;
     pda::LaunchShape shape;
     std::vector<uint32_t> h_P;      // photons per burst (host copy)
     double* d_lf = nullptr;         // log k! for the RB form
@@ -182,7 +186,12 @@ Slot* slot_of(pda_ctx* ctx, int sid);
 int sync_all(pda_ctx* ctx);
 void range_of(const pda_ctx* ctx, const Slot& s, int64_t* b0, int64_t* b1);
 void free_population(Population& p);
+void drop_graph(Population& p);
 void free_rb(RbState& r);
+
+// Walk passes (pda_kernels.hip): fast form, general form over the replay
+// list, general form over every unit.
+constexpr int kPassFast = 0, kPassReplay = 1, kPassGeneral = 2;
 
 // Grid and batch of a walk launch over `units` work units.
 struct Launch {
@@ -198,11 +207,9 @@ hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const u
                        int32_t* hist_acc, uint32_t* replay, uint32_t* replay_count,
                        uint32_t* work, float* outE, int32_t* outAcc, float* outF,
                        hipStream_t stream);
-hipError_t launch_finish(const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
-                         const ModelBlob* blob, const RunArgs& a, const float* obs,
-                         int32_t* hist_acc, const uint32_t* replay, uint32_t* replay_count,
-                         uint32_t* work, ResultHeader* result, float* outE, int32_t* outAcc,
-                         float* outF, hipStream_t stream);
+hipError_t launch_finish(const RunArgs& a, const float* obs, int32_t* hist_acc,
+                         uint32_t* replay_count, uint32_t* work, ResultHeader* result,
+                         hipStream_t stream);
 hipError_t launch_pop_walk(int pass, uint32_t grid, const BurstDesc* bursts,
                            const uint32_t* ticks, const uint2* links, const PopControl* ctl,
                            const ModelBlob* blobs, const RunArgs& a, int32_t* hist_acc,
@@ -217,6 +224,11 @@ hipError_t launch_pop_finish(const ModelBlob* blobs, const RunArgs& a, const flo
                              PopStatus* status, SetResult* results, int32_t* hists,
                              hipStream_t stream);
 hipError_t launch_primitive(int op, const void* in, void* out, int64_t n, hipStream_t stream);
+// event microbenchmarks of the cost model: grid at full occupancy, launch
+// (out: 3 uint64 per lane -- checksum, events, inner iterations)
+hipError_t microbench_shape(int op, uint32_t* grid, uint32_t* threads);
+hipError_t launch_microbench(int op, uint32_t grid, int iters, float param, const double* qn,
+                             const int32_t* bound, uint64_t* out, hipStream_t stream);
 struct RbModel;
 // which: 0/1 RB walk without/with scores; 2 forward, 3 adjoint, 4 Jacobian,
 // 5 forward in place (rb_pmf_kernel)

@@ -14,8 +14,9 @@
 //     src/gen_rand.cuh:7-25);
 //   * the common case runs in a lean form (at most 72 VGPRs); the rare units
 //     that need BTPE draws, inversion restarts or large background means are
-//     handed to finish_kernel, which replays them in full, reduces the LDS
-//     histograms of all workgroups and computes chi2 on the device.
+//     listed and replayed in full by the same kernel in its general form
+//     over the whole grid; finish_kernel then reduces the LDS histograms of
+//     all workgroups and computes chi2 on the device.
 #include <hip/hip_runtime.h>
 
 #include "pda_math.h"
@@ -27,8 +28,14 @@ namespace pda {
 // Which form a walk launch runs.
 enum Pass : int {
     kPassFast = 0,     // fast form; abandoned units go to the replay list
+    kPassReplay = 1,   // general form over the units of the replay list
     kPassGeneral = 2,  // general form for every unit (models with broadening)
 };
+
+// Work counters of a replay pass: one per XCD, on the counter lines after the
+// kMaxQueues lines of the walk (kAuxQueues of them).
+constexpr uint32_t kReplayQueues = 8;
+static_assert(kReplayQueues <= kAuxQueues, "replay counters live on the auxiliary lines");
 
 // The per-unit outputs outE / outAcc / outF are requested together (all
 // null or all set), so one test covers the three stores.
@@ -46,11 +53,14 @@ __device__ __forceinline__ void tally(int32_t* s_hist, const RunArgs& a, const W
     }
 }
 
-// The walk over every unit of the run.  Each one-wave workgroup keeps an LDS
-// histogram and adds it to its XCD's accumulator row at the end (workgroups
-// are dispatched to the 8 XCDs round-robin, so the atomics of a launch spread
-// over 8 x B addresses).  No completion ticket and no fence: the finish
-// kernel, launched behind on the same stream, reads the rows.
+// The walk over every unit of the run (kPassFast, kPassGeneral), or over the
+// units the fast pass abandoned (kPassReplay, launched right after it on the
+// same stream, so the replay list and its count are final).  Each one-wave
+// workgroup keeps an LDS histogram and adds it to its XCD's accumulator row
+// at the end (workgroups are dispatched to the 8 XCDs round-robin, so the
+// atomics of a launch spread over 8 x B addresses).  No completion ticket
+// and no fence: the finish kernel, launched behind on the same stream, reads
+// the rows.
 // The fast form is held to 72 VGPRs (7 waves per SIMD, no spills to
 // scratch): its window loop waits on dependent loads, and one more resident
 // wave per SIMD hides more of that latency than the register it gives up
@@ -65,6 +75,23 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     int32_t* __restrict__ hist_acc, uint32_t* __restrict__ replay,
     uint32_t* __restrict__ replay_count, uint32_t* __restrict__ work, float* __restrict__ outE,
     int32_t* __restrict__ outAcc, float* __restrict__ outF) {
+    constexpr bool kFastForm = kPass == kPassFast;
+    constexpr bool kReplay = kPass == kPassReplay;
+    // units of this pass, units per grab, queues and their counters
+    uint32_t nunits = a.nunits, batch = a.batch, queues = a.queues;
+    uint32_t* counters = work;
+    if (kReplay) {
+        const uint32_t c = *replay_count;
+        nunits = c < a.replay_cap ? c : a.replay_cap;
+        // as many workgroups as the list needs a wave for, each queue served
+        // by at least one of them
+        const uint32_t need = (nunits + kWave - 1u) / kWave;
+        const uint32_t alive = need < gridDim.x ? need : gridDim.x;
+        if (blockIdx.x >= alive) return;  // uniform over the workgroup
+        batch = kWave;
+        queues = alive < kReplayQueues ? alive : kReplayQueues;
+        counters = work + size_t(kMaxQueues) * kCounterStride;
+    }
     __shared__ Model sM;
     __shared__ int32_t s_hist[kMaxBins];
     const uint32_t lane = threadIdx.x;
@@ -88,12 +115,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     // one XCD and its bursts stay in that XCD's L2.  Batches are small
     // (about six grabs per wave): the wave that grabs last still walks a
     // whole batch, so the batch sets the grid's tail.  Photons are read
-    // through L1/L2.
-    constexpr bool kFastForm = kPass == kPassFast;
+    // through L1/L2.  A replay pass takes its units the same way from the
+    // replay list.
     const BinomTable* tab = &blob->tab;
-    const uint32_t queue = blockIdx.x % a.queues;
-    const uint32_t q_begin = uint32_t(uint64_t(a.nunits) * queue / a.queues);
-    const uint32_t q_end = uint32_t(uint64_t(a.nunits) * (queue + 1u) / a.queues);
+    const uint32_t queue = blockIdx.x % queues;
+    const uint32_t q_begin = uint32_t(uint64_t(nunits) * queue / queues);
+    const uint32_t q_end = uint32_t(uint64_t(nunits) * (queue + 1u) / queues);
     uint32_t pool_next = 0, pool_end = 0;  // the wave's current batch
     bool busy = false;
     uint32_t u = 0;
@@ -108,20 +135,20 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
             if (want == 0) break;
             if (pool_next >= pool_end) {
                 uint32_t off = 0;
-                if (lane == 0) off = atomicAdd(&work[queue * kCounterStride], a.batch);
+                if (lane == 0) off = atomicAdd(&counters[queue * kCounterStride], batch);
                 off = __shfl(off, 0);
                 if (off >= q_end - q_begin) {
                     drained = true;
                     break;
                 }
                 pool_next = q_begin + off;
-                pool_end = pool_next + a.batch < q_end ? pool_next + a.batch : q_end;
+                pool_end = pool_next + batch < q_end ? pool_next + batch : q_end;
             }
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                 uint32_t(want >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(want), 0u));
             const uint32_t avail = pool_end - pool_next;
             if (need && rank < avail) {
-                u = pool_next + rank;
+                u = kReplay ? replay[pool_next + rank] : pool_next + rank;
                 const uint32_t bl = u / a.R;
                 const uint32_t b = a.begin + bl;
                 d = bursts[b];
@@ -161,67 +188,23 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
 }
 
 constexpr uint32_t kFinishWaves = 16;
-constexpr uint32_t kReplayStage = 256;  // photons of a replayed burst staged in LDS
 
-// Finishes a run in one workgroup of 16 waves: replays the units the fast
-// pass abandoned (general form, one unit per wave at a time, lane 0 walking
-// the burst staged in LDS), reduces the 8 accumulator rows (leaving them zero
-// for the next run), computes the reduced chi2 of
+// Finishes a run in one workgroup of 16 waves: reduces the 8 accumulator
+// rows (leaving them zero for the next run), computes the reduced chi2 of
 // src/streamWorker.cpp:194-206 in the reference's float accumulation and in
-// FP64, and resets the run's counters.
+// FP64, and resets the run's counters (walk and replay queues, replay count).
 __global__ __launch_bounds__(kWave * kFinishWaves) void finish_kernel(
-    const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
-    const uint2* __restrict__ links, const ModelBlob* __restrict__ blob, const RunArgs a,
-    const float* __restrict__ obs, int32_t* __restrict__ hist_acc,
-    const uint32_t* __restrict__ replay, uint32_t* __restrict__ replay_count,
-    uint32_t* __restrict__ work, ResultHeader* __restrict__ result, float* __restrict__ outE,
-    int32_t* __restrict__ outAcc, float* __restrict__ outF) {
-    __shared__ Model sM;
+    const RunArgs a, const float* __restrict__ obs, int32_t* __restrict__ hist_acc,
+    uint32_t* __restrict__ replay_count, uint32_t* __restrict__ work,
+    ResultHeader* __restrict__ result) {
     __shared__ int32_t s_hist[kMaxBins];
-    __shared__ uint32_t s_tick[kFinishWaves][kReplayStage];
-    __shared__ uint2 s_link[kFinishWaves][kReplayStage];
     __shared__ double s_t64[kMaxBins];
     __shared__ double s_t32[kMaxBins];
     const uint32_t tid = threadIdx.x;
-    const uint32_t wave = tid / kWave, lane = tid % kWave;
-    {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(&blob->m);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(&sM);
-        for (uint32_t i = tid; i < sizeof(Model) / 4; i += blockDim.x) dst[i] = src[i];
-        for (uint32_t i = tid; i < uint32_t(a.B); i += blockDim.x) s_hist[i] = 0;
-    }
-    __syncthreads();
-    const uint32_t count = *replay_count < a.replay_cap ? *replay_count : a.replay_cap;
-    const uint32_t stage = a.cap < kReplayStage ? a.cap : kReplayStage;
-    for (uint32_t i = wave; i < count; i += kFinishWaves) {
-        const uint32_t u = replay[i];
-        const uint32_t bl = u / a.R;
-        const uint32_t b = a.begin + bl;
-        const BurstDesc d = bursts[b];
-        const bool staged = d.P <= stage;
-        if (staged) {
-            for (uint32_t k = lane; k < d.P; k += kWave) {
-                s_tick[wave][k] = ticks[d.off + k];
-                s_link[wave][k] = links[d.off + k];
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane == 0) {
-            const UnitKey key{a.seed, a.set, u - bl * a.R, b};
-            const WalkOut o = walk_unit<false>(sM, nullptr, key, d.P, d.stop_rel, d.T,
-                                               staged ? s_tick[wave] : ticks + d.off,
-                                               staged ? s_link[wave] : links + d.off, a);
-            tally(s_hist, a, o, u, outE, outAcc, outF);
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
-    __syncthreads();
     int32_t* rhist = reinterpret_cast<int32_t*>(result + 1);
     const float Rf = float(a.R);
     for (uint32_t b = tid; b < uint32_t(a.B); b += blockDim.x) {
-        int32_t Mb = s_hist[b];
+        int32_t Mb = 0;
         for (uint32_t x = 0; x < 8u; ++x) Mb += atomicExch(&hist_acc[size_t(x) * kMaxBins + b], 0);
         s_hist[b] = Mb;
         rhist[b] = Mb;
@@ -258,10 +241,9 @@ __global__ __launch_bounds__(kWave * kFinishWaves) void finish_kernel(
         result->effN = effN;
         result->units = int64_t(a.nunits);
         result->B = a.B;
-        result->pad = 0;
-        atomicExch(replay_count, 0u);
+        result->replays = int32_t(atomicExch(replay_count, 0u));
     }
-    if (tid < kMaxQueues) atomicExch(&work[tid * kCounterStride], 0u);
+    if (tid < kMaxQueues + kReplayQueues) atomicExch(&work[tid * kCounterStride], 0u);
 }
 
 // ---- population runs ---------------------------------------------------------
@@ -554,12 +536,167 @@ __global__ void primitive_kernel(int op, const void* __restrict__ in, void* __re
     }
 }
 
+// ---- microbenchmarks of the algorithm's events ---------------------------------
+//
+// The weights of the algorithmic cost model: each lane repeats one event of
+// the walk -- a Philox block, the scalar work of a dwell window, a
+// binary-search step, a background draw, a capped-Poisson iteration, a
+// binomial draw, a binomial inversion iteration, the per-unit start and
+// tally -- with a loop-carried dependence, counts the events it performed
+// and writes a checksum.  Launched with 256-thread workgroups at full
+// occupancy, the loops are bound by VALU issue, so the time per event times
+// the device's 32-bit lane-operation rate is the event's weight in
+// lane-issue-cycles.  Inputs vary with a 32-bit LCG step per event (part of
+// the measured cost, standing in for reading a Philox word).
+enum MicroOp : int {
+    kMbPhilox = 0,   // one Philox4x32-10 block
+    kMbWindow = 1,   // a dwell window's scalar work (next state, dwell, window end,
+                     // corrected count, its rounding and sums)
+    kMbSearch = 2,   // one step of the window-end binary search (LDS-resident ticks)
+    kMbPoisson = 3,  // one capped-Poisson draw (and its inversion iterations)
+    kMbBinom = 4,    // one binomial inversion draw from the q^m table (and its
+                     // iterations)
+    kMbUnit = 5,     // a unit's initial state draw and its tally into the histogram
+};
+constexpr int kMicroOps = 6;
+constexpr uint32_t kMbThreads = 256;
+
+__device__ __forceinline__ uint32_t lcg(uint32_t x) { return x * 1664525u + 1013904223u; }
+
+template <int kOp>
+__global__ __launch_bounds__(kMbThreads) void microbench_kernel(int iters, float param,
+                                                                const double* __restrict__ qn,
+                                                                const int32_t* __restrict__ bound,
+                                                                uint64_t* __restrict__ out) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t h = gid * 0x9E3779B9u + 0x7F4A7C15u;
+    uint64_t events = 0, inner = 0;
+    __shared__ uint32_t s_tick[128];
+    __shared__ int32_t s_hist[64];
+    if (kOp == kMbSearch || kOp == kMbUnit) {
+        for (uint32_t i = threadIdx.x; i < 128u; i += blockDim.x) s_tick[i] = i * 97u;
+        for (uint32_t i = threadIdx.x; i < 64u; i += blockDim.x) s_hist[i] = 0;
+        __syncthreads();
+    }
+    if (kOp == kMbPhilox) {
+        Block x{{h, gid, 0u, 0u}};
+        for (int i = 0; i < iters; ++i)
+            x = philox10(uint32_t(i), x.w[0], gid, x.w[1], h, x.w[2] ^ x.w[3]);
+        h = x.w[0] ^ x.w[1] ^ x.w[2] ^ x.w[3];
+        events = uint64_t(iters);
+    } else if (kOp == kMbWindow) {
+        const float cum[2] = {0.4f, 1.0f};
+        const float krate[3] = {900.0f, 3000.0f, 5500.0f};
+        const float T = 1e-3f, clk = 12.5e-9f, gmx = 0.95f, omdd = 1.0f;
+        float tsum = 0.0f, F = 0.0f, acc = 0.0f;
+        uint32_t w0 = 0;
+        int st = 0;
+        for (int i = 0; i < iters; ++i) {
+            h = lcg(h);
+            const int jj = first_above(cum, 2, uniform24(h));
+            const int j = jj >= st ? jj + 1 : jj;
+            h = lcg(h);
+            const float kk = j == 0 ? krate[0] : (j == 1 ? krate[1] : krate[2]);
+            const float tau = flog(uniform24(h)) / (-kk);
+            tsum = tsum + tau;
+            const bool end = !(tsum < T);
+            const float wf = float(w0) + tsum / clk;
+            const uint32_t w1 = end ? 80000u : uint32_t(wf);
+            const uint32_t cnt = (w1 - w0) & 31u;
+            const float nd = (h & 1u) ? float(cnt) : 0.0f;
+            const float na = (h & 1u) ? 0.0f : float(cnt);
+            const float fv = gmx * nd + omdd * na;
+            const float m = floorf(fv + 0.5f);
+            F += fv;
+            acc += m;
+            w0 = end ? 0u : w1;
+            tsum = end ? 0.0f : tsum;
+            st = j;
+        }
+        h ^= fbits(F) ^ fbits(acc);
+        events = uint64_t(iters);
+    } else if (kOp == kMbSearch) {
+        uint32_t lo = 0, hi = 128, key = h & 0x3FFFu;
+        for (int i = 0; i < iters; ++i) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_tick[mid] < key) lo = mid + 1u;
+            else hi = mid;
+            const bool done = lo >= hi;
+            h = done ? lcg(h) : h;
+            key = done ? (h & 0x3FFFu) : key;
+            lo = done ? 0u : lo;
+            hi = done ? 128u : hi;
+        }
+        h ^= lo;
+        events = uint64_t(iters);
+    } else if (kOp == kMbPoisson) {
+        uint64_t it = 0;
+        for (int i = 0; i < iters; ++i) {
+            h = lcg(h);
+            const float u1 = uniform24(h);
+            h = lcg(h);
+            const float k = background_single(param, 1e6f, u1, uniform24(h));
+            it += uint64_t(k);
+            h ^= fbits(k);
+        }
+        events = uint64_t(iters);
+        inner = it;
+    } else if (kOp == kMbBinom) {
+        const int n = int(param);
+        const double pp = 0.3, q = 0.7;
+        uint64_t it = 0;
+        for (int i = 0; i < iters; ++i) {
+            h = lcg(h);
+            const uint32_t lo32 = h;
+            h = lcg(h);
+            // the inversion of binomial_table (pda_walk.h) on a q^m table
+            double px = qn[n];
+            const int bnd = bound[n];
+            double U = uniform53(h, lo32);
+            int X = 0;
+            while (U > px) {
+                ++X;
+                if (X > bnd) break;
+                U -= px;
+                px = (double(n - X + 1) * pp * px) / (double(X) * q);
+            }
+            it += uint64_t(X);
+            h ^= uint32_t(X);
+        }
+        events = uint64_t(iters);
+        inner = it;
+    } else if (kOp == kMbUnit) {
+        const float p0cum[3] = {0.2f, 0.7f, 1.0f};
+        float acc = 3.0f, F = 7.0f;
+        for (int i = 0; i < iters; ++i) {
+            h = lcg(h);
+            const int s0 = first_above(p0cum, 3, uniform24(h));
+            acc += float(s0);
+            F += float(h & 7u);
+            const float E = F > 0.0f ? acc / F : 0.0f;
+            if (E >= 0.0f && E < 1.0f) {
+                const int bin = int(floor(double(E) * 50.0));
+                if (bin < 50) atomicAdd(&s_hist[bin], 1);
+            }
+            acc = E * 3.0f;
+        }
+        h ^= fbits(acc);
+        events = uint64_t(iters);
+    }
+    out[3u * gid] = h;
+    out[3u * gid + 1u] = events;
+    out[3u * gid + 2u] = inner;
+}
+
 // ---- launchers (called by the C-ABI layer) ---------------------------------
 
 hipError_t walk_occupancy(int pass, int* blocks_per_cu) {
     if (pass == kPassFast)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel<kPassFast>,
                                                             kWave, 0);
+    if (pass == kPassReplay)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                            walk_kernel<kPassReplay>, kWave, 0);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_kernel<kPassGeneral>,
                                                         kWave, 0);
 }
@@ -573,6 +710,10 @@ hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const u
         hipLaunchKernelGGL(walk_kernel<kPassFast>, dim3(grid), dim3(kWave), 0, stream, bursts,
                            ticks, links, blob, a, hist_acc, replay, replay_count, work, outE,
                            outAcc, outF);
+    else if (pass == kPassReplay)
+        hipLaunchKernelGGL(walk_kernel<kPassReplay>, dim3(grid), dim3(kWave), 0, stream, bursts,
+                           ticks, links, blob, a, hist_acc, replay, replay_count, work, outE,
+                           outAcc, outF);
     else
         hipLaunchKernelGGL(walk_kernel<kPassGeneral>, dim3(grid), dim3(kWave), 0, stream, bursts,
                            ticks, links, blob, a, hist_acc, replay, replay_count, work, outE,
@@ -580,14 +721,11 @@ hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const u
     return hipGetLastError();
 }
 
-hipError_t launch_finish(const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
-                         const ModelBlob* blob, const RunArgs& a, const float* obs,
-                         int32_t* hist_acc, const uint32_t* replay, uint32_t* replay_count,
-                         uint32_t* work, ResultHeader* result, float* outE, int32_t* outAcc,
-                         float* outF, hipStream_t stream) {
-    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kWave * kFinishWaves), 0, stream, bursts,
-                       ticks, links, blob, a, obs, hist_acc, replay, replay_count, work, result,
-                       outE, outAcc, outF);
+hipError_t launch_finish(const RunArgs& a, const float* obs, int32_t* hist_acc,
+                         uint32_t* replay_count, uint32_t* work, ResultHeader* result,
+                         hipStream_t stream) {
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kWave * kFinishWaves), 0, stream, a, obs,
+                       hist_acc, replay_count, work, result);
     return hipGetLastError();
 }
 
@@ -639,6 +777,44 @@ hipError_t launch_primitive(int op, const void* in, void* out, int64_t n, hipStr
     const int64_t blocks = (n + 255) / 256;
     hipLaunchKernelGGL(primitive_kernel, dim3(uint32_t(blocks)), dim3(256), 0, stream, op, in, out,
                        n);
+    return hipGetLastError();
+}
+
+hipError_t microbench_shape(int op, uint32_t* grid, uint32_t* threads) {
+    int per_cu = 0;
+    hipError_t e = hipSuccess;
+    switch (op) {
+        case kMbPhilox: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, microbench_kernel<kMbPhilox>, kMbThreads, 0); break;
+        case kMbWindow: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, microbench_kernel<kMbWindow>, kMbThreads, 0); break;
+        case kMbSearch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, microbench_kernel<kMbSearch>, kMbThreads, 0); break;
+        case kMbPoisson: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, microbench_kernel<kMbPoisson>, kMbThreads, 0); break;
+        case kMbBinom: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, microbench_kernel<kMbBinom>, kMbThreads, 0); break;
+        case kMbUnit: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, microbench_kernel<kMbUnit>, kMbThreads, 0); break;
+        default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    int dev = 0, cus = 0;
+    e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    *grid = uint32_t(cus) * uint32_t(per_cu);
+    *threads = kMbThreads;
+    return hipSuccess;
+}
+
+hipError_t launch_microbench(int op, uint32_t grid, int iters, float param, const double* qn,
+                             const int32_t* bound, uint64_t* out, hipStream_t stream) {
+    const dim3 g(grid), b(kMbThreads);
+    switch (op) {
+        case kMbPhilox: hipLaunchKernelGGL(microbench_kernel<kMbPhilox>, g, b, 0, stream, iters, param, qn, bound, out); break;
+        case kMbWindow: hipLaunchKernelGGL(microbench_kernel<kMbWindow>, g, b, 0, stream, iters, param, qn, bound, out); break;
+        case kMbSearch: hipLaunchKernelGGL(microbench_kernel<kMbSearch>, g, b, 0, stream, iters, param, qn, bound, out); break;
+        case kMbPoisson: hipLaunchKernelGGL(microbench_kernel<kMbPoisson>, g, b, 0, stream, iters, param, qn, bound, out); break;
+        case kMbBinom: hipLaunchKernelGGL(microbench_kernel<kMbBinom>, g, b, 0, stream, iters, param, qn, bound, out); break;
+        case kMbUnit: hipLaunchKernelGGL(microbench_kernel<kMbUnit>, g, b, 0, stream, iters, param, qn, bound, out); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -32,7 +32,8 @@ bool PopGraphKey::operator==(const PopGraphKey& o) const {
     return nsets == o.nsets && R == o.R && B == o.B && left_pad == o.left_pad && pass == o.pass &&
            bg_on == o.bg_on && b0 == o.b0 && b1 == o.b1 && grid == o.grid && batch == o.batch &&
            queues == o.queues && replay_cap == o.replay_cap && clk_p == o.clk_p &&
-           bg_ad == o.bg_ad && bg_dd == o.bg_dd && data == o.data && obs == o.obs;
+           bg_ad == o.bg_ad && bg_dd == o.bg_dd && bursts == o.bursts && ticks == o.ticks &&
+           links == o.links && data_gen == o.data_gen && obs == o.obs;
 }
 
 void drop_graph(Population& p) {
@@ -194,7 +195,6 @@ int run_population(pda_ctx* ctx, Slot& s, int R, uint32_t seed, uint32_t set_bas
     a.clk_p = ctx->clk_p;
     a.bg_ad = ctx->bg_ad;
     a.bg_dd = ctx->bg_dd;
-    a.cap = ctx->cap;
     a.nstates = 0;
     a.replay_cap = uint32_t(std::min<size_t>(p.replay_cap, 0xFFFFFFFFu));
     a.nsets = uint32_t(p.nsets);
@@ -227,7 +227,10 @@ int run_population(pda_ctx* ctx, Slot& s, int R, uint32_t seed, uint32_t set_bas
         k.clk_p = a.clk_p;
         k.bg_ad = a.bg_ad;
         k.bg_dd = a.bg_dd;
-        k.data = ctx->d_ticks;
+        k.bursts = ctx->d_bursts;
+        k.ticks = ctx->d_ticks;
+        k.links = ctx->d_links;
+        k.data_gen = ctx->data_gen;
         k.obs = ctx->d_obs;
         if (!p.exec || !(k == p.key)) {
             drop_graph(p);
@@ -259,6 +262,9 @@ int run_population(pda_ctx* ctx, Slot& s, int R, uint32_t seed, uint32_t set_bas
     p.in_flight = true;
     p.last_B = a.B;
     p.last_R = R;
+    p.last_left_pad = a.left_pad;
+    p.last_b0 = b0;
+    p.last_b1 = b1;
     p.last_units = units;
     p.last_seed = seed;
     p.last_base = set_base;
@@ -350,7 +356,15 @@ int pda_get_population_result(pda_ctx* ctx, int sid, int32_t* hist, int nbins, d
     const PopStatus* st = reinterpret_cast<const PopStatus*>(p.h_out);
     if (st->overflow) {
         // more abandoned units than the replay list holds: evaluate again in
-        // the general form, which abandons nothing (same keyed draws)
+        // the general form, which abandons nothing (same keyed draws) -- over
+        // the same bursts and bins as the run being repeated
+        int64_t b0, b1;
+        range_of(ctx, *s, &b0, &b1);
+        if (b0 != p.last_b0 || b1 != p.last_b1 || ctx->B != p.last_B ||
+            ctx->left_pad != p.last_left_pad)
+            return fail(PDA_E_STATE,
+                        "stream %d: the population run overflowed its replay list and the burst "
+                        "range or bins changed since; run it again", sid);
         int rc = run_population(ctx, *s, p.last_R, p.last_seed, p.last_base, p.last_stride,
                                 PDA_RUN_NO_GRAPH, true);
         if (rc) return rc;

@@ -259,7 +259,6 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     a.clk_p = ctx->clk_p;
     a.bg_ad = ctx->bg_ad;
     a.bg_dd = ctx->bg_dd;
-    a.cap = ctx->cap;
     a.nstates = s->n;
     a.nsets = 1;
     // walk grid: persistent, batches of about six grabs per wave

@@ -87,7 +87,6 @@ struct RunArgs {
     uint32_t queues;     // work counters in use, 1..kMaxQueues
     int32_t B, left_pad, bg_on;
     float clk_p, bg_ad, bg_dd;
-    uint32_t cap;        // LDS photon capacity for a replayed burst
     int32_t nstates;     // for the chi2 degrees of freedom
     uint32_t replay_cap; // capacity of the replay list
     uint32_t nsets;      // population runs: parameter sets
@@ -101,7 +100,7 @@ struct ResultHeader {
     int32_t effN;
     int64_t units;
     int32_t B;
-    int32_t pad;
+    int32_t replays;     // units the fast pass abandoned to the replay pass
 };
 
 // Population runs: random-stream keys of the sets, uploaded with the models

@@ -35,11 +35,11 @@ EXPORTS = (
     "pda_stream", "pda_load_bursts", "pda_load_bursts_tagged", "pda_num_bursts",
     "pda_set_observed", "pda_set_observed_hist", "pda_set_params", "pda_set_photophysics",
     "pda_set_burst_range", "pda_run", "pda_query", "pda_synchronize", "pda_get_result",
-    "pda_get_samples", "pda_samples_host", "pda_last_run_ms", "pda_last_pass_ms", "pda_set_tuning",
-    "pda_get_tuning", "pda_set_launch_shape", "pda_get_launch_shape", "pda_autotune",
+    "pda_run_stats", "pda_get_samples", "pda_samples_host", "pda_last_run_ms", "pda_last_pass_ms",
+    "pda_set_launch_shape", "pda_get_launch_shape", "pda_autotune",
     "pda_set_population", "pda_run_population", "pda_get_population_result",
     "pda_population_stats", "pda_rb_forward", "pda_rb_get_result", "pda_rb_adjoint",
-    "pda_rb_jacobian", "pda_rb_stats", "pda_debug_primitive",
+    "pda_rb_jacobian", "pda_rb_stats", "pda_debug_primitive", "pda_debug_microbench",
 )
 
 
@@ -92,13 +92,17 @@ def load_library(path=LIB_PATH):
     L.pda_get_result.argtypes = [_vp, ctypes.c_int, _i32p, ctypes.c_int, _f64p, _f32p,
                                  ctypes.POINTER(ctypes.c_int)]
     L.pda_get_samples.argtypes = [_vp, ctypes.c_int, _f32p, _i32p, _f32p, ctypes.c_int64]
+    L.pda_run_stats.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_int32)]
     L.pda_samples_host.argtypes = [_vp, ctypes.c_int]
     L.pda_samples_host.restype = _f32p
     L.pda_last_run_ms.argtypes = [_vp, ctypes.c_int, _f32p]
     L.pda_last_pass_ms.argtypes = [_vp, ctypes.c_int, _f32p, _f32p]
-    L.pda_set_tuning.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
-    L.pda_get_tuning.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.pda_debug_primitive.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int64]
+    L.pda_debug_microbench.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                       ctypes.POINTER(ctypes.c_float),
+                                       ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_int64)]
     _ip = ctypes.POINTER(ctypes.c_int)
     L.pda_set_launch_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.pda_get_launch_shape.argtypes = [_vp, _ip, _ip, _ip]
@@ -150,6 +154,7 @@ class MC:
         self._N = [0] * self.streamNum
         self._hist_bins = 200           # fret_hist_num default (cmdline.ggo)
         self._left_pad = 2               # src/main.cpp:155
+        self._rb_runs = [0] * self.streamNum  # RB forwards started per slot
 
     # -- lifetime ------------------------------------------------------------
     def close(self):
@@ -316,6 +321,12 @@ class MC:
                                       ctypes.byref(c32), ctypes.byref(eff)))
         return h, c64.value, c32.value, eff.value
 
+    def run_stats(self, sid=0):
+        """(units, units replayed in the general form) of the slot's last run."""
+        u, r = ctypes.c_int64(), ctypes.c_int32()
+        _check(self._L.pda_run_stats(self._h, sid, ctypes.byref(u), ctypes.byref(r)))
+        return u.value, r.value
+
     def get_samples(self, sid, n):
         E = np.zeros(n, np.float32)
         acc = np.zeros(n, np.int32)
@@ -333,14 +344,6 @@ class MC:
         a, b = ctypes.c_float(), ctypes.c_float()
         _check(self._L.pda_last_pass_ms(self._h, sid, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
-
-    def set_tuning(self, tiles_per_chunk, lds_photons):
-        _check(self._L.pda_set_tuning(self._h, tiles_per_chunk, lds_photons))
-
-    def get_tuning(self):
-        t, c = ctypes.c_int(), ctypes.c_int()
-        _check(self._L.pda_get_tuning(self._h, ctypes.byref(t), ctypes.byref(c)))
-        return t.value, c.value
 
     # -- launch shape and autotuner ---------------------------------------------
     def set_launch_shape(self, batch_tiles=0, waves_per_cu=0, queues=0):
@@ -409,6 +412,12 @@ class MC:
         seed = self.seed if seed is None else seed
         _check(self._L.pda_rb_forward(self._h, sid, R, seed, set_id,
                                       PDA_RB_SCORES if scores else 0))
+        self._rb_runs[sid] += 1
+
+    def rb_run_id(self, sid=0):
+        """Number of RB forwards started on the slot: the adjoint and the
+        Jacobian differentiate the slot's last one."""
+        return self._rb_runs[sid]
 
     def rb_result(self, sid=0, nbins=None):
         """(expected counts M_b as float64, the same in 2^-32 units, chi2, effN)."""
@@ -447,6 +456,13 @@ class MC:
         out = np.zeros(n * {0: 4}.get(op, 1), out_dtype)
         _check(self._L.pda_debug_primitive(self._h, op, inp.ctypes.data, out.ctypes.data, n))
         return out
+
+    def debug_microbench(self, op, iters, param=0.0):
+        """(GPU ms, events, inner iterations) of one event microbenchmark launch."""
+        ms, ev, inner = ctypes.c_float(), ctypes.c_int64(), ctypes.c_int64()
+        _check(self._L.pda_debug_microbench(self._h, op, iters, param, ctypes.byref(ms),
+                                            ctypes.byref(ev), ctypes.byref(inner)))
+        return ms.value, ev.value, inner.value
 
 
 def evaluate(m, n, args, R, seed, set_id=0, sid=0, begin=0, end=-1, samples=False,

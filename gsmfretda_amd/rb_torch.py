@@ -29,16 +29,15 @@ class RBExpectedHistogram(torch.autograd.Function):
             raise pda.PdaError(f"bad parameter set {a}")
         mc.rb_forward(R=R, seed=seed, set_id=set_id, sid=sid, scores=True)
         M, _, _, _ = mc.rb_result(sid)
-        mc._rb_token = getattr(mc, "_rb_token", 0) + 1
-        ctx.mc, ctx.n, ctx.sid, ctx.token = mc, n, sid, mc._rb_token
+        ctx.mc, ctx.n, ctx.sid, ctx.token = mc, n, sid, mc.rb_run_id(sid)
         ctx.dtype, ctx.device = args.dtype, args.device
         return torch.from_numpy(M).to(args.device)
 
     @staticmethod
     def backward(ctx, gM):
-        if getattr(ctx.mc, "_rb_token", None) != ctx.token:
-            raise pda.PdaError("another RB forward ran on this context since this one; "
-                               "call backward before the next forward")
+        if ctx.mc.rb_run_id(ctx.sid) != ctx.token:
+            raise pda.PdaError(f"another RB forward ran on slot {ctx.sid} since this one; "
+                               "call backward before the next forward on that slot")
         g = gM.detach().to("cpu", torch.float64).numpy()
         grad = ctx.mc.rb_adjoint(g, ctx.n, ctx.sid)
         return (torch.from_numpy(grad).to(dtype=ctx.dtype, device=ctx.device),

@@ -97,7 +97,9 @@ int pda_set_observed_hist(pda_ctx *ctx, const float *ohist, int nbins, int left_
 int pda_set_params(pda_ctx *ctx, int sid, int n, const float *args, int nargs);
 
 /* Photophysics constants, mc_kernel's default arguments (src/mc.cu:89-91):
- * gamma 0.34, DexDirAem 0.08, Dch2Ach 0.07, r0 52 until set. */
+ * gamma 0.34, DexDirAem 0.08, Dch2Ach 0.07, r0 52 until set.  PDA_E_ARG
+ * unless finite, r0 > 0, gamma >= crosstalk and dexdir <= 1 (otherwise the
+ * corrected photon count, the binomial's trial count, can be negative). */
 int pda_set_photophysics(pda_ctx *ctx, int sid, float gamma, float dexdir, float crosstalk,
                          float r0);
 
@@ -124,6 +126,12 @@ int pda_synchronize(pda_ctx *ctx, int sid);
 int pda_get_result(pda_ctx *ctx, int sid, int32_t *hist, int nbins, double *chi2_64,
                    float *chi2_ref, int *effN);
 
+/* Facts of the slot's last run (waits for it; no reference counterpart):
+ * units simulated and units the fast pass abandoned to the general-form
+ * replay pass (BTPE draws, inversion restarts, background means above 64,
+ * window ends past 2^31 ticks).  Outputs may be NULL. */
+int pda_run_stats(pda_ctx *ctx, int sid, int64_t *units, int32_t *replays);
+
 /* Per-sample values of the last run made with PDA_RUN_SAMPLES, in unit
  * order (burst - begin) * R + r: E (the reference's mcE / hmcE), acceptor
  * count and corrected photon count F.  Any output may be NULL. */
@@ -140,25 +148,13 @@ int pda_last_run_ms(pda_ctx *ctx, int sid, float *ms);
  * (replayed units, histogram reduction, chi2). */
 int pda_last_pass_ms(pda_ctx *ctx, int sid, float *walk_ms, float *finish_ms);
 
-/* Launch shape of the walk kernels (no reference counterpart; the reference
- * takes its block size from cudaOccupancyMaxPotentialBlockSize,
- * src/mc.cu:309): each one-wave workgroup takes batches of
- * tiles_per_chunk x 64 consecutive (burst, resample) units from a work
- * counter (0 = sized from the run, the default); the finishing kernel stages
- * a replayed burst in LDS when it has at most lds_photons photons (0..256,
- * default 256).
- * Results do not depend on either. */
-int pda_set_tuning(pda_ctx *ctx, int tiles_per_chunk, int lds_photons);
-int pda_get_tuning(const pda_ctx *ctx, int *tiles_per_chunk, int *lds_photons);
-
 /* Launch shape of the walk kernels, single-set and population runs (no
  * reference counterpart; the reference sizes its grid with
  * cudaOccupancyMaxPotentialBlockSize, src/mc.cu:309): batch_tiles x 64 units
  * per work-counter grab (0 = sized from the run, about six grabs per
  * wave), resident one-wave workgroups per CU (0 = as many as fit), work
  * counters, each on its own 256-byte line (0 = 32, four per XCD; at most
- * 64).  pda_set_tuning's tiles_per_chunk is the same knob as batch_tiles.
- * Results do not depend on the shape. */
+ * 64).  Results do not depend on the shape. */
 int pda_set_launch_shape(pda_ctx *ctx, int batch_tiles, int waves_per_cu, int queues);
 int pda_get_launch_shape(const pda_ctx *ctx, int *batch_tiles, int *waves_per_cu, int *queues);
 
@@ -259,6 +255,18 @@ int pda_rb_stats(pda_ctx *ctx, int sid, float *walk_ms, float *forward_ms, float
  * calibration).
  * in/out are host arrays; the layout per op is documented in pda_capi.cpp. */
 int pda_debug_primitive(pda_ctx *ctx, int op, const void *in, void *out, int64_t n);
+
+/* Diagnostics (no reference counterpart): microbenchmark of one event of the
+ * walk algorithm, the weights of the algorithmic cost model.  Every lane of a
+ * full-occupancy grid repeats the event `iters` times with a loop-carried
+ * dependence: op 0 a Philox4x32-10 block, 1 a dwell window's scalar work,
+ * 2 a window-end binary-search step, 3 a capped-Poisson draw of mean
+ * `param` (0..64), 4 a binomial inversion draw of n = `param` (0..127)
+ * trials at p = 0.3 from a q^m table, 5 a unit's start and tally.  Writes
+ * the GPU milliseconds of one launch (after an untimed one), the events run
+ * and the inner iterations (Poisson / inversion iterations; 0 otherwise). */
+int pda_debug_microbench(pda_ctx *ctx, int op, int iters, float param, float *ms,
+                         int64_t *events, int64_t *inner);
 
 #ifdef __cplusplus
 }

@@ -49,7 +49,9 @@ class OrParams(ctypes.Structure):
 class OrStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "windows", "photons_visited", "philox_blocks",
-        "binom_inv_iters", "binom_btpe_iters", "poisson_iters")]
+        "binom_inv_iters", "binom_btpe_iters", "poisson_iters",
+        "units", "live_windows", "live_philox_blocks", "search_steps", "binom_draws",
+        "live_binom_inv_iters", "live_btpe_iters", "bg_draws", "live_poisson_iters")]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

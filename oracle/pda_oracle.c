@@ -488,6 +488,12 @@ static void unit_impl(const or_data *d, const or_params *p, int64_t burst, uint3
     /* initial state from the unnormalised inflow rates (src/mc.cu:110) */
     blockw(seed, set, 0xFFFFFFFFu, 0, r, b, x, st);
     int s = or_categorical(n, p->p0, or_u01f(x[0]));
+    /* live: the walk can still change the result (cost-model counters) */
+    int live = P > 1 && T > 0.0f;
+    if (st) {
+        st->units++;
+        if (live) st->live_philox_blocks++;
+    }
 
     float tsum = 0.0f, F = 0.0f, accf = 0.0f;
     int64_t w0 = 0, w1 = 0, cursor = 0;
@@ -497,6 +503,11 @@ static void unit_impl(const or_data *d, const or_params *p, int64_t burst, uint3
         tr->stopped = P <= 1; /* no window of such a burst counts anything */
     }
     while (T > tsum && w < (1u << 22)) { /* src/mc.cu:117; window cap: see header */
+        or_stats before;
+        if (st) before = *st;
+        const int64_t c0 = cursor;
+        const int s0 = s;
+        int32_t m_draw = 0;
         uint32_t A[4];
         blockw(seed, set, w, 0, r, b, A, st);
         if (st) st->windows++;
@@ -548,6 +559,7 @@ static void unit_impl(const or_data *d, const or_params *p, int64_t burst, uint3
                 blockw(seed, set, w, 2, r, b, C, st);
                 float de = broaden(p->E[s], p->r0, p->v[s], or_normal(C[0], C[1]));
                 xdraw = or_binomial((int)m, (double)de, seed, set, b, r, w, A[2], A[3], st);
+                m_draw = (int32_t)m;
                 if (tr && !tr->stopped) {
                     if ((int)m > 0) tr->cnt[s] += (int)m;
                     double sum = 0.0;
@@ -571,6 +583,18 @@ static void unit_impl(const or_data *d, const or_params *p, int64_t burst, uint3
         w0 = w1;
         s = j;
         ++w;
+        if (st && live) {
+            st->live_windows++;
+            st->live_philox_blocks += (st->philox_blocks - before.philox_blocks) -
+                                      (p->v[s0] == 0.0f ? 1 : 0);
+            for (int64_t left = P - c0; left > 0; left >>= 1) st->search_steps++;
+            if (m_draw > 0) st->binom_draws++;
+            if (d->bg_dd >= 1e-4f) st->bg_draws += 2;
+            st->live_binom_inv_iters += st->binom_inv_iters - before.binom_inv_iters;
+            st->live_btpe_iters += st->binom_btpe_iters - before.binom_btpe_iters;
+            st->live_poisson_iters += st->poisson_iters - before.poisson_iters;
+            if (cursor == P - 1) live = 0;
+        }
     }
     float E = F > 0.0f ? accf / F : 0.0f;
     if (acc_out) *acc_out = (int32_t)accf;

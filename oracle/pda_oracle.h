@@ -43,9 +43,21 @@ typedef struct {
     float gamma, dexdir, crosstalk, r0;
 } or_params;
 
+/* Event counters.  The first six count what the oracle executes (every
+ * window until the dwell sum reaches T, as src/mc.cu:117; the linear photon
+ * scan; a normal block per window).  The rest count the events of the walk
+ * algorithm -- the work a unit's result depends on -- for the cost model:
+ * windows up to the one that closes at the last photon (later windows count
+ * no photon and change nothing), the Philox blocks those windows need (no
+ * normal block without broadening), the comparisons of a binary search for
+ * each window's end over the photons left (ceil(log2(left + 1))), binomial
+ * draws with at least one trial, background draws, and the inner iterations
+ * of those windows. */
 typedef struct {
     int64_t windows, photons_visited, philox_blocks;
     int64_t binom_inv_iters, binom_btpe_iters, poisson_iters;
+    int64_t units, live_windows, live_philox_blocks, search_steps, binom_draws;
+    int64_t live_binom_inv_iters, live_btpe_iters, bg_draws, live_poisson_iters;
 } or_stats;
 
 /* -- primitives (exported for the oracle's own pinning tests) -- */

@@ -189,6 +189,38 @@ def test_empty_burst_range(gpu_mc, oracle):
     _compare_units(g, o)
 
 
+def test_cumulative_window_ends_closed_form(gpu_mc):
+    """The HIP walk on 20,000 evenly spaced single-colour bursts x 100: the
+    mean acceptor count equals the closed form of the reference's cumulative
+    window ends (src/mc.cu:132, 184) and rejects window ends at burst start +
+    dwell sum (tests/test_oracle_walk.py derives both)."""
+    from tests.test_oracle_walk import (WINDOW_ARGS, WINDOW_P, WINDOW_PH, window_end_data,
+                                        window_end_prediction)
+    d = window_end_data(20000)
+    _load(gpu_mc, d, np.zeros(d.nbursts, np.float32))
+    g = _gpu_run(gpu_mc, 2, WINDOW_ARGS, 100, 12, ph=WINDOW_PH)
+    assert np.all(g["F"] == WINDOW_P - 1)
+    acc = g["acc"].astype(np.float64)
+    se = acc.std(ddof=1) / np.sqrt(acc.size)
+    want = window_end_prediction(cumulative=True)
+    wrong = window_end_prediction(cumulative=False, kmax=61)
+    assert abs(acc.mean() - want) < 4.5 * se, (acc.mean(), want, se)
+    assert abs(acc.mean() - wrong) > 50 * se, (acc.mean(), wrong, se)
+
+
+def test_photophysics_giving_negative_counts_are_refused(gpu_mc):
+    """gamma < crosstalk or DexDirAem > 1 would make the corrected photon count
+    (the binomial's trial count, src/mc.cu:143) negative: PDA_E_ARG, and the
+    slot keeps its constants.  The boundary cases gamma == crosstalk and
+    DexDirAem == 1 (a channel that counts nothing) are accepted."""
+    from gsmfretda_amd import pda
+    for ph in ((0.05, 0.08, 0.07, 52.0), (0.34, 1.01, 0.07, 52.0)):
+        with pytest.raises(pda.PdaError, match="negative"):
+            gpu_mc.set_photophysics(1, *ph)
+    gpu_mc.set_photophysics(1, 0.07, 1.0, 0.07, 52.0)
+    gpu_mc.set_photophysics(1, *synth.PHOTOPHYSICS)
+
+
 def test_broadening_and_four_states(gpu_mc, oracle):
     args = synth.make_args([0.1, 0.4, 0.6, 0.95], np.linspace(200, 3000, 12), v=[2.0, 0.0, 5.0, 1.0])
     d = synth.generate(800, args, 4, seed=5)
@@ -233,6 +265,8 @@ def test_replayed_units(gpu_mc, oracle):
     g = _gpu_run(gpu_mc, 8, args, 12, 5, ph=(1.0, 0.0, 0.0, 52.0))
     o = _oracle_run(oracle, d, 8, args, (1.0, 0.0, 0.0, 52.0), 12, 5)
     _compare_units(g, o)
+    units, replays = gpu_mc.run_stats(0)
+    assert units == 200 * 12 and replays > 0, (units, replays)
     from tests.test_oracle_walk import tiny_bursts
     d = tiny_bursts(np.random.default_rng(52), 60)
     d.bg_dd, d.bg_ad = 1e6, 3e6
@@ -242,6 +276,30 @@ def test_replayed_units(gpu_mc, oracle):
         g = _gpu_run(gpu_mc, 4, args, R, 1002, ph=(0.34, 0.08, 0.07, 52.0))
         o = _oracle_run(oracle, d, 4, args, (0.34, 0.08, 0.07, 52.0), R, 1002)
         _compare_units(g, o)
+        assert gpu_mc.run_stats(0)[1] > 0
+
+
+def test_many_replayed_units(gpu_mc, oracle):
+    """Workloads where a few per cent of units leave the fast form and go to
+    the grid-wide replay pass: the 3-state workload's bursts under a 30 kHz
+    donor / 15 kHz acceptor background (long windows have background means
+    above 64), and long bursts (150 + Geom(100) photons) of a near-1
+    efficiency (windows with 128 or more acceptor photons in one run).  Every
+    unit equals the oracle's, twice in a row (the finish kernel leaves the
+    replay counters zero)."""
+    cfg = synth.config(1, n_bursts=1)
+    d1 = synth.generate(2000, cfg["args"], 3, seed=93)
+    d1.bg_dd, d1.bg_ad = 3e4, 1.5e4
+    d2 = synth.generate(2000, synth.make_args([0.98, 0.98], [200.0, 200.0]), 2, base_photons=150,
+                        mean_geom=100.0, seed=94)
+    for d in (d1, d2):
+        _load(gpu_mc, d, np.zeros(d.nbursts, np.float32))
+        o = _oracle_run(oracle, d, 3, cfg["args"], cfg["photophysics"], 20, 44)
+        for _ in range(2):
+            g = _gpu_run(gpu_mc, 3, cfg["args"], 20, 44, ph=cfg["photophysics"])
+            _compare_units(g, o)
+            units, replays = gpu_mc.run_stats(0)
+            assert units == d.nbursts * 20 and replays >= units // 100, (units, replays)
 
 
 def test_launch_shape_does_not_change_results(gpu_mc, oracle):
@@ -249,14 +307,14 @@ def test_launch_shape_does_not_change_results(gpu_mc, oracle):
     d = cfg["data"]
     _load(gpu_mc, d, np.zeros(700, np.float32))
     o = _oracle_run(oracle, d, cfg["n"], cfg["args"], cfg["photophysics"], 13, 21)
-    saved = gpu_mc.get_tuning()
+    saved = gpu_mc.get_launch_shape()
     try:
-        for tpc, cap in ((1, 0), (3, 100), (16, 256), (64, 17), (0, 256)):
-            gpu_mc.set_tuning(tpc, cap)
+        for tiles, waves in ((1, 0), (3, 1), (16, 2), (64, 5), (0, 3)):
+            gpu_mc.set_launch_shape(batch_tiles=tiles, waves_per_cu=waves)
             g = _gpu_run(gpu_mc, cfg["n"], cfg["args"], 13, 21, ph=cfg["photophysics"])
             _compare_units(g, o)
     finally:
-        gpu_mc.set_tuning(*saved)
+        gpu_mc.set_launch_shape(*saved)
 
 
 def test_work_counters_do_not_change_results(gpu_mc, oracle):

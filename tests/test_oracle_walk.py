@@ -245,6 +245,102 @@ def test_single_window_binomial_law(oracle):
     assert abs(out["acc"].mean() - 6.0) < 0.1
 
 
+WINDOW_LAM, WINDOW_DELTA, WINDOW_P = 2e4, 400, 101
+
+
+def window_end_data(nb):
+    """Identical single-colour bursts (donor photons WINDOW_DELTA ticks apart,
+    WINDOW_P of them), T = the burst span, no background."""
+    P, delta = WINDOW_P, WINDOW_DELTA
+    ticks = np.arange(P, dtype=np.int64) * delta
+    start = 1000 + np.arange(nb, dtype=np.int64) * (P * delta + 1000)
+    return synth.BurstData(start=start, stop=start + P * delta,
+                           istart=np.arange(nb, dtype=np.int64) * P,
+                           phcount=np.full(nb, P, np.int32),
+                           times=(start[:, None] + ticks[None, :]).ravel(),
+                           mask_ad=np.zeros(nb * P, np.uint8), mask_dd=np.ones(nb * P, np.uint8),
+                           T=np.full(nb, P * delta * synth.CLK_P, np.float32), bg_ad=0.0, bg_dd=0.0)
+
+
+WINDOW_ARGS = synth.make_args([1.0, 0.0], [WINDOW_LAM, WINDOW_LAM])
+WINDOW_PH = (1.0, 0.0, 0.0, 52.0)
+
+
+def window_end_prediction(cumulative=True, kmax=23):
+    """Expected acceptor count per unit on window_end_data under WINDOW_ARGS:
+    2 states, E = (1, 0), both rates lam, initial state 0 (its inflow lam > 1
+    exceeds every uniform), gamma = 1 and no crosstalk, so a window's trial
+    count is its photon count and state-0 windows (k even) give every photon
+    to the acceptor, state-1 windows none.
+
+    Window k ends at W_{k+1} (seconds after the burst start) and photon j
+    (tick j delta) is counted in a window <= k iff its tick lies below the
+    truncated end tick, i.e. W_{k+1} >= (j delta + 1) clk; the last photon is
+    never counted (src/mc.cu:140-148).  A window whose dwell sum reaches T
+    ends at the stop, after every photon -- and with T the burst span this
+    changes no assignment, since then W >= S >= T.  So
+
+        E[acc] = sum_{j < P-1} sum_{k even} P(W_{k+1} >= y_j) - P(W_k >= y_j).
+
+    src/mc.cu:132 and :184 set W_{k+1} = W_k + S_{k+1} with S the cumulative
+    dwell sum (mcSpendTime), so W_k = sum_{m=1..k} m tau_m for iid
+    tau ~ Exp(lam): a hypoexponential law of rates lam / m, whose tail is
+    sum_m C_m exp(-lam y / m) with C_m = prod_{l != m} m / (m - l) (exact
+    rationals, evaluated at 50 digits).  The misreading W_k = S_k (window
+    ends at burst start + dwell sum) makes W_k Erlang(k, lam)."""
+    import mpmath as mp
+    mp.mp.dps = 50
+    lam = mp.mpf(WINDOW_LAM)
+    clk = mp.mpf(synth.CLK_P)
+
+    if cumulative:
+        coef = {}
+        for k in range(1, kmax + 2):
+            cs = []
+            for m in range(1, k + 1):
+                c = Fraction(1)
+                for l in range(1, k + 1):
+                    if l != m:
+                        c *= Fraction(m, m - l)
+                cs.append(mp.mpf(c.numerator) / c.denominator)
+            coef[k] = cs
+
+        def sf(y, k):
+            if k == 0:
+                return mp.mpf(0)
+            return mp.fsum(c * mp.exp(-lam / m * y) for m, c in enumerate(coef[k], start=1))
+    else:
+        def sf(y, k):
+            if k == 0:
+                return mp.mpf(0)
+            return mp.gammainc(k, lam * y, mp.inf, regularized=True)
+
+    tot = mp.mpf(0)
+    for j in range(WINDOW_P - 1):
+        y = mp.mpf(j * WINDOW_DELTA + 1) * clk
+        for k in range(0, kmax, 2):
+            tot += sf(y, k + 1) - sf(y, k)
+    return float(tot)
+
+
+def test_cumulative_window_ends_closed_form(oracle):
+    """Q2 (src/mc.cu:132, 184): window ends add the cumulative dwell sum to
+    the previous end.  The oracle's mean acceptor count on evenly spaced,
+    single-colour bursts equals the closed form of that reading and rejects
+    the closed form of window ends at burst start + dwell sum."""
+    d = window_end_data(2000)
+    p = oracle.make_params(2, WINDOW_ARGS, *WINDOW_PH)
+    out = oracle.run(oracle.Data.from_bursts(d), p, R=100, seed=11)
+    assert np.all(out["F"] == WINDOW_P - 1)
+    acc = out["acc"].astype(np.float64)
+    se = acc.std(ddof=1) / np.sqrt(acc.size)
+    want = window_end_prediction(cumulative=True)
+    assert abs(want - window_end_prediction(cumulative=True, kmax=19)) < 1e-9  # tail converged
+    wrong = window_end_prediction(cumulative=False, kmax=61)
+    assert abs(acc.mean() - want) < 4.5 * se, (acc.mean(), want, se)
+    assert abs(acc.mean() - wrong) > 20 * se, (acc.mean(), wrong, se)
+
+
 def test_histogram_edges(oracle):
     x = np.array([0.0, 0.02, np.nextafter(np.float32(0.02), np.float32(1)), 0.5, 0.999999, 1.0,
                   -0.1, np.nan, 0.98], np.float32)

@@ -138,6 +138,31 @@ def test_graph_replays_match_direct_launches(gpu_mc, data):
     assert m.population_stats()[2] == cap1 + 1
 
 
+def test_graph_after_reload_reads_the_new_bursts(gpu_mc, oracle, data):
+    """A captured graph holds the addresses of the resident bursts.  Loading
+    other bursts of the same count (the context frees and reallocates its
+    buffers, and may get an address back) must not replay the old graph:
+    the result equals the oracle on the new bursts, across two reloads."""
+    d, obs = data
+    m = gpu_mc
+    sets = _population(4, 31)
+    R, seed = 8, 313
+    args = synth.make_args([0.25, 0.55, 0.9], [900, 2500, 600, 2000, 1500, 5000])
+    for k, ds in enumerate((77, 79, 81)):
+        dk = d if ds == 77 else synth.generate(d.nbursts, args, 3, seed=ds)
+        _load(m, dk, obs)
+        m.set_population([n for n, _ in sets], [a for _, a in sets])
+        m.run_population(R=R, seed=seed, set_base=0, set_stride=1)
+        hist, _, c32, _ = m.get_population_result()
+        od = oracle.Data.from_bursts(dk)
+        oh = _obs_hist(oracle, obs)
+        for i, (n, a) in enumerate(sets):
+            h_o, c_o = _oracle_set(oracle, od, oh, n, a, R, seed, i)
+            assert np.array_equal(hist[i].astype(np.int64), h_o), f"load {k} set {i}"
+            assert c32[i] == np.float32(c_o), f"load {k} set {i}"
+    _load(m, d, obs)
+
+
 def test_common_random_numbers(gpu_mc, data):
     d, obs = data
     m = gpu_mc

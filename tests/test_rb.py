@@ -290,6 +290,31 @@ def test_torch_autograd_function(gpu_mc):
     assert np.allclose(args.grad.numpy(), want, rtol=1e-9, atol=0), (args.grad, want)
 
 
+def test_torch_backward_refuses_after_another_forward_on_its_slot(gpu_mc):
+    """The adjoint differentiates the slot's last forward: a direct
+    MC.rb_forward on the same slot between forward and backward is refused,
+    one on another slot is not."""
+    torch = pytest.importorskip("torch")
+    from gsmfretda_amd import rb_torch
+    d = _data(400, 83)
+    obs = _load(gpu_mc, d)
+    oh = torch.tensor(_obs_hist(obs))
+    R, seed = 8, 5
+    args = torch.tensor(np.asarray(ARGS3, np.float64), requires_grad=True)
+    loss = rb_torch.rb_chi2(args, gpu_mc, 3, R, seed, oh, sid=0)
+    _rb(gpu_mc, 3, ARGS3, R, seed + 1)  # slot 0 again
+    with pytest.raises(pda.PdaError, match="slot 0"):
+        loss.backward()
+    loss = rb_torch.rb_chi2(args, gpu_mc, 3, R, seed, oh, sid=0)
+    gpu_mc.set_photophysics(1, *PH)
+    gpu_mc.set_burst_range(0, -1, 1)
+    assert gpu_mc.set_params(3, 1, np.asarray(ARGS3, np.float32))
+    gpu_mc.rb_forward(R=R, seed=seed + 2, sid=1)  # another slot
+    gpu_mc.rb_result(1)
+    loss.backward()
+    assert np.all(np.isfinite(args.grad.numpy()))
+
+
 def test_rb_errors(gpu_mc):
     d = _data(50, 91)
     _load(gpu_mc, d)
