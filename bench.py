@@ -10,11 +10,16 @@ and chi2 read back on the host; the timed steps take turns on three streams
 time).  The kernel durations behind the
 rooflines come from a separate run of the same evaluations one at a time.
 
-Besides the headline number the line carries the VALU and HBM rooflines of
-the walk kernel, the batched-population leg (256 sets in one graph replay),
-the Rao-Blackwellised leg (forward, adjoint and Jacobian GPU times), the fit
-leg (wall seconds to 1.01 x chi2(truth) for the per-set genetic algorithm
-and the gradient fitters, 100,000 bursts) and the CPU baseline (the oracle
+Besides the headline number the line carries the VALU roofline of the walk
+kernel on two bases -- the algorithm's events (counted by the oracle on the
+same random streams) times their microbenchmarked weights, and the kernel's
+own VALU lane-cycle counter -- and its HBM roofline; a replay leg (the same
+bursts under a 30 kHz background, where a few per cent of units leave the
+fast form); the batched-population leg (256 sets in one graph replay); the
+fast-dynamics legs at 1,000,000 bursts (64 sets x 32 resamples in one graph
+replay, and the fit wall time to 1.01 x chi2(truth) of the per-set genetic
+algorithm against the gradient fitters); the Rao-Blackwellised leg
+(forward, adjoint and Jacobian GPU times) and the CPU baseline (the oracle
 on the host's cores).
 
 Prints one JSON line (rank 0).  --gpus above 1 is refused: this build runs
@@ -79,21 +84,74 @@ def cpu_baseline(cfg, threads, sample_bursts, repeats=5, single_bursts=2000):
                                f"1 thread, {dt1:.2f} s"})
 
 
-def roofline(units, walk_ms, total_ms):
-    cm = json.load(open(COST_MODEL))
+def valu_roofline(workload, units, ms, kernel, traffic=None):
+    """VALU roofline of a walk launch priced by the algorithmic cost model:
+    the workload's events per unit x their weights, over the launch time."""
+    from gsmfretda_amd import cost_model
+    cm = cost_model.load()
+    opu = cost_model.ops_per_unit(workload, cm)
+    achieved = opu * units / (ms * 1e-3) / 1e12
+    peak = cm["valu_peak_lane_ops_per_s"] / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
+            "frac": achieved / peak, "traffic": traffic, "kernel": kernel, "kernel_ms_avg": ms,
+            "ops_per_unit": opu, "basis": "algorithm: oracle events x microbenchmarked weights",
+            "events": workload}
+
+
+def roofline(units, walk_ms):
+    """The walk kernel of the headline evaluation: VALU on the algorithmic
+    basis, VALU on the counter basis (its own frozen lane-cycle count), HBM."""
+    from gsmfretda_amd import cost_model
+    cm = cost_model.load()
     t = walk_ms * 1e-3
+    traffic = cm["hbm_traffic_bytes_per_launch"]
+    valu = valu_roofline("plain_3state", units, walk_ms, "walk_kernel<0>", traffic)
     ops = cm["valu_lane_ops_per_unit"] * units
     achieved = ops / t / 1e12
     peak = cm["valu_peak_lane_ops_per_s"] / 1e12
-    valu = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
-            "frac": achieved / peak, "traffic": cm["hbm_traffic_bytes_per_launch"],
-            "kernel": "walk_kernel<0>", "kernel_ms_avg": walk_ms,
-            "ops_per_unit": cm["valu_lane_ops_per_unit"]}
+    counter = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
+               "frac": achieved / peak, "traffic": traffic, "kernel": "walk_kernel<0>",
+               "kernel_ms_avg": walk_ms, "ops_per_unit": cm["valu_lane_ops_per_unit"],
+               "basis": "SQ_THREAD_CYCLES_VALU per unit of an earlier build, frozen"}
     hb = cm["hbm_algorithmic_bytes_per_unit"] * units / t / 1e9
     hbm = {"bound": "hbm", "achieved": hb, "peak": cm["hbm_peak_bytes_per_s"] / 1e9,
-           "unit": "GB/s", "frac": hb * 1e9 / cm["hbm_peak_bytes_per_s"],
-           "traffic": cm["hbm_traffic_bytes_per_launch"]}
-    return valu, hbm
+           "unit": "GB/s", "frac": hb * 1e9 / cm["hbm_peak_bytes_per_s"], "traffic": traffic}
+    return valu, counter, hbm
+
+
+def replay_leg(pda, cfg, steps):
+    """The headline bursts under a 30 kHz donor / 15 kHz acceptor background:
+    windows longer than ~1 ms have background means above 64, so a few per
+    cent of units leave the fast form for the grid-wide replay pass.  Reports
+    the replayed units and the GPU time of the walk and of replay + finish."""
+    d = cfg["data"]
+    bg = (d.bg_ad, d.bg_dd)
+    d.bg_dd, d.bg_ad = 3e4, 1.5e4
+    try:
+        m = pda.MC(0, 1)
+        m.load(d, SgDivSr=None)
+        m.set_observed_hist(np.ones(50, np.float32))
+        m.set_photophysics(0, *cfg["photophysics"])
+        assert m.set_params(cfg["n"], 0, cfg["args"])
+        m.set_burst_range(0, -1, 0)
+        m.reSampleTimes = cfg["R"]
+        walk, rest = [], []
+        for i in range(steps + 2):
+            m.run_kernel(sid=0, seed=cfg["seeds"]["mc"], set_id=0)
+            m.synchronize(0)
+            w, f = m.last_pass_ms(0)
+            if i >= 2:
+                walk.append(w)
+                rest.append(f)
+        units, replays = m.run_stats(0)
+        m.close()
+    finally:
+        d.bg_ad, d.bg_dd = bg
+    w, r = float(np.median(walk)), float(np.median(rest))
+    return {"workload": "3 states, 100000 bursts x 100 resamples, background 30 kHz donor / "
+                        "15 kHz acceptor",
+            "units": units, "replayed_units": replays, "replayed_fraction": replays / units,
+            "walk_gpu_ms": w, "replay_and_finish_gpu_ms": r, "total_over_walk": (w + r) / w}
 
 
 def population_leg(pda, synth, steps, warmup):
@@ -130,18 +188,22 @@ def population_leg(pda, synth, steps, warmup):
     st = m.population_stats()
     m.close()
     # VALU roofline of the whole generation (graph replay timed with HIP
-    # events) at the frozen lane-operations per unit of this workload
+    # events): algorithmic basis, and the frozen lane-cycle counter
     cm = json.load(open(COST_MODEL))
+    g_ms = float(np.mean(gpu_ms))
     ops = cm["population_valu_lane_ops_per_unit"] * units
-    achieved = ops / (float(np.mean(gpu_ms)) * 1e-3) / 1e12
+    achieved = ops / (g_ms * 1e-3) / 1e12
     peak = cm["valu_peak_lane_ops_per_s"] / 1e12
     return {"workload": "batched population: 256 parameter sets (2-4 states, rates 1e2-1e4), "
                         "50000 bursts x 32 resamples, 50 bins, per-set chi2, one hipGraph replay "
                         "per generation",
             "value": units * steps / dt, "unit": "burst-resamples/s",
-            "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
-                         "frac": achieved / peak, "kernel": "pop_walk_kernel<0> (whole generation)",
-                         "ops_per_unit": cm["population_valu_lane_ops_per_unit"]},
+            "roofline": valu_roofline("population_256", units, g_ms,
+                                      "pop_walk_kernel<0> (whole generation)"),
+            "roofline_counter": {"bound": "valu", "achieved": achieved, "peak": peak,
+                                 "unit": "TOP/s", "frac": achieved / peak,
+                                 "ops_per_unit": cm["population_valu_lane_ops_per_unit"],
+                                 "basis": "SQ_THREAD_CYCLES_VALU per unit, frozen"},
             "ms_per_generation": dt / steps * 1e3, "gpu_ms_per_generation": float(np.mean(gpu_ms)),
             "units_per_generation": units, "graph_captures": st[2], "replayed_units": st[1],
             "autotuned_shape": {"batch_tiles": tuned[0], "waves_per_cu": tuned[1],
@@ -197,16 +259,52 @@ def rb_leg(pda, synth, steps):
             "chi2": c2}
 
 
-def fit_leg(pda, synth, seconds):
+def fast_population_leg(pda, cfg, steps):
+    """Fast-dynamics population: the 64 starting sets of a fit (4 states,
+    rates 1e3-1e5 s^-1) over 1,000,000 bursts x 32 resamples in one graph
+    replay per generation (2.05e9 units)."""
+    d, R, pop = cfg["data"], cfg["R"], cfg["population"]
+    m = pda.MC(0, 1)
+    m.load(d, SgDivSr=None)
+    m.set_observed_hist(np.ones(50, np.float32))
+    m.set_photophysics(0, *cfg["photophysics"])
+    m.set_burst_range(0, -1, 0)
+    m.set_population([n for n, _ in pop], [a for _, a in pop])
+    seed = cfg["seeds"]["mc"]
+    m.run_population(R=R, seed=seed)
+    m.get_population_result()
+    gpu_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.run_population(R=R, seed=seed)
+        m.get_population_result()
+        gpu_ms.append(m.population_stats()[3])
+    dt = time.perf_counter() - t0
+    st = m.population_stats()
+    m.close()
+    units = d.nbursts * R * len(pop)
+    g_ms = float(np.mean(gpu_ms))
+    return {"workload": "fast-dynamics population: 64 parameter sets (4 states, rates 1e3-1e5), "
+                        "1000000 bursts x 32 resamples, 50 bins, per-set chi2, one hipGraph "
+                        "replay per generation",
+            "value": units * steps / dt, "unit": "burst-resamples/s",
+            "units_per_generation": units, "ms_per_generation": dt / steps * 1e3,
+            "gpu_ms_per_generation": g_ms, "replayed_units": st[1],
+            "roofline": valu_roofline("fast_dynamics_population_64", units, g_ms,
+                                      "pop_walk_kernel<0> (whole generation)")}
+
+
+def fit_leg(pda, cfg, limits):
     """Fit wall time to 1.01 x chi2(truth) on the fast-dynamics workload at
-    100,000 bursts (4 states, rates 1e3-1e5 s^-1, 32 resamples): the
-    per-set genetic algorithm (one pda_run per individual on three stream
-    slots in flight, the reference's pattern) against the gradient fitters on the RB form, same 64 starting
-    sets (the gradient fitters start from the best of them, ranked by one
-    timed batched evaluation with 4 resamples per burst).  Each fitter stops
-    at the target or after `seconds`."""
+    its full size (4 states, rates 1e3-1e5 s^-1, 1,000,000 bursts x 32
+    resamples): the per-set genetic algorithm (one pda_run per individual on
+    three stream slots in flight, the reference's pattern) against the
+    gradient fitters on the RB form, from the same 64 starting sets (the
+    gradient fitters start from the best of them, ranked by one timed
+    batched evaluation with 4 resamples per burst).  Each fitter stops at the
+    target or at its wall limit; a speed-up over a fitter that did not reach
+    the target within its limit is a lower bound."""
     from gsmfretda_amd import fit, fit_study
-    cfg = synth.config(4, n_bursts=100_000)
     n = cfg["n"]
     m = pda.MC(0, 3)
     prob = fit_study.make_problem(m, cfg)
@@ -215,23 +313,27 @@ def fit_leg(pda, synth, seconds):
     pop0 = [fit.args_to_theta(a, n) for _, a in cfg["population"][:64]]
     res = {}
     h = fit.GeneticFitter(batched=False, pop_size=64, rng=np.random.default_rng(7)).run(
-        prob, pop0, seconds, target=target)
+        prob, pop0, limits["ga-per-set"], target=target)
     res["ga-per-set"] = h
     for name, make in (("adam-rb", lambda: fit.AdamFitter(lr=0.1, R_grad=8, check_every=5)),
                        ("lm-rb", lambda: fit.LMFitter(R_grad=8))):
-        h, _ = fit.fit_from_population(make(), prob, pop0, seconds, target=target,
+        h, _ = fit.fit_from_population(make(), prob, pop0, limits[name], target=target,
                                        rank_resamples=4, restarts=name == "lm-rb")
         res[name] = h
     m.close()
     t = {k: v.time_to(target) for k, v in res.items()}
-    out = {"workload": "fit to 1.01 x chi2(truth): 4 states, rates 1e3-1e5 s^-1, 100000 bursts x 32 "
-                       "resamples, 50 bins, 64 starting sets; wall seconds per fitter",
-           "chi2_truth": c_truth, "seconds_limit": seconds, "seconds_to_target": t,
-           "final_chi2": {k: v.chi2[-1] for k, v in res.items()}}
-    if t["lm-rb"]:
-        ga = t["ga-per-set"]
-        out["speedup_lm_vs_ga_per_set"] = (ga if ga else seconds) / t["lm-rb"]
-        out["speedup_is_lower_bound"] = ga is None
+    out = {"workload": f"fit to 1.01 x chi2(truth): 4 states, rates 1e3-1e5 s^-1, "
+                       f"{cfg['data'].nbursts} bursts x 32 resamples, 50 bins, 64 starting sets; "
+                       f"wall seconds per fitter",
+           "bursts": cfg["data"].nbursts, "chi2_truth": c_truth, "seconds_limit": limits,
+           "seconds_to_target": t, "final_chi2": {k: v.chi2[-1] for k, v in res.items()},
+           "evaluations": {k: v.evals[-1] for k, v in res.items()}}
+    ga = t["ga-per-set"]
+    for name in ("lm-rb", "adam-rb"):
+        if t[name]:
+            out[f"speedup_{name.split('-')[0]}_vs_ga_per_set"] = (
+                ga if ga else limits["ga-per-set"]) / t[name]
+    out["speedup_is_lower_bound"] = ga is None
     return out
 
 
@@ -244,7 +346,13 @@ def main():
     ap.add_argument("--no-population", action="store_true",
                     help="skip the batched-population leg")
     ap.add_argument("--no-fit", action="store_true", help="skip the fit wall-time leg")
-    ap.add_argument("--fit-seconds", type=float, default=30.0)
+    ap.add_argument("--fit-seconds", type=float, default=60.0,
+                    help="wall limit of the genetic algorithm and Adam (LM: half)")
+    ap.add_argument("--fit-bursts", type=int, default=1_000_000,
+                    help="bursts of the fast-dynamics workload (fit and population legs)")
+    ap.add_argument("--no-replay", action="store_true", help="skip the replay leg")
+    ap.add_argument("--no-fast", action="store_true",
+                    help="skip the fast-dynamics population leg")
     ap.add_argument("--no-rb", action="store_true",
                     help="skip the Rao-Blackwellised (differentiable) leg")
     ap.add_argument("--cpu-bursts", type=int, default=100_000)
@@ -313,14 +421,19 @@ def main():
             if i >= S:
                 m.synchronize(i % S)
                 out.append(m.get_hist_chi2(i % S))
+                walk_in_flight.append(m.last_pass_ms(i % S)[0])
             m.run_kernel(sid=i % S, seed=seed, set_id=0)
         for i in range(max(0, k - S), k):
             m.synchronize(i % S)
             out.append(m.get_hist_chi2(i % S))
+            walk_in_flight.append(m.last_pass_ms(i % S)[0])
         return out
+
+    walk_in_flight = []
 
     # whole-job throughput (timed)
     evaluations(args.warmup)
+    walk_in_flight.clear()
     t0 = time.perf_counter()
     res = evaluations(args.steps)
     dt = time.perf_counter() - t0
@@ -351,15 +464,26 @@ def main():
                           "ms_per_step": t_one / args.steps * 1e3},
         "chi2": chi2_64,
     }
-    out["roofline"], out["roofline_hbm"] = roofline(units, float(np.mean(walk_ms)),
-                                                    float(np.mean(kernel_ms)))
+    out["roofline"], out["roofline_counter"], out["roofline_hbm"] = roofline(
+        units, float(np.mean(walk_ms)))
+    out["walk_ms_in_flight_avg"] = float(np.mean(walk_in_flight))
     m.close()
+    if not args.no_replay:
+        out["replay"] = replay_leg(pda, cfg, max(3, args.steps // 4))
     if not args.no_population:
         out["population"] = population_leg(pda, synth, max(1, args.steps // 4), 1)
+    if not (args.no_fast and args.no_fit):
+        cfg4 = synth.config(4, n_bursts=args.fit_bursts)
+        if not args.no_fast:
+            out["fast_dynamics_population"] = fast_population_leg(pda, cfg4,
+                                                                  max(2, args.steps // 4))
+        if not args.no_fit:
+            lim = args.fit_seconds
+            out["fit"] = fit_leg(pda, cfg4, {"ga-per-set": lim, "adam-rb": lim,
+                                             "lm-rb": lim / 2})
+        del cfg4
     if not args.no_rb:
         out["rao_blackwellised"] = rb_leg(pda, synth, max(2, args.steps // 4))
-    if not args.no_fit:
-        out["fit"] = fit_leg(pda, synth, args.fit_seconds)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, threads, args.cpu_bursts)

@@ -552,7 +552,8 @@ enum MicroOp : int {
     kMbPhilox = 0,   // one Philox4x32-10 block
     kMbWindow = 1,   // a dwell window's scalar work (next state, dwell, window end,
                      // corrected count, its rounding and sums)
-    kMbSearch = 2,   // one step of the window-end binary search (LDS-resident ticks)
+    kMbSearch = 2,   // one step of the window-end binary search (LDS-resident ticks,
+                     // whole 7-step searches)
     kMbPoisson = 3,  // one capped-Poisson draw (and its inversion iterations)
     kMbBinom = 4,    // one binomial inversion draw from the q^m table (and its
                      // iterations)
@@ -572,16 +573,19 @@ __global__ __launch_bounds__(kMbThreads) void microbench_kernel(int iters, float
     uint32_t h = gid * 0x9E3779B9u + 0x7F4A7C15u;
     uint64_t events = 0, inner = 0;
     __shared__ uint32_t s_tick[128];
-    __shared__ int32_t s_hist[64];
+    __shared__ int32_t s_hist[kMbThreads];
     if (kOp == kMbSearch || kOp == kMbUnit) {
         for (uint32_t i = threadIdx.x; i < 128u; i += blockDim.x) s_tick[i] = i * 97u;
-        for (uint32_t i = threadIdx.x; i < 64u; i += blockDim.x) s_hist[i] = 0;
+        for (uint32_t i = threadIdx.x; i < kMbThreads; i += blockDim.x) s_hist[i] = 0;
         __syncthreads();
     }
     if (kOp == kMbPhilox) {
+        // every counter and key word changes per block (no part of the
+        // rounds can be hoisted out of the loop)
         Block x{{h, gid, 0u, 0u}};
         for (int i = 0; i < iters; ++i)
-            x = philox10(uint32_t(i), x.w[0], gid, x.w[1], h, x.w[2] ^ x.w[3]);
+            x = philox10(uint32_t(i) ^ x.w[3], x.w[0], gid ^ x.w[2], x.w[1], h ^ uint32_t(i),
+                         x.w[2] ^ x.w[3]);
         h = x.w[0] ^ x.w[1] ^ x.w[2] ^ x.w[3];
         events = uint64_t(iters);
     } else if (kOp == kMbWindow) {
@@ -616,19 +620,23 @@ __global__ __launch_bounds__(kMbThreads) void microbench_kernel(int iters, float
         h ^= fbits(F) ^ fbits(acc);
         events = uint64_t(iters);
     } else if (kOp == kMbSearch) {
-        uint32_t lo = 0, hi = 128, key = h & 0x3FFFu;
+        // whole lower_bound searches over 127 ticks (7 steps each), a new
+        // key per search
+        uint32_t acc = 0;
         for (int i = 0; i < iters; ++i) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_tick[mid] < key) lo = mid + 1u;
-            else hi = mid;
-            const bool done = lo >= hi;
-            h = done ? lcg(h) : h;
-            key = done ? (h & 0x3FFFu) : key;
-            lo = done ? 0u : lo;
-            hi = done ? 128u : hi;
+            h = lcg(h);
+            const uint32_t key = h & 0x3FFFu;
+            uint32_t lo = 0, hi = 127;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_tick[mid] < key) lo = mid + 1u;
+                else hi = mid;
+            }
+            acc += lo;
         }
-        h ^= lo;
-        events = uint64_t(iters);
+        h ^= acc;
+        events = uint64_t(iters) * 7u;
     } else if (kOp == kMbPoisson) {
         uint64_t it = 0;
         for (int i = 0; i < iters; ++i) {
@@ -675,8 +683,10 @@ __global__ __launch_bounds__(kMbThreads) void microbench_kernel(int iters, float
             F += float(h & 7u);
             const float E = F > 0.0f ? acc / F : 0.0f;
             if (E >= 0.0f && E < 1.0f) {
+                // the histogram update; one address per lane (how often lanes
+                // collide on a bin depends on the data, not on the algorithm)
                 const int bin = int(floor(double(E) * 50.0));
-                if (bin < 50) atomicAdd(&s_hist[bin], 1);
+                if (bin < 50) atomicAdd(&s_hist[threadIdx.x], bin + 1);
             }
             acc = E * 3.0f;
         }

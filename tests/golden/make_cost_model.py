@@ -35,15 +35,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 
-COST_MODEL = os.path.join(ROOT, "gsmfretda_amd", "cost_model.json")
+from gsmfretda_amd.cost_model import PATH as COST_MODEL  # noqa: E402
 PEAK = 256 * 64 * 2.4e9
 EVENTS = ("units", "live_windows", "live_philox_blocks", "search_steps", "binom_draws",
           "live_binom_inv_iters", "live_btpe_iters", "bg_draws", "live_poisson_iters")
-# weight of each event (names in cost_model.json "weights_lane_cycles")
-WEIGHT_OF = {"units": "unit", "live_windows": "window", "live_philox_blocks": "philox_block",
-             "search_steps": "search_step", "binom_draws": "binom_draw",
-             "live_binom_inv_iters": "binom_iter", "bg_draws": "bg_draw",
-             "live_poisson_iters": "poisson_iter", "live_btpe_iters": "btpe_iter"}
 
 _G = {}
 
@@ -107,7 +102,7 @@ def weights_model():
     w, raw = {}, {}
     with pda.MC(0, 1) as m:
         for name, op, iters in (("philox_block", 0, 4000), ("window", 1, 4000),
-                                ("search_step", 2, 40000), ("unit", 5, 8000)):
+                                ("search_step", 2, 6000), ("unit", 5, 8000)):
             s, _, ms = _slope(m, op, iters)
             w[name] = s
             raw[name] = dict(op=op, iters=[iters, 2 * iters], ms_long=ms)
