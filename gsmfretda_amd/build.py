@@ -23,7 +23,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
           "-Wall", "-Wno-unused-command-line-argument", "-I", os.path.join(ROOT, "include")]
 
 SOURCES = ["pda_kernels.hip", "pda_layout.cpp", "pda_capi.cpp", "pda_population.cpp", "pda_rb_kernels.hip", "pda_rb.cpp",
-           "mc_shim.cpp"]
+           "pda_noalloc.hip", "mc_shim.cpp"]
 
 
 def _run(cmd, verbose):

@@ -197,6 +197,8 @@ int pda_create(int device, int nstreams, pda_ctx** out) {
     if (nstreams == 0) nstreams = 4;
     if (nstreams < 0 || nstreams > 64) return fail(PDA_E_ARG, "nstreams %d outside [1, 64]", nstreams);
     HIP_TRY(hipSetDevice(device));
+    int rc0 = noalloc_setup(device);
+    if (rc0) return rc0;
     auto* ctx = new pda_ctx();
     ctx->device = device;
     hipDeviceProp_t prop;

@@ -187,6 +187,9 @@ int sync_all(pda_ctx* ctx);
 void range_of(const pda_ctx* ctx, const Slot& s, int64_t* b0, int64_t* b1);
 void free_population(Population& p);
 void drop_graph(Population& p);
+// pda_noalloc.hip: looks up the device's copy of the no-alloc scratch array
+// (called by pda_create with the device current)
+int noalloc_setup(int device);
 void free_rb(RbState& r);
 
 // Walk passes (pda_kernels.hip): fast form, general form over the replay

@@ -39,7 +39,8 @@ EXPORTS = (
     "pda_set_launch_shape", "pda_get_launch_shape", "pda_autotune",
     "pda_set_population", "pda_run_population", "pda_get_population_result",
     "pda_population_stats", "pda_rb_forward", "pda_rb_get_result", "pda_rb_adjoint",
-    "pda_rb_jacobian", "pda_rb_stats", "pda_debug_primitive", "pda_debug_microbench",
+    "pda_rb_jacobian", "pda_rb_stats", "pda_run_noalloc", "pda_debug_primitive",
+    "pda_debug_microbench",
 )
 
 
@@ -99,6 +100,8 @@ def load_library(path=LIB_PATH):
     L.pda_last_run_ms.argtypes = [_vp, ctypes.c_int, _f32p]
     L.pda_last_pass_ms.argtypes = [_vp, ctypes.c_int, _f32p, _f32p]
     L.pda_debug_primitive.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int64]
+    L.pda_run_noalloc.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                  ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int64]
     L.pda_debug_microbench.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                        ctypes.POINTER(ctypes.c_float),
                                        ctypes.POINTER(ctypes.c_int64),
@@ -320,6 +323,14 @@ class MC:
         _check(self._L.pda_get_result(self._h, sid, _p(h, _i32p), nb, ctypes.byref(c64),
                                       ctypes.byref(c32), ctypes.byref(eff)))
         return h, c64.value, c32.value, eff.value
+
+    def run_noalloc(self, d_result, result_bytes, R=None, seed=None, set_id=0, sid=0):
+        """pda_run without device allocation: the result (pda_result header,
+        then int32 hist[nbins]) goes to the device buffer at address
+        d_result; asynchronous on the slot's stream."""
+        R = self.reSampleTimes if R is None else R
+        seed = self.seed if seed is None else seed
+        _check(self._L.pda_run_noalloc(self._h, sid, R, seed, set_id, d_result, result_bytes))
 
     def run_stats(self, sid=0):
         """(units, units replayed in the general form) of the slot's last run."""

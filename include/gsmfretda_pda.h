@@ -245,6 +245,36 @@ int pda_rb_jacobian(pda_ctx *ctx, int sid, double *J, int nbins, int nargs);
  * Outputs may be NULL. */
 int pda_rb_stats(pda_ctx *ctx, int sid, float *walk_ms, float *forward_ms, float *adjoint_ms);
 
+/* Device-side result of pda_run_noalloc: this header, then int32 hist[nbins]
+ * (the layout pda_get_result reads from). */
+typedef struct pda_result {
+    double chi2_64;   /* reduced chi2 in FP64                                  */
+    float chi2_ref;   /* reduced chi2, the reference's float accumulation      */
+    int32_t effN;     /* bins kept                                             */
+    int64_t units;    /* (burst, resample) units simulated                     */
+    int32_t nbins;
+    int32_t replays;  /* units the fast pass abandoned to the replay pass      */
+} pda_result;
+
+/* Largest bursts x R of one pda_run_noalloc call. */
+#define PDA_NOALLOC_MAX_UNITS 16000000
+
+/* No reference counterpart: pda_run for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same evaluation, random streams and result as pda_run of the
+ * slot's parameter set over its burst range, asynchronous on the slot's
+ * stream; no per-sample output and no result copy to the host: the result
+ * (pda_result, then int32 hist[nbins]) is written to the caller's device
+ * buffer d_result of result_bytes (at least sizeof(pda_result) + 4 * nbins).
+ * The run's scratch (work counters, replay list, histogram rows) is a fixed
+ * 64 MiB device array that each GPU holds until the process ends, so a call
+ * takes at most PDA_NOALLOC_MAX_UNITS units (else PDA_E_ARG) and calls no
+ * allocator.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (any slot or context) race and must be serialised by
+ * the caller. */
+int pda_run_noalloc(pda_ctx *ctx, int sid, int R, uint32_t seed, uint32_t set_id,
+                    void *d_result, int64_t result_bytes);
+
 /* Diagnostics (no reference counterpart): evaluates one primitive of the
  * shared variate library on the GPU for n inputs, so tests can compare it
  * bit for bit with the host.  op: 0 philox (in: 6 u32 per item -> 4 u32),
