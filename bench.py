@@ -41,6 +41,14 @@ METRIC = "burst-resamples/sec on 1 MI355X; fit wall-time to chi2 tol (grad+batch
 COST_MODEL = os.path.join(ROOT, "gsmfretda_amd", "cost_model.json")
 
 
+_T0 = time.perf_counter()
+
+
+def note(msg):
+    """Progress line on stderr (the JSON result line is the only stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -250,13 +258,34 @@ def rb_leg(pda, synth, steps):
             jac.append(j)
     m.close()
     units = d.nbursts * R
-    return {"workload": "Rao-Blackwellised PDA, 3 states, 100000 bursts x 64 resamples, 50 bins: "
-                        "expected histogram + chi2, then d chi2 / d(E, K)",
-            "forward_units_per_s": units / (np.mean(fwd) * 1e-3),
-            "forward_gpu_ms": float(np.mean(fwd)), "walk_gpu_ms": float(np.mean(walk)),
-            "adjoint_gpu_ms": float(np.mean(adj)), "jacobian_gpu_ms": float(np.mean(jac)),
-            "forward_plus_adjoint_units_per_s": units / ((np.mean(fwd) + np.mean(adj)) * 1e-3),
-            "chi2": c2}
+    out = {"workload": "Rao-Blackwellised PDA, 3 states, 100000 bursts x 64 resamples, 50 bins: "
+                       "expected histogram + chi2, then d chi2 / d(E, K)",
+           "forward_units_per_s": units / (np.mean(fwd) * 1e-3),
+           "forward_gpu_ms": float(np.mean(fwd)), "walk_gpu_ms": float(np.mean(walk)),
+           "adjoint_gpu_ms": float(np.mean(adj)), "jacobian_gpu_ms": float(np.mean(jac)),
+           "forward_plus_adjoint_units_per_s": units / ((np.mean(fwd) + np.mean(adj)) * 1e-3),
+           "chi2": c2}
+    # rooflines of the pmf passes: the algorithm's multiply-adds per unit
+    # (from each unit's photon counts per state, frozen in the cost model)
+    # over the pass time; a float multiply-add is two lane-ops (no
+    # contraction) and two 4-byte LDS reads, a double one two lane-ops
+    from gsmfretda_amd import cost_model
+    cm = cost_model.load()
+    w = cm["rb_work_per_unit"]
+    peak = cm["valu_peak_lane_ops_per_s"]
+    lds_peak = 256 * 128 * 2.4e9  # bytes/s: 128 B per clock per CU
+    pmf_fwd_ms = float(np.mean(fwd)) - float(np.mean(walk))
+    for name, ms, macs, lds_bytes in (
+            ("forward_pmf", pmf_fwd_ms, w["conv_macs"], 8.0 * w["conv_macs"]),
+            ("adjoint", float(np.mean(adj)), w["deconv_macs"], 4.0 * w["deconv_macs"]),
+            ("jacobian", float(np.mean(jac)), w["deconv_macs"], 4.0 * w["deconv_macs"])):
+        t = ms * 1e-3
+        out[f"roofline_{name}"] = {
+            "bound": "valu", "achieved": 2.0 * macs * units / t / 1e12, "peak": peak / 1e12,
+            "unit": "TOP/s", "frac": 2.0 * macs * units / t / peak, "traffic": None,
+            "lds_frac": lds_bytes * units / t / lds_peak, "macs_per_unit": macs,
+            "kernel_ms": ms}
+    return out
 
 
 def fast_population_leg(pda, cfg, steps):
@@ -299,9 +328,10 @@ def fit_leg(pda, cfg, limits):
     its full size (4 states, rates 1e3-1e5 s^-1, 1,000,000 bursts x 32
     resamples): the per-set genetic algorithm (one pda_run per individual on
     three stream slots in flight, the reference's pattern) against the
-    gradient fitters on the RB form, from the same 64 starting sets (the
-    gradient fitters start from the best of them, ranked by one timed
-    batched evaluation with 4 resamples per burst).  Each fitter stops at the
+    gradient fitters on the RB form, single-chain and batched (four chains
+    in lockstep through the RB population passes), from the same 64 starting
+    sets (the gradient fitters start from the best of them, ranked by one
+    timed batched evaluation with 4 resamples per burst).  Each fitter stops at the
     target or at its wall limit; a speed-up over a fitter that did not reach
     the target within its limit is a lower bound."""
     from gsmfretda_amd import fit, fit_study
@@ -312,13 +342,18 @@ def fit_leg(pda, cfg, limits):
     target = 1.01 * c_truth
     pop0 = [fit.args_to_theta(a, n) for _, a in cfg["population"][:64]]
     res = {}
+    note(f"fit: chi2(truth) {c_truth:.5g}, per-set genetic algorithm (limit {limits['ga-per-set']} s)")
     h = fit.GeneticFitter(batched=False, pop_size=64, rng=np.random.default_rng(7)).run(
         prob, pop0, limits["ga-per-set"], target=target)
     res["ga-per-set"] = h
     for name, make in (("adam-rb", lambda: fit.AdamFitter(lr=0.1, R_grad=8, check_every=5)),
-                       ("lm-rb", lambda: fit.LMFitter(R_grad=8))):
+                       ("lm-rb", lambda: fit.LMFitter(R_grad=8)),
+                       ("lm-rb-batched", lambda: fit.BatchedLMFitter(chains=4, R_grad=8)),
+                       ("adam-rb-batched",
+                        lambda: fit.BatchedAdamFitter(chains=4, lr=0.1, R_grad=8, check_every=5))):
+        note(f"fit: {name} (limit {limits[name]} s)")
         h, _ = fit.fit_from_population(make(), prob, pop0, limits[name], target=target,
-                                       rank_resamples=4, restarts=name == "lm-rb")
+                                       rank_resamples=4, restarts=name != "adam-rb")
         res[name] = h
     m.close()
     t = {k: v.time_to(target) for k, v in res.items()}
@@ -329,9 +364,9 @@ def fit_leg(pda, cfg, limits):
            "seconds_to_target": t, "final_chi2": {k: v.chi2[-1] for k, v in res.items()},
            "evaluations": {k: v.evals[-1] for k, v in res.items()}}
     ga = t["ga-per-set"]
-    for name in ("lm-rb", "adam-rb"):
+    for name in ("lm-rb", "adam-rb", "lm-rb-batched", "adam-rb-batched"):
         if t[name]:
-            out[f"speedup_{name.split('-')[0]}_vs_ga_per_set"] = (
+            out[f"speedup_{name.replace('-rb', '').replace('-', '_')}_vs_ga_per_set"] = (
                 ga if ga else limits["ga-per-set"]) / t[name]
     out["speedup_is_lower_bound"] = ga is None
     return out
@@ -346,7 +381,7 @@ def main():
     ap.add_argument("--no-population", action="store_true",
                     help="skip the batched-population leg")
     ap.add_argument("--no-fit", action="store_true", help="skip the fit wall-time leg")
-    ap.add_argument("--fit-seconds", type=float, default=60.0,
+    ap.add_argument("--fit-seconds", type=float, default=45.0,
                     help="wall limit of the genetic algorithm and Adam (LM: half)")
     ap.add_argument("--fit-bursts", type=int, default=1_000_000,
                     help="bursts of the fast-dynamics workload (fit and population legs)")
@@ -468,22 +503,30 @@ def main():
         units, float(np.mean(walk_ms)))
     out["walk_ms_in_flight_avg"] = float(np.mean(walk_in_flight))
     m.close()
+    note(f"headline: {value:.4g} burst-resamples/s")
     if not args.no_replay:
         out["replay"] = replay_leg(pda, cfg, max(3, args.steps // 4))
+        note("replay leg done")
     if not args.no_population:
         out["population"] = population_leg(pda, synth, max(1, args.steps // 4), 1)
+        note("population leg done")
     if not (args.no_fast and args.no_fit):
+        note(f"generating the fast-dynamics workload ({args.fit_bursts} bursts)")
         cfg4 = synth.config(4, n_bursts=args.fit_bursts)
         if not args.no_fast:
             out["fast_dynamics_population"] = fast_population_leg(pda, cfg4,
                                                                   max(2, args.steps // 4))
+            note("fast-dynamics population leg done")
         if not args.no_fit:
             lim = args.fit_seconds
             out["fit"] = fit_leg(pda, cfg4, {"ga-per-set": lim, "adam-rb": lim,
-                                             "lm-rb": lim / 2})
+                                             "lm-rb": lim / 2, "lm-rb-batched": lim / 2,
+                                             "adam-rb-batched": lim})
+            note("fit leg done")
         del cfg4
     if not args.no_rb:
         out["rao_blackwellised"] = rb_leg(pda, synth, max(2, args.steps // 4))
+        note("Rao-Blackwellised leg done")
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, threads, args.cpu_bursts)

@@ -150,9 +150,14 @@ int load_common(pda_ctx* ctx, const int64_t* start, const int64_t* stop, const i
     for (auto& s : ctx->slots) {
         s.begin = 0;
         s.end = -1;
-        s.rb.poff_b0 = -1;  // the RB pmf offsets follow the photon counts
-        s.rb.pmf_valid = false;
-        s.rb.ran = false;    // no derivative of a run over the previous data
+        auto reset = [](RbState& r) {
+            r.poff_b0 = -1;  // the RB pmf offsets follow the photon counts
+            r.pmf_valid = false;
+            r.ran = false;   // no derivative of a run over the previous data
+        };
+        reset(s.rb);
+        for (auto& r : s.rbpop) reset(r);
+        s.rbpop_sets = 0;
     }
     return PDA_OK;
 }
@@ -260,6 +265,7 @@ int pda_destroy(pda_ctx* ctx) {
         if (s.ev1) (void)hipEventDestroy(s.ev1);
         free_population(s.pop);
         free_rb(s.rb);
+        for (auto& r : s.rbpop) free_rb(r);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
     free_data(ctx);

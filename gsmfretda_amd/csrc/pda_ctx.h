@@ -91,6 +91,10 @@ struct Population {
 struct RbState {
     void* d_rbm = nullptr;             // RbModel
     void* h_rbm = nullptr;             // pinned
+    // the walk's model when the state is one set of an RB population (the
+    // single-set forward walks the slot's own d_blob)
+    ModelBlob* d_blob = nullptr;
+    ModelBlob* h_blob = nullptr;       // pinned
     uint2* d_cnt = nullptr;            // per unit: photon counts per state
     float* d_F = nullptr;              // per unit: F
     float* d_S = nullptr;              // per unit: scores of the rates
@@ -158,6 +162,8 @@ struct Slot {
     hipEvent_t ev0 = nullptr, ev_mid = nullptr, ev1 = nullptr;
     Population pop;
     RbState rb;
+    std::vector<RbState> rbpop;  // RB population: one state per set
+    int rbpop_sets = 0;          // sets of the last RB population forward
 };
 
 }  // namespace pda

@@ -27,6 +27,8 @@ namespace pda {
 void free_rb(RbState& r) {
     (void)hipFree(r.d_rbm);
     (void)hipHostFree(r.h_rbm);
+    (void)hipFree(r.d_blob);
+    (void)hipHostFree(r.h_blob);
     (void)hipFree(r.d_cnt);
     (void)hipFree(r.d_F);
     (void)hipFree(r.d_S);
@@ -177,32 +179,18 @@ void build_rb_model(const Model& M, RbModel& R, bool* rates_positive, bool* p_in
     }
 }
 
-}  // namespace
-
-extern "C" {
-
-int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int flags) {
-    Slot* s = slot_of(ctx, sid);
-    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
-    if (R < 1) return fail(PDA_E_ARG, "R = %d < 1", R);
-    if (!s->have_params) return fail(PDA_E_STATE, "stream %d: no parameters set", sid);
-    if (ctx->B < 1 || !ctx->d_obs) return fail(PDA_E_STATE, "no observed histogram set");
-    if (s->n > kRbMaxStates)
-        return fail(PDA_E_ARG, "the RB form takes n <= %d states (n = %d)", kRbMaxStates, s->n);
-    if (s->any_v)
-        return fail(PDA_E_ARG, "the RB form needs v = 0 (no distance broadening) in every state");
-    HIP_TRY(hipSetDevice(ctx->device));
-    RbState& r = s->rb;
+// Enqueues one RB forward of model M -- the walk reading M from the device
+// blob d_blob -- over the slot's burst range into state r, on the slot's
+// stream.  sync_model: first wait for the stream, whose last run may still be
+// copying r's pinned model (a population forward waits once for all sets).
+int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
+                       const ModelBlob* d_blob, int R, uint32_t seed, uint32_t set_id, int flags,
+                       bool sync_model) {
     int rc = ensure_rb(ctx, r);
     if (rc) return rc;
-    Model M;
-    {
-        std::string err;
-        rc = build_model(s->ph, s->n, s->args.data(), int(s->args.size()), M, err);
-        if (rc) return fail(rc, "%s", err.c_str());
-    }
+    const int n = M.n;
     int64_t b0, b1;
-    range_of(ctx, *s, &b0, &b1);
+    range_of(ctx, s, &b0, &b1);
     const int64_t units = (b1 - b0) * int64_t(R);
     if (units > int64_t(0x7FFFFFFF) - kWave)
         return fail(PDA_E_ARG, "%lld bursts x R = %d exceeds 2^31 units", (long long)(b1 - b0), R);
@@ -223,7 +211,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
 
     // buffers of the per-unit summaries
     if (size_t(units) > r.cap_units) {
-        HIP_TRY(hipStreamSynchronize(s->stream));
+        HIP_TRY(hipStreamSynchronize(s.stream));
         (void)hipFree(r.d_cnt);
         (void)hipFree(r.d_F);
         r.d_cnt = nullptr;
@@ -233,16 +221,16 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
         HIP_TRY(hipMalloc(&r.d_F, sizeof(float) * size_t(std::max<int64_t>(units, 1))));
         r.cap_units = size_t(units);
     }
-    const size_t nsc = scores ? size_t(units) * size_t(s->n * (s->n - 1)) : 0;
+    const size_t nsc = scores ? size_t(units) * size_t(n * (n - 1)) : 0;
     if (nsc > r.cap_scores) {
-        HIP_TRY(hipStreamSynchronize(s->stream));
+        HIP_TRY(hipStreamSynchronize(s.stream));
         (void)hipFree(r.d_S);
         r.d_S = nullptr;
         r.cap_scores = 0;
         HIP_TRY(hipMalloc(&r.d_S, sizeof(float) * nsc));
         r.cap_scores = nsc;
     }
-    HIP_TRY(hipStreamSynchronize(s->stream));  // the pinned model may still be in use
+    if (sync_model) HIP_TRY(hipStreamSynchronize(s.stream));  // the pinned model may be in use
     RbModel* hm = static_cast<RbModel*>(r.h_rbm);
     bool rates_positive = false, p_interior = false;
     build_rb_model(M, *hm, &rates_positive, &p_interior);
@@ -259,7 +247,7 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     a.clk_p = ctx->clk_p;
     a.bg_ad = ctx->bg_ad;
     a.bg_dd = ctx->bg_dd;
-    a.nstates = s->n;
+    a.nstates = n;
     a.nsets = 1;
     // walk grid: persistent, batches of about six grabs per wave
     int per_cu = 0;
@@ -294,26 +282,26 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
 
     float* pmf_io = nullptr;
     if (scores) {
-        int rc = ensure_pmf_store(ctx, r, b0, b1, R, hm->coef, s->stream);
+        int rc = ensure_pmf_store(ctx, r, b0, b1, R, hm->coef, s.stream);
         if (rc) return rc;
         pmf_io = r.d_pmf;  // null when the store did not fit: derivatives rebuild
     }
-    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
-    HIP_TRY(hipMemcpyAsync(r.d_rbm, r.h_rbm, sizeof(RbModel), hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipEventRecord(r.ev0, s->stream));
-    HIP_TRY(launch_rb_walk(scores, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob,
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
+    HIP_TRY(hipMemcpyAsync(r.d_rbm, r.h_rbm, sizeof(RbModel), hipMemcpyHostToDevice, s.stream));
+    HIP_TRY(hipEventRecord(r.ev0, s.stream));
+    HIP_TRY(launch_rb_walk(scores, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, d_blob,
                            static_cast<const RbModel*>(r.d_rbm), a, r.d_cnt, r.d_F, r.d_S,
-                           r.d_work, s->stream));
-    HIP_TRY(hipEventRecord(r.ev_walk, s->stream));
-    HIP_TRY(launch_rb_table(static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, r.d_pmf_tab, s->n,
-                            s->stream));
+                           r.d_work, s.stream));
+    HIP_TRY(hipEventRecord(r.ev_walk, s.stream));
+    HIP_TRY(launch_rb_table(static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, r.d_pmf_tab, n,
+                            s.stream));
     HIP_TRY(launch_rb_pmf(fwd_mode, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
                           r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, r.d_hist, nullptr,
-                          r.d_work, pmf_io, r.d_poff, s->stream));
-    HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s->stream));
-    HIP_TRY(hipEventRecord(r.ev1, s->stream));
+                          r.d_work, pmf_io, r.d_poff, s.stream));
+    HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s.stream));
+    HIP_TRY(hipEventRecord(r.ev1, s.stream));
     HIP_TRY(hipMemcpyAsync(r.h_res, r.d_res, sizeof(RbResultHeaderH) + sizeof(uint64_t) * size_t(a.B),
-                           hipMemcpyDeviceToHost, s->stream));
+                           hipMemcpyDeviceToHost, s.stream));
     r.ran = true;
     r.adj_ran = false;
     r.scores = scores;
@@ -321,8 +309,8 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     r.R = R;
     r.B = a.B;
     r.left_pad = a.left_pad;
-    r.n = s->n;
-    r.K = s->n * (s->n - 1);
+    r.n = n;
+    r.K = n * (n - 1);
     r.cap = cap;
     r.b0 = b0;
     r.b1 = b1;
@@ -332,6 +320,30 @@ int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id,
     r.rates_positive = rates_positive;
     r.p_interior = p_interior;
     return PDA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pda_rb_forward(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int flags) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    if (R < 1) return fail(PDA_E_ARG, "R = %d < 1", R);
+    if (!s->have_params) return fail(PDA_E_STATE, "stream %d: no parameters set", sid);
+    if (ctx->B < 1 || !ctx->d_obs) return fail(PDA_E_STATE, "no observed histogram set");
+    if (s->n > kRbMaxStates)
+        return fail(PDA_E_ARG, "the RB form takes n <= %d states (n = %d)", kRbMaxStates, s->n);
+    if (s->any_v)
+        return fail(PDA_E_ARG, "the RB form needs v = 0 (no distance broadening) in every state");
+    HIP_TRY(hipSetDevice(ctx->device));
+    Model M;
+    {
+        std::string err;
+        int rc = build_model(s->ph, s->n, s->args.data(), int(s->args.size()), M, err);
+        if (rc) return fail(rc, "%s", err.c_str());
+    }
+    return rb_enqueue_forward(ctx, *s, s->rb, M, s->d_blob, R, seed, set_id, flags, true);
 }
 
 int pda_rb_get_result(pda_ctx* ctx, int sid, uint64_t* M_fixed, double* M, int nbins,
@@ -394,6 +406,66 @@ int derivative_pass(pda_ctx* ctx, const RbState& r, int mode, RunArgs& a, uint32
     return PDA_OK;
 }
 
+// Enqueues the adjoint pass of state r's last forward for the cotangent g
+// (nbins entries; r's pinned buffers must be free: the caller waited for the
+// stream), its gradient copied to r.h_grad.
+int rb_enqueue_adjoint(pda_ctx* ctx, Slot& s, RbState& r, const double* g) {
+    std::memcpy(r.h_g, g, sizeof(double) * size_t(r.B));
+    RunArgs a;
+    uint32_t pgrid = 0;
+    if (int rc = derivative_pass(ctx, r, 1, a, pgrid)) return rc;
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
+    HIP_TRY(hipMemsetAsync(r.d_grad, 0, sizeof(double) * kGradLen, s.stream));
+    HIP_TRY(hipMemcpyAsync(r.d_g, r.h_g, sizeof(double) * size_t(r.B), hipMemcpyHostToDevice,
+                           s.stream));
+    HIP_TRY(hipEventRecord(r.ev_a0, s.stream));
+    HIP_TRY(launch_rb_pmf(1, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
+                          r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, r.d_g, nullptr,
+                          r.d_grad, r.d_work, r.pmf_valid ? r.d_pmf : nullptr, r.d_poff, s.stream));
+    HIP_TRY(hipEventRecord(r.ev_a1, s.stream));
+    HIP_TRY(hipMemcpyAsync(r.h_grad, r.d_grad, sizeof(double) * kGradLen, hipMemcpyDeviceToHost,
+                           s.stream));
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
+    r.adj_ran = true;
+    return PDA_OK;
+}
+
+// The adjoint's gradient in mc::set_params layout (after the stream finished).
+void rb_collect_adjoint(const RbState& r, double* grad, int nargs) {
+    const int n = r.n, K = r.K;
+    for (int i = 0; i < n; ++i) grad[i] = r.h_grad[i];            // d chi2 / dE_i (dp/dE = 1)
+    for (int i = 0; i < K; ++i) grad[n + i] = r.h_grad[n + i];    // rates, args order
+    for (int i = n + K; i < nargs; ++i) grad[i] = 0.0;            // v (held at 0)
+}
+
+// Enqueues the Jacobian pass of state r's last forward, copied to r.h_jac.
+int rb_enqueue_jacobian(pda_ctx* ctx, Slot& s, RbState& r) {
+    RunArgs a;
+    uint32_t pgrid = 0;
+    if (int rc = derivative_pass(ctx, r, 2, a, pgrid)) return rc;
+    const size_t bytes = sizeof(double) * kGradLen * size_t(r.B);
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
+    HIP_TRY(hipMemsetAsync(r.d_jac, 0, bytes, s.stream));
+    HIP_TRY(hipEventRecord(r.ev_a0, s.stream));
+    HIP_TRY(launch_rb_pmf(2, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
+                          r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, nullptr,
+                          r.d_jac, r.d_work, r.pmf_valid ? r.d_pmf : nullptr, r.d_poff, s.stream));
+    HIP_TRY(hipEventRecord(r.ev_a1, s.stream));
+    HIP_TRY(hipMemcpyAsync(r.h_jac, r.d_jac, bytes, hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
+    r.adj_ran = true;
+    return PDA_OK;
+}
+
+// The Jacobian rows J[b * nargs + i] (after the stream finished).
+void rb_collect_jacobian(const RbState& r, double* J, int nbins, int nargs) {
+    const int w = r.n + r.K;
+    for (int b = 0; b < nbins; ++b) {
+        for (int i = 0; i < w; ++i) J[size_t(b) * size_t(nargs) + size_t(i)] = r.h_jac[size_t(b) * kGradLen + size_t(i)];
+        for (int i = w; i < nargs; ++i) J[size_t(b) * size_t(nargs) + size_t(i)] = 0.0;  // v
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -401,66 +473,147 @@ extern "C" {
 int pda_rb_adjoint(pda_ctx* ctx, int sid, const double* g, int nbins, double* grad, int nargs) {
     Slot* s = slot_of(ctx, sid);
     if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
-    RbState& r = s->rb;
     if (!g || !grad) return fail(PDA_E_ARG, "null argument");
-    if (int rc = check_derivative_run(r, sid, nbins, nargs)) return rc;
+    if (int rc = check_derivative_run(s->rb, sid, nbins, nargs)) return rc;
     for (int b = 0; b < nbins; ++b)
         if (!std::isfinite(g[b])) return fail(PDA_E_ARG, "g[%d] not finite", b);
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    std::memcpy(r.h_g, g, sizeof(double) * size_t(nbins));
-    RunArgs a;
-    uint32_t pgrid = 0;
-    if (int rc = derivative_pass(ctx, r, 1, a, pgrid)) return rc;
-    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
-    HIP_TRY(hipMemsetAsync(r.d_grad, 0, sizeof(double) * kGradLen, s->stream));
-    HIP_TRY(hipMemcpyAsync(r.d_g, r.h_g, sizeof(double) * size_t(nbins), hipMemcpyHostToDevice,
-                           s->stream));
-    HIP_TRY(hipEventRecord(r.ev_a0, s->stream));
-    HIP_TRY(launch_rb_pmf(1, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
-                          r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, r.d_g, nullptr, r.d_grad,
-                          r.d_work, r.pmf_valid ? r.d_pmf : nullptr, r.d_poff, s->stream));
-    HIP_TRY(hipEventRecord(r.ev_a1, s->stream));
-    HIP_TRY(hipMemcpyAsync(r.h_grad, r.d_grad, sizeof(double) * kGradLen, hipMemcpyDeviceToHost,
-                           s->stream));
-    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
+    if (int rc = rb_enqueue_adjoint(ctx, *s, s->rb, g)) return rc;
     HIP_TRY(hipStreamSynchronize(s->stream));
-    const int n = r.n, K = r.K;
-    for (int i = 0; i < n; ++i) grad[i] = r.h_grad[i];            // d chi2 / dE_i (dp/dE = 1)
-    for (int i = 0; i < K; ++i) grad[n + i] = r.h_grad[n + i];    // rates, args order
-    for (int i = n + K; i < nargs; ++i) grad[i] = 0.0;            // v (held at 0)
-    r.adj_ran = true;
+    rb_collect_adjoint(s->rb, grad, nargs);
     return PDA_OK;
 }
 
 int pda_rb_jacobian(pda_ctx* ctx, int sid, double* J, int nbins, int nargs) {
     Slot* s = slot_of(ctx, sid);
     if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
-    RbState& r = s->rb;
     if (!J) return fail(PDA_E_ARG, "null argument");
-    if (int rc = check_derivative_run(r, sid, nbins, nargs)) return rc;
+    if (int rc = check_derivative_run(s->rb, sid, nbins, nargs)) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    RunArgs a;
-    uint32_t pgrid = 0;
-    if (int rc = derivative_pass(ctx, r, 2, a, pgrid)) return rc;
-    const size_t bytes = sizeof(double) * kGradLen * size_t(r.B);
-    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
-    HIP_TRY(hipMemsetAsync(r.d_jac, 0, bytes, s->stream));
-    HIP_TRY(hipEventRecord(r.ev_a0, s->stream));
-    HIP_TRY(launch_rb_pmf(2, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
-                          r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, nullptr,
-                          r.d_jac, r.d_work, r.pmf_valid ? r.d_pmf : nullptr, r.d_poff, s->stream));
-    HIP_TRY(hipEventRecord(r.ev_a1, s->stream));
-    HIP_TRY(hipMemcpyAsync(r.h_jac, r.d_jac, bytes, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s->stream));
+    if (int rc = rb_enqueue_jacobian(ctx, *s, s->rb)) return rc;
     HIP_TRY(hipStreamSynchronize(s->stream));
-    const int w = r.n + r.K;
-    for (int b = 0; b < nbins; ++b) {
-        for (int i = 0; i < w; ++i) J[size_t(b) * size_t(nargs) + size_t(i)] = r.h_jac[size_t(b) * kGradLen + size_t(i)];
-        for (int i = w; i < nargs; ++i) J[size_t(b) * size_t(nargs) + size_t(i)] = 0.0;  // v
+    rb_collect_jacobian(s->rb, J, nbins, nargs);
+    return PDA_OK;
+}
+
+// ---- populations -------------------------------------------------------------
+
+int pda_rb_forward_population(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_base,
+                              uint32_t set_stride, int flags) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    Population& p = s->pop;
+    if (p.nsets < 1) return fail(PDA_E_STATE, "no population set");
+    if (R < 1) return fail(PDA_E_ARG, "R = %d < 1", R);
+    if (ctx->B < 1 || !ctx->d_obs) return fail(PDA_E_STATE, "no observed histogram set");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<Model> models(size_t(p.nsets));
+    for (int k = 0; k < p.nsets; ++k) {
+        const int n = p.n[size_t(k)];
+        if (n > kRbMaxStates)
+            return fail(PDA_E_ARG, "set %d: the RB form takes n <= %d states (n = %d)", k,
+                        kRbMaxStates, n);
+        std::string err;
+        int rc = build_model(s->ph, n, p.args.data() + p.off[size_t(k)], n * (n + 1),
+                             models[size_t(k)], err);
+        if (rc) return fail(rc, "set %d: %s", k, err.c_str());
+        if (models[size_t(k)].any_v)
+            return fail(PDA_E_ARG, "set %d: the RB form needs v = 0 in every state", k);
     }
-    r.adj_ran = true;
+    // every set's pinned model may still be in use by the previous run
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->rbpop.size() < size_t(p.nsets)) s->rbpop.resize(size_t(p.nsets));
+    for (int k = 0; k < p.nsets; ++k) {
+        RbState& r = s->rbpop[size_t(k)];
+        if (!r.d_blob) HIP_TRY(hipMalloc(reinterpret_cast<void**>(&r.d_blob), sizeof(ModelBlob)));
+        if (!r.h_blob)
+            HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.h_blob), sizeof(ModelBlob),
+                                  hipHostMallocDefault));
+        r.h_blob->m = models[size_t(k)];
+        HIP_TRY(hipMemcpyAsync(r.d_blob, r.h_blob, sizeof(Model), hipMemcpyHostToDevice,
+                               s->stream));
+        int rc = rb_enqueue_forward(ctx, *s, r, models[size_t(k)], r.d_blob, R, seed,
+                                    set_base + uint32_t(k) * set_stride, flags, false);
+        if (rc) {
+            s->rbpop_sets = 0;
+            return fail(rc, "set %d: %s", k, last_error().c_str());
+        }
+    }
+    s->rbpop_sets = p.nsets;
+    return PDA_OK;
+}
+
+int pda_rb_get_population_result(pda_ctx* ctx, int sid, uint64_t* M_fixed, double* M, int nbins,
+                                 double* chi2, int32_t* effN) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    if (s->rbpop_sets < 1) return fail(PDA_E_STATE, "stream %d has no RB population run", sid);
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < s->rbpop_sets; ++k) {
+        const RbState& r = s->rbpop[size_t(k)];
+        if ((M_fixed || M) && nbins != r.B) return fail(PDA_E_ARG, "nbins %d != run's %d", nbins, r.B);
+        const RbResultHeaderH* h = reinterpret_cast<const RbResultHeaderH*>(r.h_res);
+        const uint64_t* m = reinterpret_cast<const uint64_t*>(h + 1);
+        for (int b = 0; b < r.B; ++b) {
+            if (M_fixed) M_fixed[size_t(k) * size_t(nbins) + size_t(b)] = m[b];
+            if (M) M[size_t(k) * size_t(nbins) + size_t(b)] = double(m[b]) * 0x1p-32;
+        }
+        if (chi2) chi2[k] = h->chi2;
+        if (effN) effN[k] = h->effN;
+    }
+    return PDA_OK;
+}
+
+int pda_rb_adjoint_population(pda_ctx* ctx, int sid, const double* g, int nbins, double* grad,
+                              int nargs) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    if (!g || !grad) return fail(PDA_E_ARG, "null argument");
+    if (s->rbpop_sets < 1) return fail(PDA_E_STATE, "stream %d has no RB population run", sid);
+    const int K = s->rbpop_sets;
+    for (int k = 0; k < K; ++k) {
+        if (int rc = check_derivative_run(s->rbpop[size_t(k)], sid, nbins, nargs))
+            return fail(rc, "set %d: %s", k, last_error().c_str());
+        for (int b = 0; b < nbins; ++b)
+            if (!std::isfinite(g[size_t(k) * size_t(nbins) + size_t(b)]))
+                return fail(PDA_E_ARG, "set %d: g[%d] not finite", k, b);
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < K; ++k)
+        if (int rc = rb_enqueue_adjoint(ctx, *s, s->rbpop[size_t(k)], g + size_t(k) * size_t(nbins)))
+            return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < K; ++k)
+        rb_collect_adjoint(s->rbpop[size_t(k)], grad + size_t(k) * size_t(nargs), nargs);
+    return PDA_OK;
+}
+
+int pda_rb_jacobian_population(pda_ctx* ctx, int sid, const int32_t* which, double* J, int nbins,
+                               int nargs) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    if (!J) return fail(PDA_E_ARG, "null argument");
+    if (s->rbpop_sets < 1) return fail(PDA_E_STATE, "stream %d has no RB population run", sid);
+    const int K = s->rbpop_sets;
+    for (int k = 0; k < K; ++k)
+        if (!which || which[k])
+            if (int rc = check_derivative_run(s->rbpop[size_t(k)], sid, nbins, nargs))
+                return fail(rc, "set %d: %s", k, last_error().c_str());
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < K; ++k)
+        if (!which || which[k])
+            if (int rc = rb_enqueue_jacobian(ctx, *s, s->rbpop[size_t(k)])) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const size_t per = size_t(nbins) * size_t(nargs);
+    for (int k = 0; k < K; ++k) {
+        if (!which || which[k]) rb_collect_jacobian(s->rbpop[size_t(k)], J + size_t(k) * per, nbins, nargs);
+        else std::memset(J + size_t(k) * per, 0, sizeof(double) * per);
+    }
     return PDA_OK;
 }
 

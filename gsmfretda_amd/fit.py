@@ -416,3 +416,187 @@ class LMFitter:
                 cap *= 0.5
         if last_recorded is not theta:
             hist.record(time.perf_counter() - t0, prob.chi2(theta), prob.evaluations, theta)
+
+
+def rb_cotangent(M, observed, R, n, left_pad=2):
+    """d chi2 / d M_b of the reduced chi2 of the expected counts M (summed
+    over R resamples): (1 - O_b^2 / m_b^2) / (R dof) on the kept bins
+    (b >= left_pad, M_b > 0), m_b = M_b / R."""
+    m = M / R
+    keep = (np.arange(len(M)) >= left_pad) & (M > 0)
+    dof = keep.sum() - n * (n + 1)
+    g = np.zeros(len(M))
+    g[keep] = (1.0 - observed[keep] ** 2 / m[keep] ** 2) / (R * dof)
+    return g
+
+
+class BatchedLMFitter(LMFitter):
+    """Levenberg-Marquardt chains in lockstep, several starting points at
+    once: every iteration evaluates the candidate step of every chain in one
+    RB population forward (pda_rb_forward_population) and the Jacobians of
+    the chains whose step was accepted in one population Jacobian pass
+    (pda_rb_jacobian_population).  A chain whose descent converges short of
+    the target restarts from the next point of `restarts`.  The history
+    keeps the best plain chi2 of all chains (one batched plain evaluation of
+    the improved chains per iteration)."""
+
+    def __init__(self, chains=4, **kw):
+        super().__init__(**kw)
+        self.chains = chains
+
+    def _forward_population(self, prob, thetas, R):
+        n = prob.n
+        prob.mc.set_population([n] * len(thetas), [theta_to_args(t, n) for t in thetas])
+        prob.mc.rb_forward_population(R=R, seed=prob.seed, set_base=0, set_stride=0)
+        M, _, c2, _ = prob.mc.rb_population_result()
+        return M, c2
+
+    def _system_from(self, prob, theta, M, J, R):
+        n = prob.n
+        J = J[:, :n * n]
+        args = theta_to_args(theta, n).astype(np.float64)
+        E = args[:n]
+        dargs = np.concatenate([E * (1.0 - E), args[n:n * n]])
+        O = prob.observed
+        m = M / R
+        keep = (np.arange(len(M)) >= self.left_pad) & (M > 0)
+        r = np.zeros(len(M))
+        r[keep] = (O[keep] - m[keep]) / np.sqrt(m[keep])
+        dr = np.zeros(len(M))
+        dr[keep] = -(O[keep] + m[keep]) / (2.0 * R * m[keep] ** 1.5)
+        Jr = (dr[:, None] * J) * dargs[None, :]
+        return Jr.T @ Jr, Jr.T @ r
+
+    def run(self, prob, theta0, max_seconds, target=None, max_steps=100000, restarts=()):
+        n = prob.n
+        R = self.R_grad or prob.R
+        hist = History("lm-rb-batched")
+        t0 = time.perf_counter()
+        pending = [clip_theta(t, n) for t in [theta0, *restarts]]
+        K = min(self.chains, len(pending))
+        theta = [pending.pop(0) for _ in range(K)]
+        fresh = [True] * K                 # evaluated at theta, not yet a step
+        c2 = [np.inf] * K
+        M = [None] * K
+        A = [None] * K
+        g = [None] * K
+        lam = [self.lam0] * K
+        cap = [self.step_max] * K
+        alive = [True] * K
+        c0 = prob.chi2_population(theta)
+        for k in range(K):
+            hist.record(time.perf_counter() - t0, float(c0[k]), prob.evaluations, theta[k])
+        for _ in range(max_steps):
+            if target is not None and hist.chi2[-1] <= target:
+                break
+            if time.perf_counter() - t0 > max_seconds or not any(alive):
+                break
+            cand = []
+            for k in range(K):
+                if fresh[k] or not alive[k]:
+                    cand.append(theta[k])
+                    continue
+                dA = np.diag(A[k])
+                D = np.diag(np.maximum(dA, 1e-3 * max(dA.max(), 1e-300)))
+                try:
+                    delta = np.linalg.solve(A[k] + lam[k] * D, -g[k])
+                except np.linalg.LinAlgError:
+                    lam[k] *= 4.0
+                    cand.append(theta[k])
+                    continue
+                big = np.abs(delta).max()
+                if big > cap[k]:
+                    delta *= cap[k] / big
+                cand.append(clip_theta(theta[k] + delta, n))
+            Mc, cc = self._forward_population(prob, cand, R)
+            prob.evaluations += K
+            accepted = np.zeros(K, np.int32)
+            for k in range(K):
+                if not alive[k]:
+                    continue
+                if fresh[k] or (np.isfinite(cc[k]) and cc[k] < c2[k]):
+                    accepted[k] = 1
+                    if not fresh[k]:
+                        lam[k] = max(lam[k] / 3.0, 1e-12)
+                        cap[k] = min(2.0 * cap[k], 2.0)
+                    theta[k], M[k], c2[k] = cand[k], Mc[k], cc[k]
+                    fresh[k] = False
+                else:
+                    lam[k] *= 4.0
+                    cap[k] *= 0.5
+                    if lam[k] > self.lam_max or cap[k] < self.step_min:
+                        # converged short of the target: the next start
+                        if pending:
+                            theta[k] = pending.pop(0)
+                            fresh[k], c2[k] = True, np.inf
+                            lam[k], cap[k] = self.lam0, self.step_max
+                        else:
+                            alive[k] = False
+            if accepted.any():
+                J = prob.mc.rb_jacobian_population(n, len(Mc[0]), which=accepted)
+                for k in np.flatnonzero(accepted):
+                    A[k], g[k] = self._system_from(prob, theta[k], M[k], J[k], R)
+                improved = [theta[k] for k in np.flatnonzero(accepted)]
+                cp = prob.chi2_population(improved)
+                for th, c in zip(improved, cp):
+                    hist.record(time.perf_counter() - t0, float(c), prob.evaluations, th)
+        return hist
+
+
+class BatchedAdamFitter:
+    """Adam chains in lockstep from several starting points: every step one
+    RB population forward with scores and one population adjoint
+    (pda_rb_adjoint_population) give the gradients of every chain; Adam on
+    (logit E, log K) per chain; every `check_every` steps one batched plain
+    evaluation of all chains records the best plain chi2."""
+
+    def __init__(self, chains=4, lr=0.1, betas=(0.9, 0.999), eps=1e-8, check_every=5,
+                 R_grad=None, left_pad=2):
+        self.chains = chains
+        self.lr = lr
+        self.betas = betas
+        self.eps = eps
+        self.check_every = check_every
+        self.R_grad = R_grad
+        self.left_pad = left_pad
+
+    def run(self, prob, theta0, max_seconds, target=None, max_steps=100000, restarts=()):
+        n = prob.n
+        R = self.R_grad or prob.R
+        hist = History("adam-rb-batched")
+        t0 = time.perf_counter()
+        starts = [clip_theta(t, n) for t in [theta0, *restarts]]
+        K = min(self.chains, len(starts))
+        theta = np.array(starts[:K])
+        m = np.zeros_like(theta)
+        v = np.zeros_like(theta)
+        b1, b2 = self.betas
+        c0 = prob.chi2_population(list(theta))
+        for k in range(K):
+            hist.record(time.perf_counter() - t0, float(c0[k]), prob.evaluations, theta[k])
+        for step in range(max_steps):
+            if target is not None and hist.chi2[-1] <= target:
+                break
+            if time.perf_counter() - t0 > max_seconds:
+                break
+            prob.mc.set_population([n] * K, [theta_to_args(t, n) for t in theta])
+            prob.mc.rb_forward_population(R=R, seed=prob.seed, set_base=0, set_stride=0)
+            M, _, _, _ = prob.mc.rb_population_result()
+            gM = np.array([rb_cotangent(M[k], prob.observed, R, n, self.left_pad)
+                           for k in range(K)])
+            ga = prob.mc.rb_adjoint_population(gM, n)
+            prob.evaluations += K
+            args = np.array([theta_to_args(t, n) for t in theta], np.float64)
+            E = args[:, :n]
+            grad = np.concatenate([ga[:, :n] * E * (1.0 - E), ga[:, n:n * n] * args[:, n:n * n]],
+                                  axis=1)
+            m = b1 * m + (1 - b1) * grad
+            v = b2 * v + (1 - b2) * grad ** 2
+            mh = m / (1 - b1 ** (step + 1))
+            vh = v / (1 - b2 ** (step + 1))
+            theta = np.array([clip_theta(t, n) for t in theta - self.lr * mh / (np.sqrt(vh) + self.eps)])
+            if (step + 1) % self.check_every == 0:
+                cp = prob.chi2_population(list(theta))
+                for k in range(K):
+                    hist.record(time.perf_counter() - t0, float(cp[k]), prob.evaluations, theta[k])
+        return hist

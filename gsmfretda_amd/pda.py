@@ -39,7 +39,9 @@ EXPORTS = (
     "pda_set_launch_shape", "pda_get_launch_shape", "pda_autotune",
     "pda_set_population", "pda_run_population", "pda_get_population_result",
     "pda_population_stats", "pda_rb_forward", "pda_rb_get_result", "pda_rb_adjoint",
-    "pda_rb_jacobian", "pda_rb_stats", "pda_run_noalloc", "pda_debug_primitive",
+    "pda_rb_jacobian", "pda_rb_stats", "pda_rb_forward_population",
+    "pda_rb_get_population_result", "pda_rb_adjoint_population", "pda_rb_jacobian_population",
+    "pda_run_noalloc", "pda_debug_primitive",
     "pda_debug_microbench",
 )
 
@@ -100,6 +102,15 @@ def load_library(path=LIB_PATH):
     L.pda_last_run_ms.argtypes = [_vp, ctypes.c_int, _f32p]
     L.pda_last_pass_ms.argtypes = [_vp, ctypes.c_int, _f32p, _f32p]
     L.pda_debug_primitive.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int64]
+    L.pda_rb_forward_population.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+    L.pda_rb_get_population_result.argtypes = [_vp, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_uint64), _f64p,
+                                               ctypes.c_int, _f64p, _i32p]
+    L.pda_rb_adjoint_population.argtypes = [_vp, ctypes.c_int, _f64p, ctypes.c_int, _f64p,
+                                            ctypes.c_int]
+    L.pda_rb_jacobian_population.argtypes = [_vp, ctypes.c_int, _i32p, _f64p, ctypes.c_int,
+                                             ctypes.c_int]
     L.pda_run_noalloc.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                   ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int64]
     L.pda_debug_microbench.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
@@ -454,6 +465,47 @@ class MC:
         mc::set_params layout; g @ J equals rb_adjoint(g)."""
         J = np.zeros((nbins, n * (n + 1)), np.float64)
         _check(self._L.pda_rb_jacobian(self._h, sid, _p(J, _f64p), nbins, J.shape[1]))
+        return J
+
+    def rb_forward_population(self, R=None, seed=None, set_base=0, set_stride=1, sid=0,
+                              scores=True):
+        """Asynchronous RB forward of every set of the slot's population;
+        set k uses the key (seed, set_base + k * set_stride)."""
+        R = self.reSampleTimes if R is None else R
+        seed = self.seed if seed is None else seed
+        _check(self._L.pda_rb_forward_population(self._h, sid, R, seed, set_base, set_stride,
+                                                 PDA_RB_SCORES if scores else 0))
+        self._rb_runs[sid] += 1
+
+    def rb_population_result(self, sid=0, nbins=None):
+        """(M [nsets, B] float64, the same in 2^-32 units, chi2 [nsets], effN [nsets])."""
+        B = self._hist_bins if nbins is None else nbins
+        S = self._pop_sets
+        fixed = np.zeros((S, B), np.uint64)
+        M = np.zeros((S, B), np.float64)
+        c2 = np.zeros(S, np.float64)
+        eff = np.zeros(S, np.int32)
+        _check(self._L.pda_rb_get_population_result(
+            self._h, sid, fixed.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), _p(M, _f64p), B,
+            _p(c2, _f64p), _p(eff, _i32p)))
+        return M, fixed, c2, eff
+
+    def rb_adjoint_population(self, g, n, sid=0):
+        """d chi2 / d args of every set for cotangents g [nsets, B]: [nsets, n (n + 1)]."""
+        g = np.ascontiguousarray(g, np.float64)
+        out = np.zeros((g.shape[0], n * (n + 1)), np.float64)
+        _check(self._L.pda_rb_adjoint_population(self._h, sid, _p(g, _f64p), g.shape[1],
+                                                 _p(out, _f64p), n * (n + 1)))
+        return out
+
+    def rb_jacobian_population(self, n, nbins, which=None, sid=0):
+        """d M_b / d args of the sets of the last RB population forward:
+        [nsets, nbins, n (n + 1)] (rows of sets with which[k] == 0 are zero)."""
+        S = self._pop_sets
+        J = np.zeros((S, nbins, n * (n + 1)), np.float64)
+        w = None if which is None else np.ascontiguousarray(which, np.int32)
+        _check(self._L.pda_rb_jacobian_population(self._h, sid, None if w is None else _p(w, _i32p),
+                                                  _p(J, _f64p), nbins, n * (n + 1)))
         return J
 
     def rb_stats(self, sid=0):

@@ -240,6 +240,36 @@ int pda_rb_adjoint(pda_ctx *ctx, int sid, const double *g, int nbins, double *gr
  * pmf buffers. */
 int pda_rb_jacobian(pda_ctx *ctx, int sid, double *J, int nbins, int nargs);
 
+/* RB population (no reference counterpart; the reference evaluates one set
+ * per mc::run_kernel and differentiates none): the RB forward of every set of
+ * the slot's population (pda_set_population; every set n <= 4, v = 0), set k
+ * with the key (seed, set_base + k * set_stride), each bit-identical to
+ * pda_rb_forward of that set with that set id.  Every set keeps its own walk
+ * summaries (and, with PDA_RB_SCORES, its own scores and pmf store), so
+ * pda_rb_adjoint_population / pda_rb_jacobian_population differentiate all
+ * of them; the passes of all sets are enqueued back to back on the slot's
+ * stream and waited for once.  Asynchronous. */
+int pda_rb_forward_population(pda_ctx *ctx, int sid, int R, uint32_t seed, uint32_t set_base,
+                              uint32_t set_stride, int flags);
+
+/* Results of the slot's last RB population forward (waits for it): M and M
+ * in 2^-32 units as [nsets][nbins], chi2 and bins kept per set.  Outputs may
+ * be NULL. */
+int pda_rb_get_population_result(pda_ctx *ctx, int sid, uint64_t *M_fixed, double *M, int nbins,
+                                 double *chi2, int32_t *effN);
+
+/* Adjoint of every set of the last RB population forward (blocking): for
+ * cotangents g[nsets][nbins], grad[nsets][nargs] as pda_rb_adjoint; every set
+ * has the same nargs. */
+int pda_rb_adjoint_population(pda_ctx *ctx, int sid, const double *g, int nbins, double *grad,
+                              int nargs);
+
+/* Jacobians of the sets of the last RB population forward (blocking):
+ * J[nsets][nbins][nargs] as pda_rb_jacobian, for the sets with which[k] != 0
+ * (which may be NULL: every set); the others' rows are zero. */
+int pda_rb_jacobian_population(pda_ctx *ctx, int sid, const int32_t *which, double *J, int nbins,
+                               int nargs);
+
 /* GPU milliseconds of the slot's last RB runs: the walk, the whole forward
  * sequence, the last adjoint or Jacobian kernel (0 before the first).
  * Outputs may be NULL. */

@@ -90,6 +90,46 @@ def events_model(procs):
     return out
 
 
+def rb_model(bursts=2000):
+    """Per-unit work of the Rao-Blackwellised kernels on the 3-state workload
+    (100,000 bursts x 64; the first `bursts` bursts), from each unit's
+    photon counts per state m_s (the oracle's rb_unit):
+      forward: the pmf of the acceptor total is the chain of convolutions of
+        Binomial(m_s) over the states with m_s > 0 -- the first is a copy,
+        each later one of a pmf of length l with one of length m_s + 1 costs
+        l (m_s + 1) multiply-adds (float, two LDS reads each); then one
+        histogram update per outcome (Mt + 1 of them, Mt = sum m_s);
+      adjoint / Jacobian: per state with m_s > 0 the pmf without one trial of
+        that state (a first-order recurrence over the Mt outcomes) and its
+        contraction with the cotangent differences (Mt more), plus the
+        cotangent-weighted sum of the pmf (Mt + 1): multiply-adds in double."""
+    from gsmfretda_amd import synth
+    from oracle import oracle as O
+    cfg = synth.config(3)
+    od = O.Data.from_bursts(cfg["data"])
+    p = O.make_params(cfg["n"], cfg["args"], *cfg["photophysics"])
+    R, seed = cfg["R"], cfg["seeds"]["mc"]
+    macs = outcomes = dmacs = entries = 0
+    units = 0
+    for b in range(bursts):
+        for r in range(R):
+            cnt, F, _ = O.rb_unit(od, p, b, r, seed, 0)
+            ms = [int(m) for m in cnt if m > 0]
+            mt = sum(ms)
+            length = 1
+            for i, m in enumerate(ms):
+                entries += m + 1
+                if i > 0:
+                    macs += length * (m + 1)
+                length += m
+            outcomes += mt + 1
+            dmacs += (mt + 1) + 2 * mt * len(ms)
+            units += 1
+    return {"conv_macs": macs / units, "pmf_entries": entries / units,
+            "outcomes": outcomes / units, "deconv_macs": dmacs / units,
+            "sample": f"bursts [0, {bursts}) of 100000 x {R}, the true 3-state set, {units} units"}
+
+
 def _slope(m, op, iters, param=0.0):
     """(lane-issue-cycles per event, mean inner iterations per event)."""
     ms1, e1, i1 = m.debug_microbench(op, iters, param)
@@ -126,6 +166,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--events", action="store_true")
     ap.add_argument("--weights", action="store_true")
+    ap.add_argument("--rb", action="store_true")
     ap.add_argument("--procs", type=int, default=8)
     ap.add_argument("--print", action="store_true", help="print instead of writing the model")
     a = ap.parse_args()
@@ -135,6 +176,10 @@ def main():
         cm["events_source"] = ("oracle event counters (or_stats, from `units` on) on the "
                                "workloads' first bursts: python tests/golden/make_cost_model.py "
                                "--events")
+    if a.rb:
+        cm["rb_work_per_unit"] = rb_model()
+        cm["rb_work_source"] = ("per-state photon counts of each unit (oracle rb_unit): python "
+                                "tests/golden/make_cost_model.py --rb")
     if a.weights:
         w, raw = weights_model()
         cm["weights_lane_cycles"] = w

@@ -143,6 +143,37 @@ def test_read_bursts_evaluate_like_the_originals(oracle, tmp_path):
 
 
 @pytest.mark.gpu
+def test_hdf5_bursts_on_the_gpu_equal_the_oracle_on_the_originals(gpu_mc, oracle, tmp_path):
+    """Bursts written in the reference's schema, read back and loaded into
+    the GPU: every unit's acceptor count, corrected count and E, the
+    histogram and chi2 equal the oracle's run on the original in-memory
+    arrays (src/loadHdf5.cpp:37-74, 101-128)."""
+    from tests.conftest import bits
+    d, sg = _bursts(500, seed=21)
+    p = str(tmp_path / "f2.hdf5")
+    hdf5.write_bursts(p, d, sg)
+    n, args, ph, R, seed = 3, synth.make_args([0.2, 0.5, 0.8], [700, 2500, 900, 3000, 1200, 4000]), \
+        synth.PHOTOPHYSICS, 24, 606
+    gpu_mc.load_hdf5(p, nbins=50, left_pad=2)
+    gpu_mc.set_photophysics(0, *ph)
+    assert gpu_mc.set_params(n, 0, args)
+    gpu_mc.set_burst_range(0, -1, 0)
+    gpu_mc.reSampleTimes = R
+    gpu_mc.run_kernel(sid=0, seed=seed, set_id=2, samples=True)
+    gpu_mc.synchronize(0)
+    hist, c64, c32, _ = gpu_mc.get_hist_chi2(0)
+    E, acc, F = gpu_mc.get_samples(0, d.nbursts * R)
+    o = oracle.run(oracle.Data.from_bursts(d), oracle.make_params(n, args, *ph), R=R, seed=seed,
+                   set_id=2)
+    assert np.array_equal(acc, o["acc"])
+    assert np.array_equal(bits(F), bits(o["F"])) and np.array_equal(bits(E), bits(o["E"]))
+    assert np.array_equal(hist.astype(np.int64), o["hist"])
+    w32, w64, _ = oracle.chi2(oracle.hist(sg).astype(np.float32), o["hist"], R, n)
+    assert np.float32(c32).view(np.uint32) == np.float32(w32).view(np.uint32)
+    assert abs(c64 - w64) <= 1e-12 * abs(w64)
+
+
+@pytest.mark.gpu
 def test_hdf5_into_the_gpu(gpu_mc, tmp_path):
     from gsmfretda_amd import pda
     d, sg = _bursts(400, seed=12)
