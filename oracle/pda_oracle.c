@@ -492,7 +492,9 @@ static void unit_impl(const or_data *d, const or_params *p, int64_t burst, uint3
     int live = P > 1 && T > 0.0f;
     if (st) {
         st->units++;
-        if (live) st->live_philox_blocks++;
+        /* the draw matters only when the first running sum can be <= u, i.e.
+         * the inflow of state 0 is at most 1 (u is in (0, 1]) */
+        if (live && !(p->p0[0] > 1.0f)) st->live_philox_blocks++;
     }
 
     float tsum = 0.0f, F = 0.0f, accf = 0.0f;

@@ -49,7 +49,8 @@ typedef struct {
  * algorithm -- the work a unit's result depends on -- for the cost model:
  * windows up to the one that closes at the last photon (later windows count
  * no photon and change nothing), the Philox blocks those windows need (no
- * normal block without broadening), the comparisons of a binary search for
+ * normal block without broadening; the initial-state block only when the
+ * inflow weights leave the initial state open), the comparisons of a binary search for
  * each window's end over the photons left (ceil(log2(left + 1))), binomial
  * draws with at least one trial, background draws, and the inner iterations
  * of those windows. */

@@ -46,7 +46,12 @@ def test_counters_on_a_hand_sized_case(oracle):
     assert st["live_windows"] == 10 * 4                  # P = 1 bursts have no live window
     assert st["search_steps"] == 10 * 4 * 4              # ceil(log2(14)) = 4
     assert st["bg_draws"] == 0 and st["binom_draws"] == 10 * 4
-    assert st["live_philox_blocks"] == 10 * 4 * 2        # the initial state and the window
+    # the window's block and the initial state's: the inflow of state 0 is
+    # 1e-6 /s (at most 1), so the initial-state draw is open and counts
+    assert st["live_philox_blocks"] == 10 * 4 * 2
+    p = oracle.make_params(2, synth.make_args([0.5, 0.5], [2.0, 2.0]), 1.0, 0.0, 0.0, 52.0)
+    st = oracle.run(od, p, R=4, seed=3, stats=True)["stats"]
+    assert st["live_philox_blocks"] - st["live_windows"] == 0  # inflow 2 /s: state 0 is fixed
     assert st["windows"] >= st["live_windows"]
 
 
