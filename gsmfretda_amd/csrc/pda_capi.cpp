@@ -151,9 +151,7 @@ int load_common(pda_ctx* ctx, const int64_t* start, const int64_t* stop, const i
         s.begin = 0;
         s.end = -1;
         auto reset = [](RbState& r) {
-            r.poff_b0 = -1;  // the RB pmf offsets follow the photon counts
-            r.pmf_valid = false;
-            r.ran = false;   // no derivative of a run over the previous data
+            r.ran = false;  // no derivative of a run over the previous data
         };
         reset(s.rb);
         for (auto& r : s.rbpop) reset(r);
@@ -271,6 +269,7 @@ int pda_destroy(pda_ctx* ctx) {
     free_data(ctx);
     (void)hipFree(ctx->d_obs);
     (void)hipFree(ctx->d_lf);
+    free_rb_memo(ctx);
     delete ctx;
     return PDA_OK;
 }

@@ -100,7 +100,7 @@ struct RbState {
     float* d_S = nullptr;              // per unit: scores of the rates
     size_t cap_units = 0, cap_scores = 0;
     unsigned long long* d_hist = nullptr;  // [8][kMaxBins]
-    uint32_t* d_work = nullptr;        // kWorkWords: walk queues + pmf burst counter
+    uint32_t* d_work = nullptr;        // kWorkWords: walk queues + pmf pass queues
     uint8_t* d_res = nullptr;          // header + uint64 M[kMaxBins]
     uint8_t* h_res = nullptr;          // pinned
     double* d_g = nullptr;             // cotangent per bin
@@ -109,17 +109,18 @@ struct RbState {
     float* d_pmf_tab = nullptr;        // binomial pmfs of the states, counts < 256
     double* d_jac = nullptr;           // [kMaxBins][n + K] rows of 16
     double* h_jac = nullptr;           // pinned
-    // each unit's pmf from the last forward with scores, for the derivative
-    // passes: unit r of burst b0 + i at d_pmf + poff[i] + r * (bound_i + 1)
-    float* d_pmf = nullptr;
-    size_t cap_pmf = 0;                // floats allocated
-    uint64_t* d_poff = nullptr;        // nbursts + 1 offsets of the run
-    size_t cap_poff = 0;
-    std::vector<uint64_t> h_poff;
-    int64_t poff_b0 = -1, poff_b1 = -1;
-    int poff_R = 0;
-    float poff_coef = -1.0f;
-    bool pmf_valid = false;
+    // units whose acceptor total is too large for the per-lane pass, listed
+    // by the walk (capacity cap_units), and their count
+    uint32_t* d_big = nullptr;
+    uint32_t* d_big_count = nullptr;
+    double* d_Sbar = nullptr;          // per burst: mean score of its R units
+    size_t cap_sbar = 0;
+    // pmf memo of the last forward (layout: the context's RbMemoLayout of n
+    // states): every count vector's pmf, and (with scores) its pmfs with
+    // one trial of each counted state removed
+    float* d_poolP = nullptr;
+    float* d_poolD = nullptr;
+    size_t cap_poolP = 0, cap_poolD = 0;
     double* h_grad = nullptr;          // pinned
     hipEvent_t ev0 = nullptr, ev_walk = nullptr, ev1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;
     bool ready = false;                // every fixed-size buffer above allocated
@@ -168,6 +169,20 @@ struct Slot {
 
 }  // namespace pda
 
+namespace pda {
+// Parameter-independent layout of the RB pmf memo for n states: the count
+// vectors with sum < kRbMemoCap, listed by total (so the lanes of a wave
+// build pmfs of one length), and per vector its offsets in the pools.
+struct RbMemoLayout {
+    uint32_t nvec = 0;
+    uint64_t poolP = 0, poolD = 0;  // floats
+    uint4* d_list = nullptr;        // list order: counts (x, y), offP, 0
+    uint4* d_listD = nullptr;       // list order: offD of the 1st..4th counted state
+    uint32_t* d_rank_offP = nullptr;  // by rb_rank
+    uint4* d_rank_offD = nullptr;     // by rb_rank
+};
+}  // namespace pda
+
 struct pda_ctx {
     int device = 0;
     int num_cu = 256;
@@ -184,6 +199,7 @@ struct pda_ctx {
     std::vector<uint32_t> h_P;      // photons per burst (host copy)
     double* d_lf = nullptr;         // log k! for the RB form
     size_t lds_per_block = 65536;   // LDS a workgroup may allocate
+    pda::RbMemoLayout rb_memo[5];   // by state count (2..4 used)
 };
 
 namespace pda {
@@ -197,6 +213,7 @@ void drop_graph(Population& p);
 // (called by pda_create with the device current)
 int noalloc_setup(int device);
 void free_rb(RbState& r);
+void free_rb_memo(pda_ctx* ctx);
 
 // Walk passes (pda_kernels.hip): fast form, general form over the replay
 // list, general form over every unit.
@@ -239,24 +256,35 @@ hipError_t microbench_shape(int op, uint32_t* grid, uint32_t* threads);
 hipError_t launch_microbench(int op, uint32_t grid, int iters, float param, const double* qn,
                              const int32_t* bound, uint64_t* out, hipStream_t stream);
 struct RbModel;
-// which: 0/1 RB walk without/with scores; 2 forward, 3 adjoint, 4 Jacobian,
-// 5 forward in place (rb_pmf_kernel)
+// which: 0/1 RB walk without/with scores; 2/3/4 rb_unit_kernel forward,
+// adjoint, Jacobian; 5/6/7 rb_big_kernel
 hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu);
 hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
                           const uint2* links, const ModelBlob* blob, const RbModel* rbm,
                           const RunArgs& a, uint2* cnt, float* F, float* S, uint32_t* work,
-                          hipStream_t stream);
-// mode: 0 forward, 1 adjoint, 2 Jacobian, 3 forward in place; stored: the
-// pass reads the forward's stored pmfs (fewer LDS buffers)
-size_t rb_pmf_lds(int B, int cap_big, int mode, bool stored);
+                          uint32_t* big, uint32_t* big_count, hipStream_t stream);
+// dynamic LDS bytes (mode: 0 forward, 1 adjoint, 2 Jacobian)
+int rb_unit_threads();
+size_t rb_unit_lds_bytes(int B, int mode);
+size_t rb_big_lds_bytes(int B, int cap, int mode);
+size_t rb_memo_lds_bytes(int n, bool deriv);
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
                            hipStream_t stream);
-hipError_t launch_rb_pmf(int mode, uint32_t grid, int cap_big, const RbModel* rbm,
-                         const double* lf, const float* pmf_tab, const RunArgs& a,
-                         const BurstDesc* bursts, const uint2* cnt, const float* F,
-                         const float* S, const double* g, unsigned long long* hist_acc,
-                         double* grad, uint32_t* work, float* pmf_io, const uint64_t* poff,
-                         hipStream_t stream);
+hipError_t launch_rb_memo(bool deriv, int n, const RbModel* rbm, const float* pmf_tab,
+                          const uint4* list, const uint4* listD, uint32_t nvec, float* poolP,
+                          float* poolD, hipStream_t stream);
+hipError_t launch_rb_unit(int mode, uint32_t grid, const RbModel* rbm, const RunArgs& a,
+                          const uint2* cnt, const float* F, const float* S, const double* Sbar,
+                          const double* g, const uint32_t* rank_offP, const uint4* rank_offD,
+                          const float* poolP, const float* poolD, unsigned long long* hist_acc,
+                          double* grad, uint32_t* work, hipStream_t stream);
+hipError_t launch_rb_big(int mode, uint32_t grid, int cap, const RbModel* rbm, const double* lf,
+                         const float* pmf_tab, const RunArgs& a, const uint2* cnt, const float* F,
+                         const float* S, const double* Sbar, const double* g,
+                         const uint32_t* big, const uint32_t* big_count,
+                         unsigned long long* hist_acc, double* grad, hipStream_t stream);
+hipError_t launch_rb_score_mean(const RunArgs& a, int K, const float* S, double* Sbar,
+                                hipStream_t stream);
 hipError_t launch_rb_finish(const RunArgs& a, const float* obs, unsigned long long* hist_acc,
                             uint32_t* work, void* res, hipStream_t stream);
 

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -42,12 +44,25 @@ void free_rb(RbState& r) {
     (void)hipFree(r.d_pmf_tab);
     (void)hipFree(r.d_jac);
     (void)hipHostFree(r.h_jac);
-    (void)hipFree(r.d_pmf);
-    (void)hipFree(r.d_poff);
+    (void)hipFree(r.d_big);
+    (void)hipFree(r.d_big_count);
+    (void)hipFree(r.d_Sbar);
+    (void)hipFree(r.d_poolP);
+    (void)hipFree(r.d_poolD);
     (void)hipHostFree(r.h_grad);
     for (hipEvent_t e : {r.ev0, r.ev_walk, r.ev1, r.ev_a0, r.ev_a1})
         if (e) (void)hipEventDestroy(e);
     r = RbState();
+}
+
+void free_rb_memo(pda_ctx* ctx) {
+    for (RbMemoLayout& L : ctx->rb_memo) {
+        (void)hipFree(L.d_list);
+        (void)hipFree(L.d_listD);
+        (void)hipFree(L.d_rank_offP);
+        (void)hipFree(L.d_rank_offD);
+        L = RbMemoLayout();
+    }
 }
 
 }  // namespace pda
@@ -65,49 +80,6 @@ constexpr size_t kRbResBytes = sizeof(RbResultHeaderH) + sizeof(uint64_t) * kMax
 constexpr int kGradLen = kRbMaxStates + kRbMaxPairs;
 constexpr size_t kRbCounters = kWorkWords;  // queue counters + the pmf pass's
 
-// Per-burst offsets of the unit pmf store for bursts [b0, b1) with R
-// resamples (each unit of burst b gets rb_total_bound(P_b) + 1 floats), and
-// the store itself.  When the device cannot hold the store, d_pmf stays
-// null and the derivative passes rebuild each pmf (same values).
-int ensure_pmf_store(pda_ctx* ctx, RbState& r, int64_t b0, int64_t b1, int R, float coef,
-                     hipStream_t stream) {
-    if (r.poff_b0 != b0 || r.poff_b1 != b1 || r.poff_R != R || r.poff_coef != coef) {
-        const size_t nb = size_t(b1 - b0);
-        r.h_poff.assign(nb + 1, 0u);
-        for (size_t i = 0; i < nb; ++i)
-            r.h_poff[i + 1] = r.h_poff[i] + uint64_t(R) *
-                uint64_t(rb_total_bound(ctx->h_P[size_t(b0) + i], coef) + 1);
-        HIP_TRY(hipStreamSynchronize(stream));
-        if (nb + 1 > r.cap_poff) {
-            (void)hipFree(r.d_poff);
-            r.d_poff = nullptr;
-            r.cap_poff = 0;
-            HIP_TRY(hipMalloc(&r.d_poff, sizeof(uint64_t) * (nb + 1)));
-            r.cap_poff = nb + 1;
-        }
-        HIP_TRY(hipMemcpy(r.d_poff, r.h_poff.data(), sizeof(uint64_t) * (nb + 1),
-                          hipMemcpyHostToDevice));
-        r.poff_b0 = b0;
-        r.poff_b1 = b1;
-        r.poff_R = R;
-        r.poff_coef = coef;
-    }
-    const size_t need = size_t(r.h_poff.back());
-    if (need > r.cap_pmf || !r.d_pmf) {
-        HIP_TRY(hipStreamSynchronize(stream));
-        (void)hipFree(r.d_pmf);
-        r.d_pmf = nullptr;
-        r.cap_pmf = 0;
-        if (hipMalloc(&r.d_pmf, sizeof(float) * std::max<size_t>(need, 1)) != hipSuccess) {
-            (void)hipGetLastError();  // out of memory: no store
-            r.d_pmf = nullptr;
-        } else {
-            r.cap_pmf = need;
-        }
-    }
-    return PDA_OK;
-}
-
 int ensure_rb(pda_ctx* ctx, RbState& r) {
     if (!r.ready) {
         // each buffer once: a call that fails part way (out of memory) leaves
@@ -117,6 +89,7 @@ int ensure_rb(pda_ctx* ctx, RbState& r) {
         if (!r.d_hist) HIP_TRY(hipMalloc(&r.d_hist, sizeof(unsigned long long) * 8u * kMaxBins));
         HIP_TRY(hipMemset(r.d_hist, 0, sizeof(unsigned long long) * 8u * kMaxBins));
         if (!r.d_work) HIP_TRY(hipMalloc(&r.d_work, sizeof(uint32_t) * kRbCounters));
+        if (!r.d_big_count) HIP_TRY(hipMalloc(&r.d_big_count, sizeof(uint32_t)));
         HIP_TRY(hipMemset(r.d_work, 0, sizeof(uint32_t) * kRbCounters));
         if (!r.d_res) HIP_TRY(hipMalloc(&r.d_res, kRbResBytes));
         if (!r.h_res)
@@ -179,6 +152,135 @@ void build_rb_model(const Model& M, RbModel& R, bool* rates_positive, bool* p_in
     }
 }
 
+// The memo layout of n states (built once per context and state count):
+// the count vectors with sum < kRbMemoCap in lexicographic order are ranked
+// 0, 1, ... (rb_rank); the build list holds them by total, and the pools
+// follow the list, so that the blocks written by one wave of rb_memo_kernel
+// (64 consecutive list entries) are adjacent: P blocks of Mt + 1 floats in
+// list order; per wave, for each state s in order, the leave-one-out blocks
+// (Mt floats) of the wave's vectors that count state s, in list order.
+int ensure_memo_layout(pda_ctx* ctx, int n) {
+    RbMemoLayout& L = ctx->rb_memo[n];
+    if (L.d_list) return PDA_OK;
+    const int T = kRbMemoCap - 1;
+    struct Vec {
+        int m[kRbMaxStates];
+        int Mt;
+        uint32_t rank;
+    };
+    std::vector<Vec> v;
+    int m[kRbMaxStates] = {0, 0, 0, 0};
+    // lexicographic enumeration of (m_0 .. m_{n-1}) with sum <= T
+    std::function<void(int, int)> rec = [&](int i, int left) {
+        if (i == n) {
+            Vec e{};
+            int Mt = 0;
+            for (int s = 0; s < kRbMaxStates; ++s) {
+                e.m[s] = s < n ? m[s] : 0;
+                Mt += e.m[s];
+            }
+            e.Mt = Mt;
+            e.rank = uint32_t(v.size());
+            v.push_back(e);
+            return;
+        }
+        for (int x = 0; x <= left; ++x) {
+            m[i] = x;
+            rec(i + 1, left - x);
+        }
+        m[i] = 0;
+    };
+    rec(0, T);
+    for (const Vec& e : v)
+        if (rb_rank(e.m, n, T) != e.rank)
+            return fail(PDA_E_STATE, "RB memo: rank of a count vector disagrees with the enumeration");
+    std::stable_sort(v.begin(), v.end(), [](const Vec& x, const Vec& y) { return x.Mt < y.Mt; });
+    const size_t nv = v.size();
+    std::vector<uint4> list(nv), listD(nv, make_uint4(0u, 0u, 0u, 0u));
+    std::vector<uint32_t> rank_offP(nv);
+    std::vector<uint4> rank_offD(nv, make_uint4(0u, 0u, 0u, 0u));
+    uint64_t offP = 0, offD = 0;
+    for (size_t i = 0; i < nv; ++i) {
+        uint32_t cnt[kRbMaxStates];
+        for (int s = 0; s < kRbMaxStates; ++s) cnt[s] = uint32_t(v[i].m[s]);
+        const uint2 c = rb_pack(cnt);
+        list[i] = make_uint4(c.x, c.y, uint32_t(offP), 0u);
+        rank_offP[v[i].rank] = uint32_t(offP);
+        offP += uint64_t(v[i].Mt) + 1u;
+    }
+    constexpr size_t kWaveVecs = kWave;  // list entries per wave of rb_memo_kernel
+    for (size_t w0 = 0; w0 < nv; w0 += kWaveVecs) {
+        const size_t w1 = std::min(nv, w0 + kWaveVecs);
+        for (int s = 0; s < n; ++s)
+            for (size_t i = w0; i < w1; ++i) {
+                if (v[i].m[s] == 0) continue;
+                int j = 0;  // slot: counted states before s
+                for (int t = 0; t < s; ++t) j += v[i].m[t] > 0 ? 1 : 0;
+                uint32_t* d = &listD[i].x;
+                d[j] = uint32_t(offD);
+                offD += uint64_t(v[i].Mt);
+            }
+        for (size_t i = w0; i < w1; ++i) rank_offD[v[i].rank] = listD[i];
+    }
+    if (offP >= (uint64_t(1) << 32) || offD >= (uint64_t(1) << 32))
+        return fail(PDA_E_STATE, "RB memo pools exceed 2^32 entries");
+    auto upload = [&](auto** dst, const auto& src) -> int {
+        using T = std::remove_reference_t<decltype(src[0])>;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), sizeof(T) * src.size()));
+        HIP_TRY(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+        return PDA_OK;
+    };
+    RbMemoLayout out;
+    int rc = upload(&out.d_list, list);
+    if (!rc) rc = upload(&out.d_listD, listD);
+    if (!rc) rc = upload(&out.d_rank_offP, rank_offP);
+    if (!rc) rc = upload(&out.d_rank_offD, rank_offD);
+    if (rc) {
+        (void)hipFree(out.d_list);
+        (void)hipFree(out.d_listD);
+        (void)hipFree(out.d_rank_offP);
+        (void)hipFree(out.d_rank_offD);
+        return rc;
+    }
+    out.nvec = uint32_t(nv);
+    out.poolP = offP;
+    out.poolD = offD;
+    L = out;
+    return PDA_OK;
+}
+
+// Launch shape of one pmf pass (mode 0 forward, 1 adjoint, 2 Jacobian):
+// rb_unit_kernel on every slot its LDS allows (at most enough workgroups for
+// the units), about eight grabs per wave; rb_big_kernel on every slot its
+// LDS allows (its workgroups beyond the listed units exit at once).
+struct PassShape {
+    uint32_t unit_grid = 1, batch = kWave, big_grid = 1;
+};
+int pass_shape(pda_ctx* ctx, int mode, int B, int cap, uint32_t nunits, PassShape& ps) {
+    static const char* names[] = {"forward", "adjoint", "Jacobian"};
+    constexpr size_t kStatic = sizeof(RbModel) + 64;  // the kernels' static LDS
+    const size_t lds = rb_unit_lds_bytes(B, mode);
+    if (lds + kStatic > ctx->lds_per_block)
+        return fail(PDA_E_DATA, "%d bins exceed the LDS of the RB %s pass", B, names[mode]);
+    int per = 0;
+    HIP_TRY(rb_occupancy(2 + mode, int(lds), &per));
+    const uint64_t threads = uint64_t(rb_unit_threads());
+    const uint64_t g = std::min<uint64_t>(uint64_t(ctx->num_cu) * uint64_t(std::max(per, 1)),
+                                          (uint64_t(nunits) + threads - 1) / threads);
+    ps.unit_grid = uint32_t(std::max<uint64_t>(g, 1));
+    uint64_t bt = uint64_t(nunits) / (uint64_t(ps.unit_grid) * (threads / kWave) * 8u);
+    bt = (bt + kWave - 1) / kWave * kWave;
+    ps.batch = uint32_t(std::min<uint64_t>(std::max<uint64_t>(bt, kWave), 4096));
+    const size_t blds = rb_big_lds_bytes(B, cap, mode);
+    if (blds + kStatic > ctx->lds_per_block)
+        return fail(PDA_E_DATA, "%d bins and bursts needing %d acceptor outcomes exceed the LDS "
+                    "of the RB %s pass", B, cap, names[mode]);
+    per = 0;
+    HIP_TRY(rb_occupancy(5 + mode, int(blds), &per));
+    ps.big_grid = uint32_t(std::max(1, ctx->num_cu * std::max(per, 1)));
+    return PDA_OK;
+}
+
 // Enqueues one RB forward of model M -- the walk reading M from the device
 // blob d_blob -- over the slot's burst range into state r, on the slot's
 // stream.  sync_model: first wait for the stream, whose last run may still be
@@ -204,21 +306,25 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     if (bound > kRbMaxCount)
         return fail(PDA_E_DATA, "the RB form supports acceptor totals up to %d (bursts of up to "
                     "%u photons need %d)", kRbMaxCount, maxP, bound);
-    const int cap = bound + 1;
+    const int cap = bound + 1;  // rb_big_kernel's pmf buffers
     const bool scores = (flags & PDA_RB_SCORES) != 0;
-    if (rb_pmf_lds(ctx->B, cap, 0, false) + sizeof(RbModel) > ctx->lds_per_block)
+    if (rb_big_lds_bytes(ctx->B, cap, 0) + sizeof(RbModel) > ctx->lds_per_block)
         return fail(PDA_E_DATA, "bursts of %u photons exceed the LDS of the RB kernels", maxP);
 
-    // buffers of the per-unit summaries
+    // buffers of the per-unit summaries and the walk's list of long units
     if (size_t(units) > r.cap_units) {
         HIP_TRY(hipStreamSynchronize(s.stream));
         (void)hipFree(r.d_cnt);
         (void)hipFree(r.d_F);
+        (void)hipFree(r.d_big);
         r.d_cnt = nullptr;
         r.d_F = nullptr;
+        r.d_big = nullptr;
         r.cap_units = 0;
-        HIP_TRY(hipMalloc(&r.d_cnt, sizeof(uint2) * size_t(std::max<int64_t>(units, 1))));
-        HIP_TRY(hipMalloc(&r.d_F, sizeof(float) * size_t(std::max<int64_t>(units, 1))));
+        const size_t nu = size_t(std::max<int64_t>(units, 1));
+        HIP_TRY(hipMalloc(&r.d_cnt, sizeof(uint2) * nu));
+        HIP_TRY(hipMalloc(&r.d_F, sizeof(float) * nu));
+        HIP_TRY(hipMalloc(&r.d_big, sizeof(uint32_t) * nu));
         r.cap_units = size_t(units);
     }
     const size_t nsc = scores ? size_t(units) * size_t(n * (n - 1)) : 0;
@@ -229,6 +335,16 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
         r.cap_scores = 0;
         HIP_TRY(hipMalloc(&r.d_S, sizeof(float) * nsc));
         r.cap_scores = nsc;
+    }
+    const bool sbar = scores && R >= 2;  // the derivative passes need R >= 2
+    const size_t nsb = sbar ? size_t(b1 - b0) * size_t(n * (n - 1)) : 0;
+    if (nsb > r.cap_sbar) {
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        (void)hipFree(r.d_Sbar);
+        r.d_Sbar = nullptr;
+        r.cap_sbar = 0;
+        HIP_TRY(hipMalloc(&r.d_Sbar, sizeof(double) * nsb));
+        r.cap_sbar = nsb;
     }
     if (sync_model) HIP_TRY(hipStreamSynchronize(s.stream));  // the pinned model may be in use
     RbModel* hm = static_cast<RbModel*>(r.h_rbm);
@@ -263,41 +379,51 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min(max_grid, (uint64_t(units) + batch - 1) / batch)));
     a.batch = uint32_t(batch);
     a.queues = std::min<uint32_t>(ctx->shape.queues ? ctx->shape.queues : kDefaultQueues, grid);
-    // the in-place forward (two LDS buffers, more registers) when it lets
-    // more waves share a CU; otherwise (bursts large enough that the
-    // whole-wave layout sets the LDS) the three-buffer one
-    int fwd_mode = 0;
-    {
-        int per_ip = 0;
-        HIP_TRY(rb_occupancy(2, int(rb_pmf_lds(ctx->B, cap, 0, false)), &per_cu));
-        HIP_TRY(rb_occupancy(5, int(rb_pmf_lds(ctx->B, cap, 3, false)), &per_ip));
-        if (per_ip > per_cu) {
-            fwd_mode = 3;
-            per_cu = per_ip;
-        }
+    PassShape ps;
+    if (int rc = pass_shape(ctx, 0, a.B, cap, a.nunits, ps)) return rc;
+    RunArgs la = a;
+    la.batch = ps.batch;
+    // the memo of this set: pmfs, and with scores the leave-one-out pmfs
+    if (int rc = ensure_memo_layout(ctx, n)) return rc;
+    const RbMemoLayout& ML = ctx->rb_memo[n];
+    if (rb_memo_lds_bytes(n, scores) + 512 > ctx->lds_per_block)
+        return fail(PDA_E_STATE, "the RB memo kernel exceeds the LDS of a workgroup");
+    if (ML.poolP > r.cap_poolP) {
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        (void)hipFree(r.d_poolP);
+        r.d_poolP = nullptr;
+        r.cap_poolP = 0;
+        HIP_TRY(hipMalloc(&r.d_poolP, sizeof(float) * ML.poolP));
+        r.cap_poolP = ML.poolP;
     }
-    per_cu = std::max(per_cu, 1);
-    const uint32_t pgrid = uint32_t(std::max<int64_t>(
-        1, std::min<int64_t>(int64_t(ctx->num_cu) * per_cu, b1 - b0)));
+    if (scores && ML.poolD > r.cap_poolD) {
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        (void)hipFree(r.d_poolD);
+        r.d_poolD = nullptr;
+        r.cap_poolD = 0;
+        HIP_TRY(hipMalloc(&r.d_poolD, sizeof(float) * ML.poolD));
+        r.cap_poolD = ML.poolD;
+    }
 
-    float* pmf_io = nullptr;
-    if (scores) {
-        int rc = ensure_pmf_store(ctx, r, b0, b1, R, hm->coef, s.stream);
-        if (rc) return rc;
-        pmf_io = r.d_pmf;  // null when the store did not fit: derivatives rebuild
-    }
     HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
+    HIP_TRY(hipMemsetAsync(r.d_big_count, 0, sizeof(uint32_t), s.stream));
     HIP_TRY(hipMemcpyAsync(r.d_rbm, r.h_rbm, sizeof(RbModel), hipMemcpyHostToDevice, s.stream));
+    const RbModel* d_rbm = static_cast<const RbModel*>(r.d_rbm);
     HIP_TRY(hipEventRecord(r.ev0, s.stream));
     HIP_TRY(launch_rb_walk(scores, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, d_blob,
-                           static_cast<const RbModel*>(r.d_rbm), a, r.d_cnt, r.d_F, r.d_S,
-                           r.d_work, s.stream));
+                           d_rbm, a, r.d_cnt, r.d_F, r.d_S, r.d_work, r.d_big, r.d_big_count,
+                           s.stream));
     HIP_TRY(hipEventRecord(r.ev_walk, s.stream));
-    HIP_TRY(launch_rb_table(static_cast<const RbModel*>(r.d_rbm), ctx->d_lf, r.d_pmf_tab, n,
-                            s.stream));
-    HIP_TRY(launch_rb_pmf(fwd_mode, pgrid, cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
-                          r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, r.d_hist, nullptr,
-                          r.d_work, pmf_io, r.d_poff, s.stream));
+    HIP_TRY(launch_rb_table(d_rbm, ctx->d_lf, r.d_pmf_tab, n, s.stream));
+    HIP_TRY(launch_rb_memo(scores, n, d_rbm, r.d_pmf_tab, ML.d_list, ML.d_listD, ML.nvec,
+                           r.d_poolP, r.d_poolD, s.stream));
+    HIP_TRY(launch_rb_unit(0, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, nullptr, nullptr, nullptr,
+                           ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, r.d_poolD, r.d_hist,
+                           nullptr, r.d_work, s.stream));
+    HIP_TRY(launch_rb_big(0, ps.big_grid, cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt, r.d_F,
+                          nullptr, nullptr, nullptr, r.d_big, r.d_big_count, r.d_hist, nullptr,
+                          s.stream));
+    if (sbar) HIP_TRY(launch_rb_score_mean(a, n * (n - 1), r.d_S, r.d_Sbar, s.stream));
     HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s.stream));
     HIP_TRY(hipEventRecord(r.ev1, s.stream));
     HIP_TRY(hipMemcpyAsync(r.h_res, r.d_res, sizeof(RbResultHeaderH) + sizeof(uint64_t) * size_t(a.B),
@@ -305,7 +431,6 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     r.ran = true;
     r.adj_ran = false;
     r.scores = scores;
-    r.pmf_valid = pmf_io != nullptr;
     r.R = R;
     r.B = a.B;
     r.left_pad = a.left_pad;
@@ -384,25 +509,33 @@ int check_derivative_run(const RbState& r, int sid, int nbins, int nargs) {
     return PDA_OK;
 }
 
-// Launch arguments and grid of an rb_pmf_kernel pass over the run's bursts.
-int derivative_pass(pda_ctx* ctx, const RbState& r, int mode, RunArgs& a, uint32_t& pgrid) {
-    a = RunArgs{};
+// Enqueues the lane and long-unit kernels of a derivative pass (mode 1
+// adjoint with cotangent r.d_g, 2 Jacobian) of state r's last forward,
+// accumulating into out (zeroed by the caller).
+int enqueue_derivative(pda_ctx* ctx, Slot& s, RbState& r, int mode, double* out) {
+    RunArgs a{};
     a.begin = uint32_t(r.b0);
     a.nunits = uint32_t((r.b1 - r.b0) * int64_t(r.R));
     a.R = uint32_t(r.R);
     a.B = r.B;
     a.left_pad = r.left_pad;
     a.nstates = r.n;
-    // the launch gets the stored pmfs when the forward kept them
-    const size_t lds = rb_pmf_lds(r.B, r.cap, mode, r.pmf_valid && r.d_pmf);
-    if (lds + sizeof(RbModel) > ctx->lds_per_block)
-        return fail(PDA_E_DATA, "%d bins and bursts needing %d acceptor outcomes exceed the LDS of "
-                    "the %s kernel", r.B, r.cap, mode == 2 ? "Jacobian" : "adjoint");
-    int per_cu = 0;
-    HIP_TRY(rb_occupancy(mode == 2 ? 4 : 3, int(lds), &per_cu));
-    per_cu = std::max(per_cu, 1);
-    pgrid = uint32_t(std::max<int64_t>(
-        1, std::min<int64_t>(int64_t(ctx->num_cu) * per_cu, r.b1 - r.b0)));
+    PassShape ps;
+    if (int rc = pass_shape(ctx, mode, r.B, r.cap, a.nunits, ps)) return rc;
+    RunArgs la = a;
+    la.batch = ps.batch;
+    const RbModel* d_rbm = static_cast<const RbModel*>(r.d_rbm);
+    const RbMemoLayout& ML = ctx->rb_memo[r.n];
+    const double* g = mode == 1 ? r.d_g : nullptr;
+    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
+    HIP_TRY(hipEventRecord(r.ev_a0, s.stream));
+    HIP_TRY(launch_rb_unit(mode, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, r.d_S, r.d_Sbar, g,
+                           ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, r.d_poolD, nullptr, out,
+                           r.d_work, s.stream));
+    HIP_TRY(launch_rb_big(mode, ps.big_grid, r.cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt,
+                          r.d_F, r.d_S, r.d_Sbar, g, r.d_big, r.d_big_count, nullptr, out,
+                          s.stream));
+    HIP_TRY(hipEventRecord(r.ev_a1, s.stream));
     return PDA_OK;
 }
 
@@ -411,18 +544,10 @@ int derivative_pass(pda_ctx* ctx, const RbState& r, int mode, RunArgs& a, uint32
 // stream), its gradient copied to r.h_grad.
 int rb_enqueue_adjoint(pda_ctx* ctx, Slot& s, RbState& r, const double* g) {
     std::memcpy(r.h_g, g, sizeof(double) * size_t(r.B));
-    RunArgs a;
-    uint32_t pgrid = 0;
-    if (int rc = derivative_pass(ctx, r, 1, a, pgrid)) return rc;
-    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
     HIP_TRY(hipMemsetAsync(r.d_grad, 0, sizeof(double) * kGradLen, s.stream));
     HIP_TRY(hipMemcpyAsync(r.d_g, r.h_g, sizeof(double) * size_t(r.B), hipMemcpyHostToDevice,
                            s.stream));
-    HIP_TRY(hipEventRecord(r.ev_a0, s.stream));
-    HIP_TRY(launch_rb_pmf(1, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
-                          r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, r.d_g, nullptr,
-                          r.d_grad, r.d_work, r.pmf_valid ? r.d_pmf : nullptr, r.d_poff, s.stream));
-    HIP_TRY(hipEventRecord(r.ev_a1, s.stream));
+    if (int rc = enqueue_derivative(ctx, s, r, 1, r.d_grad)) return rc;
     HIP_TRY(hipMemcpyAsync(r.h_grad, r.d_grad, sizeof(double) * kGradLen, hipMemcpyDeviceToHost,
                            s.stream));
     HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
@@ -440,17 +565,9 @@ void rb_collect_adjoint(const RbState& r, double* grad, int nargs) {
 
 // Enqueues the Jacobian pass of state r's last forward, copied to r.h_jac.
 int rb_enqueue_jacobian(pda_ctx* ctx, Slot& s, RbState& r) {
-    RunArgs a;
-    uint32_t pgrid = 0;
-    if (int rc = derivative_pass(ctx, r, 2, a, pgrid)) return rc;
     const size_t bytes = sizeof(double) * kGradLen * size_t(r.B);
-    HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
     HIP_TRY(hipMemsetAsync(r.d_jac, 0, bytes, s.stream));
-    HIP_TRY(hipEventRecord(r.ev_a0, s.stream));
-    HIP_TRY(launch_rb_pmf(2, pgrid, r.cap, static_cast<const RbModel*>(r.d_rbm), ctx->d_lf,
-                          r.d_pmf_tab, a, ctx->d_bursts, r.d_cnt, r.d_F, r.d_S, nullptr, nullptr,
-                          r.d_jac, r.d_work, r.pmf_valid ? r.d_pmf : nullptr, r.d_poff, s.stream));
-    HIP_TRY(hipEventRecord(r.ev_a1, s.stream));
+    if (int rc = enqueue_derivative(ctx, s, r, 2, r.d_jac)) return rc;
     HIP_TRY(hipMemcpyAsync(r.h_jac, r.d_jac, bytes, hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
     r.adj_ran = true;

@@ -49,23 +49,23 @@ struct RbModel {
 // Largest acceptor total of a unit in a burst of P photons: the windows
 // count disjoint photon runs and window w contributes at most
 // floor(c n_w + 1/2), which is <= n_w when c <= 1.
-PDA_HD inline int rb_total_bound(uint32_t P, float coef) {
+PDA_HD int rb_total_bound(uint32_t P, float coef) {
     if (coef <= 1.0f) return int(P);
     return int(ceil((double(coef) + 0.5) * double(P)));
 }
 
 // Args index (after the n efficiencies) of the rate s -> j, K(j, s): row j,
 // column s of genMatK's row-major off-diagonal list (src/eigenhelper.cpp:4-21).
-PDA_HD inline int rate_pair(int n, int s, int j) { return j * (n - 1) + (s < j ? s : s - 1); }
+PDA_HD int rate_pair(int n, int s, int j) { return j * (n - 1) + (s < j ? s : s - 1); }
 
 // log k! for k = 0..count, summed in order (lf[k] = lf[k-1] + log k).
-PDA_HD inline void log_factorials(double* lf, int count) {
+PDA_HD void log_factorials(double* lf, int count) {
     lf[0] = 0.0;
     for (int k = 1; k <= count; ++k) lf[k] = lf[k - 1] + dlog(double(k));
 }
 
 // Binomial(m, p)(k) in double from log factorials; exact 0/1 at p = 0, 1.
-PDA_HD inline double rb_binom_pmf(int m, int k, const double* lf, float p, double lp, double lq) {
+PDA_HD double rb_binom_pmf(int m, int k, const double* lf, float p, double lp, double lq) {
     if (p <= 0.0f) return k == 0 ? 1.0 : 0.0;
     if (p >= 1.0f) return k == m ? 1.0 : 0.0;
     double l = lf[m] - lf[k];
@@ -77,7 +77,7 @@ PDA_HD inline double rb_binom_pmf(int m, int k, const double* lf, float p, doubl
 
 // One output of Z = Y (*) X: sum_{k} Y[k] X[a - k], k ascending over the
 // overlap, float accumulation from 0.
-PDA_HD inline float rb_conv_at(const float* Y, int ylen, const float* X, int xlen, int a) {
+PDA_HD float rb_conv_at(const float* Y, int ylen, const float* X, int xlen, int a) {
     int k0 = a - (xlen - 1);
     if (k0 < 0) k0 = 0;
     const int k1 = a < ylen - 1 ? a : ylen - 1;
@@ -88,7 +88,7 @@ PDA_HD inline float rb_conv_at(const float* Y, int ylen, const float* X, int xle
 
 // Histogram bin of a acceptors out of F corrected photons, as the plain
 // walk bins acc / F (-1: dropped).
-PDA_HD inline int rb_bin(int a, float F, int B) {
+PDA_HD int rb_bin(int a, float F, int B) {
     const float E = F > 0.0f ? float(a) / F : 0.0f;
     if (!(E >= 0.0f && E < 1.0f)) return -1;
     const int b = int(floor(double(E) * double(B)));
@@ -100,19 +100,48 @@ PDA_HD inline int rb_bin(int a, float F, int B) {
 // rb_binom_pmf(m, k, ...).
 constexpr int kRbPmfTable = 256;
 constexpr int kRbPmfStride = kRbPmfTable * (kRbPmfTable + 1) / 2;
-PDA_HD inline int rb_pmf_index(int s, int m, int k) { return s * kRbPmfStride + m * (m + 1) / 2 + k; }
+PDA_HD int rb_pmf_index(int s, int m, int k) { return s * kRbPmfStride + m * (m + 1) / 2 + k; }
+
+// Pmf memo: a unit's pmf depends only on its photon counts per state (and
+// the set's efficiencies), so the pmfs of every count vector with acceptor
+// total Mt < kRbMemoCap are computed once per forward and looked up by the
+// units; units with larger totals compute their own.
+constexpr int kRbMemoCap = 48;
+
+// C(x, y) for 0 <= y <= kRbMaxStates and x >= 0 (0 when x < y); every
+// partial product C(x, i) is an integer, so the division is exact.
+PDA_HD uint32_t rb_choose(int x, int y) {
+    uint32_t c = 1;
+    for (int i = 0; i < y; ++i) c = c * uint32_t(x - i) / uint32_t(i + 1);
+    return x < y ? 0u : c;
+}
+
+// Lexicographic rank of the count vector (m_0 .. m_{n-1}) among all
+// n-vectors of counts >= 0 with sum <= L: for each position i, the vectors
+// that agree before i and have a smaller count at i, counted by the hockey
+// stick identity sum_{w <= W} C(w + k, k) = C(W + k + 1, k + 1).
+PDA_HD uint32_t rb_rank(const int* m, int n, int L) {
+    uint32_t r = 0;
+    int left = L;
+    for (int i = 0; i < n; ++i) {
+        const int k = n - 1 - i;
+        r += rb_choose(left + k + 1, k + 1) - rb_choose(left - m[i] + k + 1, k + 1);
+        left -= m[i];
+    }
+    return r;
+}
 
 // Fixed-point weight of a probability (2^-32 units), rounded half up.
-PDA_HD inline uint64_t rb_fixed(float P) { return uint64_t(double(P) * 4294967296.0 + 0.5); }
+PDA_HD uint64_t rb_fixed(float P) { return uint64_t(double(P) * 4294967296.0 + 0.5); }
 
 // Summary of one unit's walk: per-state photon counts (16 bits each), F.
-PDA_HD inline uint2 rb_pack(const uint32_t* cnt) {
+PDA_HD uint2 rb_pack(const uint32_t* cnt) {
     uint2 v;
     v.x = (cnt[0] & 0xFFFFu) | (cnt[1] << 16);
     v.y = (cnt[2] & 0xFFFFu) | (cnt[3] << 16);
     return v;
 }
-PDA_HD inline uint32_t rb_count(uint2 v, int s) {
+PDA_HD uint32_t rb_count(uint2 v, int s) {
     const uint32_t w = s < 2 ? v.x : v.y;
     return (s & 1) ? (w >> 16) : (w & 0xFFFFu);
 }

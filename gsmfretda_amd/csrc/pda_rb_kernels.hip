@@ -1,19 +1,27 @@
 // pda_rb_kernels.hip -- gfx950 kernels of the Rao-Blackwellised (RB) PDA form.
 //
-//   rb_walk_kernel   the walk of every (burst, resample) unit with the keyed
-//                    draws of the plain kernel (src/mc.cu:92-191) but no
-//                    acceptor draw; per unit it stores the photon counts per
-//                    state, F, and (for rate gradients) the score of the
-//                    walk's draws
-//   rb_pmf_kernel    one wave per burst, 4 units at a time on groups of 16
-//                    lanes (the whole wave per unit for large bursts): the
-//                    pmf of each unit's acceptor total (a convolution of one
-//                    binomial per state, in LDS, lanes over outcomes) spread
-//                    into the E histogram as 2^-32 fixed point -- or, for the
-//                    adjoint, the cotangent-weighted pmf and its derivatives,
-//                    or, for the Jacobian, the derivatives of every bin
-//   rb_finish_kernel reduces the histogram rows, computes chi2 on the
-//                    expected counts, resets the counters
+//   rb_walk_kernel       the walk of every (burst, resample) unit with the
+//                        keyed draws of the plain kernel (src/mc.cu:92-191)
+//                        but no acceptor draw; per unit it stores the photon
+//                        counts per state, F, and (for rate gradients) the
+//                        score of the walk's draws, and lists the units whose
+//                        acceptor total is too large for the memo
+//   rb_table_kernel      the binomial pmfs of every state, counts < 256
+//   rb_memo_kernel       the pmf (a convolution of one binomial per state)
+//                        of every count vector with a small acceptor total,
+//                        one lane per vector, in a strip of LDS of its own,
+//                        and (for derivatives) its leave-one-trial-out pmfs
+//   rb_unit_kernel       every unit with a small total: its pmf from the
+//                        memo spread into the E histogram as 2^-32 fixed
+//                        point (forward), or the cotangent-weighted pmf and
+//                        its derivatives (adjoint), or the derivatives of
+//                        every bin (Jacobian); lanes over the (unit,
+//                        outcome) pairs of 64 units at a time
+//   rb_big_kernel        the same for the listed units, one wave per unit,
+//                        lanes over acceptor totals, the pmf built in LDS
+//   rb_score_mean_kernel per burst, the mean score of its R resamples
+//   rb_finish_kernel     reduces the histogram rows, computes chi2 on the
+//                        expected counts, resets the counters
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,40 +33,43 @@
 
 namespace pda {
 
-// rb_pmf_kernel takes bursts in batches from up to kAuxQueues counters
-// after the walk's (workgroup g from counter g % kAuxQueues, one per XCD,
-// each over a contiguous eighth of the bursts): one counter and one grab per
-// burst made the pass wait on a million serialised atomics at 10^6 bursts.
-// In the derivative passes batches shrink to 1 as a queue empties and a
-// workgroup whose queue is empty takes single bursts from the others, so the
-// pass ends with every wave within about one burst of the last (the forward,
-// whose bursts are cheap, keeps fixed batches: fewer grabs measured faster).
-constexpr uint32_t kRbPmfCounter = kMaxQueues * kCounterStride;
-constexpr uint32_t kRbMaxBurstBatch = 8;
+// A unit's pmf has Mt + 1 entries, Mt its acceptor total (the sum of its
+// photon counts per state).  The pmfs of every count vector with
+// Mt < kRbMemoCap are built once per forward (rb_memo_kernel) and the units
+// look theirs up (rb_unit_kernel); the walk lists the units with larger
+// totals for rb_big_kernel.  The mean Mt is about 6 on the 3-state
+// workloads and 24 on the 4-state fast-dynamics one, where 97 % of the
+// units have Mt < 48.
+//
+// rb_memo_kernel: one lane per count vector, the pmf in a strip of LDS of
+// its own; kRbStride floats between the strips of consecutive lanes (odd,
+// so that lanes reading the same offset hit distinct LDS banks).
+constexpr int kRbStride = kRbMemoCap + 1;
+constexpr int kRbMemoWaves = 1;  // waves per workgroup of rb_memo_kernel
+// Entries per state of the pmf table rb_memo_kernel keeps in LDS: counts
+// m < kRbMemoCap, entry (m, k) at m (m + 1) / 2 + k (the first entries of
+// the state's block of the global table).
+constexpr int kRbMemoTab = kRbMemoCap * (kRbMemoCap + 1) / 2;
+// rb_unit_kernel: waves per workgroup, and its work counters (the
+// kAuxQueues lines after the walk's, workgroup g on counter g % kAuxQueues,
+// each over a contiguous share of the units: one queue per XCD).
+constexpr int kRbUnitWaves = 4;
+constexpr uint32_t kRbUnitCounter = kMaxQueues * kCounterStride;
+// Jacobian rows in global memory: kRbMaxStates efficiency columns, then
+// kRbMaxPairs rates; in LDS one more (odd rows of doubles spread the rows'
+// entries over the banks).
+constexpr int kRbJacCols = kRbMaxStates + kRbMaxPairs;
+constexpr int kRbJacRow = kRbJacCols + 1;
 
-// Jacobian scratch in LDS (doubles, after the float buffers): per bin the
-// efficiency and rate derivatives, the current unit's histogram per lane
-// group, the burst's histogram, the burst's score sums, the current units'
-// scores per lane group.
-struct JacLds {
-    double* JE;  // [B][kRbMaxStates]
-    double* JK;  // [B][kRbMaxPairs]
-    double* hu;  // [kWave / kRbSmallLanes][B]
-    double* Hb;  // [B]
-    double* Sk;  // [kRbMaxPairs]
-    double* Su;  // [kWave / kRbSmallLanes][kRbMaxPairs]
-};
-__host__ __device__ inline size_t rb_jac_doubles(int B) {
-    return size_t(B) * (kRbMaxStates + kRbMaxPairs + kWave / 16 + 1) +
-           size_t(kRbMaxPairs) * (1 + kWave / 16);
-}
+constexpr int kRbFwd = 0, kRbGrad = 1, kRbJac = 2;
 
 template <bool kScores>
 __global__ __launch_bounds__(kWave) void rb_walk_kernel(
     const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
     const uint2* __restrict__ links, const ModelBlob* __restrict__ blob,
     const RbModel* __restrict__ rbm, const RunArgs a, uint2* __restrict__ cnt_out,
-    float* __restrict__ F_out, float* __restrict__ S_out, uint32_t* __restrict__ work) {
+    float* __restrict__ F_out, float* __restrict__ S_out, uint32_t* __restrict__ work,
+    uint32_t* __restrict__ big, uint32_t* __restrict__ big_count) {
     __shared__ Model sM;
     __shared__ RbModel sR;
     __shared__ uint32_t s_cnt[kRbMaxStates][kWave];
@@ -94,6 +105,20 @@ __global__ __launch_bounds__(kWave) void rb_walk_kernel(
         F_out[unit] = st.F;
         if (kScores)
             for (int i = 0; i < K; ++i) S_out[size_t(unit) * size_t(K) + i] = s_S[i][lane];
+        // units too long for the memo join the list, one atomic per wave
+        // and call site: the lanes storing now, ranked among themselves
+        const uint32_t mt = s_cnt[0][lane] + s_cnt[1][lane] + s_cnt[2][lane] + s_cnt[3][lane];
+        const bool longer = mt >= uint32_t(kRbMemoCap);
+        const uint64_t m = __ballot(longer);
+        if (m) {
+            const int leader = __ffsll(static_cast<long long>(m)) - 1;
+            uint32_t base = 0;
+            if (int(lane) == leader) base = atomicAdd(big_count, uint32_t(__popcll(m)));
+            base = __shfl(base, leader);
+            if (longer)
+                big[base + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u))] = unit;
+        }
     };
     while (true) {
         bool need = !busy;
@@ -155,422 +180,667 @@ __global__ __launch_bounds__(kWave) void rb_walk_kernel(
     }
 }
 
-// Sum over the L lanes of a lane group (every lane gets the total).
-template <int L>
-__device__ __forceinline__ double group_sum(double v) {
+// Sum over the wave (every lane gets the total, the same on every run).
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, L);
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
-constexpr int kRbSmallLanes = 16;  // lanes per unit for small bursts (4 units per wave)
-constexpr int kRbSmallCap = 256;   // acceptor totals below this use the small layout
+// Inclusive prefix sum / maximum over the lanes of the wave.
+__device__ __forceinline__ int wave_scan_sum(int v, int lane) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_up(v, d);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_scan_max(int v, int lane) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_up(v, d);
+        if (lane >= d) v = t > v ? t : v;
+    }
+    return v;
+}
 
-// All R units of one burst, kWave / L at a time (one unit per group of L
-// lanes, lanes over acceptor totals).  Per group, LDS buffers X (the pmf so
-// far), Y (the next state's binomial pmf) and Z (a free buffer) of `cap`
-// floats each; the group's buffer roles are swapped instead of copied.  The
-// in-place forward (kRbFwdInPlace) writes each 16-lane product back over X
-// and has no Z.
+// Orders the LDS accesses of the wave's lanes: every lane's earlier LDS
+// writes are visible to every lane's later LDS reads.  The lanes of a wave
+// share data through LDS here without a workgroup barrier; the compiler
+// treats each lane as a thread of its own and, without this fence, may keep
+// a value a lane stored in a register instead of reading what another lane
+// wrote over it, or move accesses across the exchange.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Spreading the elements of 64 variable-length items over the lanes: item
+// l (owned by lane l) has len_l elements at positions pos_l .. pos_l +
+// len_l - 1 of the wave's concatenation (pos = exclusive prefix sum of len).
+// Window w holds positions 64 w .. 64 w + 63, one per lane; the owner of a
+// lane's position is the last item starting at or before it: each item
+// starting inside the window marks its start slot, and a running maximum
+// over the slots (carried across windows) names the owner.  slot: 64 ints of
+// the wave's LDS.  Calls body(j, owner) for each lane (j >= total: no
+// element), with every lane of the wave present at each call (shuffles may
+// be used); what the caller's lanes wrote to LDS before the call is visible
+// to body, and the body's LDS accesses end before the next window's.
+template <typename Body>
+__device__ __forceinline__ void wave_windows(int len, int pos, int total, int lane, int* slot,
+                                             Body&& body) {
+    int carry = -1;
+    wave_lds_sync();
+    for (int wb = 0; wb < total; wb += kWave) {
+        slot[lane] = -1;
+        wave_lds_sync();
+        if (len > 0 && pos >= wb && pos < wb + kWave) slot[pos - wb] = lane;
+        wave_lds_sync();
+        int own = wave_scan_max(slot[lane], lane);
+        own = own > carry ? own : carry;
+        carry = __shfl(own, kWave - 1);
+        body(wb + lane, own);
+        wave_lds_sync();
+    }
+}
+
+// What the passes compute per unit, with P(a), a = 0..Mt, the unit's pmf
+// and bin(a) the histogram bin of E = a / F:
 //
-// Forward (kGrad = false): the unit's pmf P(a), a = 0..Mt, goes into the LDS
-// histogram at E = a / F in 2^-32 fixed point.
+// Forward: P(a) into the bin of a, in 2^-32 fixed point (integer sums: the
+// result does not depend on the order in which units or outcomes arrive).
 //
-// Adjoint (kMode = kRbGrad): with G(a) the cotangent of a's bin, the unit
-// contributes h = sum_a G(a) P(a) and, per state s with m_s photons,
-//   d h / d p_s = m_s sum_c P_s(c) (G(c + 1) - G(c))
+// Adjoint, for a cotangent G over the bins: h = sum_a G(bin(a)) P(a) and,
+// per state s with m_s photons,
+//   d h / d p_s = m_s sum_c P_s(c) (G(bin(c + 1)) - G(bin(c)))
 // where P_s is the pmf with one trial of state s removed,
 // P(a) = q_s P_s(a) + p_s P_s(a - 1).  P_s follows from P by a first-order
-// recurrence run as a lane-parallel scan of affine maps, upwards when
-// p_s <= 1/2 and downwards otherwise (the direction in which the recurrence
-// damps rounding errors).  Per burst, the rate gradient is the likelihood
-// ratio R / (R - 1) sum_r (h_r - mean h) S_r.
+// recurrence, upwards when p_s <= 1/2 and downwards otherwise (the direction
+// in which it damps rounding errors).  The rate gradient is the likelihood
+// ratio R / (R - 1) sum_units h (S - Sbar), S the unit's score and Sbar the
+// mean score of its burst's R resamples: the per-burst leave-one-out
+// baseline, sum_r h_r S_r - (sum_r h_r)(sum_r S_r) / R, summed per unit.
 //
-// Jacobian (kMode = kRbJac): the adjoint for every indicator cotangent
-// G_b(a) = [bin(a) = b] at once.  G_b(c + 1) - G_b(c) is nonzero only where
-// c and c + 1 fall in different bins, so d M_b / d p_s gathers m_s P_s(c) at
-// the unit's bin boundaries; the rate columns use the unit's histogram h_rb
-// in place of h_r (sum_r h_rb S_r here, the mean term per burst).
-// kRbFwdInPlace: the forward with the 16-lane products written back over X
-// from registers (two buffers per group instead of three, more registers):
-// chosen when that lets more waves share a CU.
-constexpr int kRbFwd = 0, kRbGrad = 1, kRbJac = 2, kRbFwdInPlace = 3;
+// Jacobian: the adjoint for every indicator cotangent G_b(a) = [bin(a) = b]
+// at once.  G_b(c + 1) - G_b(c) is nonzero only where c and c + 1 fall in
+// different bins, so d M_b / d p_s gathers m_s P_s(c) at the unit's bin
+// boundaries; the rate columns add P(a) (S - Sbar) to the row of bin(a).
 
-template <int L, int kMode>
-__device__ __forceinline__ void rb_burst(const RbModel& sR, const double* __restrict__ lf,
-                                         const float* __restrict__ pmf_tab,
-                                         const RunArgs& a, uint32_t bl,
-                                         const uint2* __restrict__ cnt_in,
-                                         const float* __restrict__ F_in,
-                                         const float* __restrict__ S_in, const double* s_g,
-                                         unsigned long long* s_h, float* s_buf, int nbuf, int cap,
-                                         uint32_t lane, double& gp_acc, double& gk_acc,
-                                         const JacLds& jl, float* __restrict__ pmf_io,
-                                         const uint64_t* __restrict__ poff) {
-    constexpr bool kGrad = kMode == kRbGrad || kMode == kRbJac;
-    constexpr int G = kWave / L;
-    const int g = int(lane) / L, lg = int(lane) % L;
-    const int n = sR.n, K = sR.K, B = a.B;
-    float* base = s_buf + g * nbuf * cap;
-    // pmf_io: the forward stores each unit's pmf at poff[bl] + r * stride,
-    // the derivative passes read it back instead of rebuilding it
-    const uint64_t slot0 = pmf_io ? poff[bl] : 0u;
-    const uint64_t stride = pmf_io ? (poff[bl + 1] - slot0) / a.R : 0u;
-    double H = 0.0, Sg = 0.0, Ac = 0.0;  // lane lg of each group: rate pair lg
-    for (uint32_t r0 = 0; r0 < a.R; r0 += G) {
-        const uint32_t r = r0 + uint32_t(g);
-        const bool act = r < a.R;
-        const uint32_t u = bl * a.R + (act ? r : 0u);
-        const uint2 c = act ? cnt_in[u] : make_uint2(0u, 0u);
-        const float F = act ? F_in[u] : 0.0f;
-        // the forward in the 16-lane layout convolves in place (X and Y
-        // only); every other case writes products to Z
-        constexpr bool kInPlace = kMode == kRbFwdInPlace && L == kRbSmallLanes;
-        int bX = 0, bZ = 1, bY = kInPlace ? 1 : 2;
-        int xl = 1;
-        if (lg == 0) base[0] = 1.0f;
-        if (kGrad && pmf_io) {
-            // the pmf of the forward run: length 1 + sum of the counts
-            for (int s = 0; s < kRbMaxStates; ++s) xl += (act && s < n) ? int(rb_count(c, s)) : 0;
-            const float* src = pmf_io + slot0 + uint64_t(r) * stride;
-            for (int q = lg; act && q < xl; q += L) base[q] = src[q];
-            __syncthreads();
-        } else {
-#pragma unroll
-            for (int s = 0; s < kRbMaxStates; ++s) {
-                const int m = (act && s < n) ? int(rb_count(c, s)) : 0;
-                float* Y = base + bY * cap;
-                for (int k = lg; m > 0 && k <= m; k += L)
-                    Y[k] = m < kRbPmfTable
-                               ? pmf_tab[rb_pmf_index(s, m, k)]
-                               : float(rb_binom_pmf(m, k, lf, sR.p[s], sR.lp[s], sR.lq[s]));
-                __syncthreads();
-                if (m > 0) {
-                    if (xl == 1) {  // X = [1]: the product is Y itself, bit for bit
-                        const int t = bX;
-                        bX = bY;
-                        bY = t;
-                        xl = m + 1;
-                    } else if (kInPlace) {
-                        // at most kRbSmallCap / L outputs per lane, held in
-                        // registers until every read of X is done: the
-                        // group's lanes are lanes of one wave, so all the
-                        // loads below are issued before any store, and each
-                        // store waits on the loads its value came from
-                        constexpr int kOut = kRbSmallCap / kRbSmallLanes;
-                        float* X = base + bX * cap;
-                        float zr[kOut];
-#pragma unroll
-                        for (int i = 0; i < kOut; ++i) {
-                            const int q = lg + i * L;
-                            zr[i] = q < xl + m ? rb_conv_at(Y, m + 1, X, xl, q) : 0.0f;
-                        }
-#pragma unroll
-                        for (int i = 0; i < kOut; ++i) {
-                            const int q = lg + i * L;
-                            if (q < xl + m) X[q] = zr[i];
-                        }
-                        xl += m;
-                    } else {
-                        const float* X = base + bX * cap;
-                        float* Z = base + bZ * cap;
-                        for (int q = lg; q < xl + m; q += L) Z[q] = rb_conv_at(Y, m + 1, X, xl, q);
-                        const int t = bX;
-                        bX = bZ;
-                        bZ = t;
-                        xl += m;
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        const float* P = base + bX * cap;
-        if (!kGrad) {
-            float* dst = pmf_io ? pmf_io + slot0 + uint64_t(r) * stride : nullptr;
-            for (int q = lg; act && q < xl; q += L) {
-                const int bin = rb_bin(q, F, B);
-                if (bin >= 0) atomicAdd(&s_h[bin], (unsigned long long)rb_fixed(P[q]));
-                if (dst) dst[q] = P[q];
-            }
-        } else {
-            float* dG = base + bZ * cap;  // G(c + 1) - G(c)
-            double hp = 0.0;
-            double* hu = jl.hu + g * B;   // Jacobian: this unit's histogram
-            for (int q = lg; q < xl; q += L) {
-                const int b0 = rb_bin(q, F, B);
-                if (kMode == kRbJac) {
-                    if (act && b0 >= 0) atomicAdd(&hu[b0], double(P[q]));
-                    continue;
-                }
-                const double G0 = b0 >= 0 ? s_g[b0] : 0.0;
-                hp += G0 * double(P[q]);
-                if (q + 1 < xl) {
-                    const int b1 = rb_bin(q + 1, F, B);
-                    dG[q] = float((b1 >= 0 ? s_g[b1] : 0.0) - G0);
-                }
-            }
-            __syncthreads();
-            const double h = group_sum<L>(hp);
-            const int Mt = xl - 1;
-#pragma unroll
-            for (int s = 0; s < kRbMaxStates; ++s) {
-                const int m = (act && s < n) ? int(rb_count(c, s)) : 0;
-                double part = 0.0;
-                if (m > 0) {  // uniform within the group
-                    const double p = double(sR.p[s]), q = 1.0 - p;
-                    const bool up = p <= 0.5;
-                    const double alpha = up ? -p / q : -q / p;
-                    const double inv = up ? 1.0 / q : 1.0 / p;
-                    double carry = 0.0;
-                    for (int j0 = 0; j0 < Mt; j0 += L) {
-                        const int j = j0 + lg;
-                        const bool v = j < Mt;
-                        double A = v ? alpha : 1.0;
-                        double Bv = v ? double(P[up ? j : Mt - j]) * inv : 0.0;
-#pragma unroll
-                        for (int d = 1; d < L; d <<= 1) {
-                            const double A2 = __shfl_up(A, d, L);
-                            const double B2 = __shfl_up(Bv, d, L);
-                            if (lg >= d) {
-                                Bv = A * B2 + Bv;
-                                A = A * A2;
-                            }
-                        }
-                        const double x = A * carry + Bv;  // P_s(up ? j : Mt - 1 - j)
-                        carry = __shfl(x, L - 1, L);
-                        const int ci = up ? j : Mt - 1 - j;
-                        if (kMode == kRbJac) {
-                            const int b0 = rb_bin(ci, F, B), b1 = rb_bin(ci + 1, F, B);
-                            if (v && b0 != b1) {
-                                const double w = double(m) * x;
-                                if (b1 >= 0) atomicAdd(&jl.JE[b1 * kRbMaxStates + s], w);
-                                if (b0 >= 0) atomicAdd(&jl.JE[b0 * kRbMaxStates + s], -w);
-                            }
-                        } else if (v) {
-                            part += x * double(dG[ci]);
-                        }
-                    }
-                    if (kMode == kRbGrad) part = group_sum<L>(part) * double(m);
-                }
-                if (kMode == kRbGrad && lg == s) gp_acc += part;
-            }
-            if (kMode == kRbJac) {
-                // rates: JK[b][k] += sum_g hu_g[b] S_g[k] over the wave's G
-                // units, and the burst's histogram Hb[b] += sum_g hu_g[b];
-                // every entry owned by one lane (lanes over k = 0..K, the
-                // column K being Hb), in a fixed order: no atomics
-                if (lg < K) {
-                    const double si = act ? double(S_in[size_t(u) * size_t(K) + size_t(lg)]) : 0.0;
-                    Sg += si;
-                    jl.Su[g * kRbMaxPairs + lg] = si;
-                }
-                __syncthreads();
-                const int w = K + 1, per = kWave / w;
-                const int k = int(lane) % w, bb = int(lane) / w;
-                if (bb < per) {
-                    double sk[G];
-#pragma unroll
-                    for (int q = 0; q < G; ++q) sk[q] = k < K ? jl.Su[q * kRbMaxPairs + k] : 1.0;
-                    for (int b = bb; b < B; b += per) {
-                        double acc = 0.0;
-#pragma unroll
-                        for (int q = 0; q < G; ++q) acc += jl.hu[q * B + b] * sk[q];
-                        if (k < K) jl.JK[b * kRbMaxPairs + k] += acc;
-                        else jl.Hb[b] += acc;
-                    }
-                }
-                __syncthreads();
-                for (int b = lg; b < B; b += L) hu[b] = 0.0;
-            } else if (act) {
-                H += h;
-                if (lg < K) {
-                    const double si = double(S_in[size_t(u) * size_t(K) + size_t(lg)]);
-                    Sg += si;
-                    Ac += h * si;
-                }
-            }
-        }
-        __syncthreads();
+// Bytes of rb_memo_kernel's dynamic LDS: the n states' pmf tables, then per
+// lane one strip (the pmf) or, building the leave-one-out pmfs, two.
+__host__ __device__ inline size_t rb_memo_lds(int n, bool deriv) {
+    return sizeof(float) * (size_t(n) * kRbMemoTab +
+                            size_t(deriv ? 2 : 1) * kRbMemoWaves * kWave * kRbStride);
+}
+
+// The memo: for each count vector of the list (one lane each) its pmf P
+// into poolP at its offP, and (kDeriv) for its j-th state with a nonzero
+// count s, P_s(c), c < Mt, rounded to float into poolD at its j-th offD.
+// The list runs by total, the offsets follow it, and each wave copies its
+// lanes' blocks out of LDS together: adjacent blocks, coalesced writes.
+template <bool kDeriv>
+__global__ __launch_bounds__(kWave * kRbMemoWaves) void rb_memo_kernel(
+    const RbModel* __restrict__ rbm, const float* __restrict__ pmf_tab,
+    const uint4* __restrict__ list, const uint4* __restrict__ listD, uint32_t nvec,
+    float* __restrict__ poolP, float* __restrict__ poolD) {
+    extern __shared__ float s_f[];
+    __shared__ RbModel sR;
+    __shared__ double s_alpha[kRbMaxStates], s_inv[kRbMaxStates];
+    __shared__ int s_slot[kRbMemoWaves][kWave];
+    __shared__ int s_pos[kRbMemoWaves][kWave];
+    __shared__ uint32_t s_dst[kRbMemoWaves][kWave];
+    const int tid = int(threadIdx.x), lane = tid % kWave, wave = tid / kWave;
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(rbm);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&sR);
+        for (uint32_t i = uint32_t(tid); i < sizeof(RbModel) / 4; i += blockDim.x) dst[i] = src[i];
     }
-    if (kMode == kRbGrad) {
+    __syncthreads();
+    const int n = sR.n;
+    float* tab = s_f;
+    for (int i = tid; i < n * kRbMemoTab; i += int(blockDim.x)) {
+        const int s = i / kRbMemoTab;
+        tab[i] = pmf_tab[size_t(s) * kRbPmfStride + size_t(i - s * kRbMemoTab)];
+    }
+    if (kDeriv && tid < n) {
+        // the recurrence P_s(c) = alpha P_s(c -/+ 1) + P(.) inv
+        const double p = double(sR.p[tid]), q = 1.0 - p;
+        const bool up = p <= 0.5;
+        s_alpha[tid] = up ? -p / q : -q / p;
+        s_inv[tid] = up ? 1.0 / q : 1.0 / p;
+    }
+    __syncthreads();
+    float* strips = tab + n * kRbMemoTab + size_t(wave) * kWave * kRbStride;
+    float* X = strips + lane * kRbStride;
+    float* W = X + size_t(kRbMemoWaves) * kWave * kRbStride;  // kDeriv
+    const uint32_t v = blockIdx.x * blockDim.x + uint32_t(tid);
+    const bool act = v < nvec;
+    const uint4 e = act ? list[v] : make_uint4(0u, 0u, 0u, 0u);
+    const uint2 c = make_uint2(e.x, e.y);
+    // the pmf in X: the states' binomials convolved in state order, each
+    // output summed k ascending from 0.0f (the oracle's order); outputs
+    // descending, so that each overwrites an entry of X no later output reads
+    int xl = 1;
+    X[0] = 1.0f;
 #pragma unroll
-        for (int o = L; o < kWave; o <<= 1) {
-            H += __shfl_xor(H, o);
-            Sg += __shfl_xor(Sg, o);
-            Ac += __shfl_xor(Ac, o);
+    for (int s = 0; s < kRbMaxStates; ++s) {
+        const int m = s < n ? int(rb_count(c, s)) : 0;
+        if (m == 0) continue;
+        const float* Y = tab + s * kRbMemoTab + m * (m + 1) / 2;
+        if (xl == 1) {
+            for (int k = 0; k <= m; ++k) X[k] = Y[k];
+        } else {
+            for (int q = xl + m - 1; q >= 0; --q) {
+                const int k0 = q - (xl - 1) > 0 ? q - (xl - 1) : 0;
+                const int k1 = q < m ? q : m;
+                float z = 0.0f;
+                for (int k = k0; k <= k1; ++k) z = z + Y[k] * X[q - k];
+                X[q] = z;
+            }
         }
-        if (int(lane) < K && a.R > 1)
-            gk_acc += (double(a.R) / double(a.R - 1)) * (Ac - H * Sg / double(a.R));
-    } else if (kMode == kRbJac) {
-        // the burst's mean term: JK[b][k] -= H_b S_k / R (the factor
-        // R / (R - 1) is applied once, when the kernel flushes JK)
+        xl += m;
+    }
+    int* slot = s_slot[wave];
+    int* spos = s_pos[wave];
+    uint32_t* sdst = s_dst[wave];
+    // copy the wave's pmfs out: lane l's block of xl floats at offP
+    auto copy_out = [&](int len, uint32_t dst, int which, float* pool) {
+        const int incl = wave_scan_sum(len, lane);
+        const int pos = incl - len;
+        const int total = __shfl(incl, kWave - 1);
+        spos[lane] = pos;
+        sdst[lane] = dst;
+        wave_windows(len, pos, total, lane, slot, [&](int j, int own) {
+            if (j < total) {
+                const int ai = j - spos[own];
+                const float* src = strips + own * kRbStride +
+                                   (which ? size_t(kRbMemoWaves) * kWave * kRbStride : 0u);
+                pool[sdst[own] + uint32_t(ai)] = src[ai];
+            }
+        });
+    };
+    copy_out(act ? xl : 0, e.z, 0, poolP);
+    if (kDeriv) {
+        const uint4 od = act ? listD[v] : make_uint4(0u, 0u, 0u, 0u);
+        const int Mt = xl - 1;
+        int j = 0;  // the lane's counted states so far
 #pragma unroll
-        for (int o = L; o < kWave; o <<= 1) Sg += __shfl_xor(Sg, o);
-        if (int(lane) < K) jl.Sk[lane] = Sg;
-        __syncthreads();
-        const double inv_r = 1.0 / double(a.R);
-        for (int i = int(lane); i < B * K; i += kWave) {
-            const int b = i / K, k = i - b * K;
-            jl.JK[b * kRbMaxPairs + k] -= jl.Hb[b] * jl.Sk[k] * inv_r;
+        for (int s = 0; s < kRbMaxStates; ++s) {
+            // every lane takes part in each copy (wave-uniform loop)
+            const int m = (act && s < n) ? int(rb_count(c, s)) : 0;
+            if (m > 0) {
+                const bool up = sR.p[s] <= 0.5f;
+                const double al = s_alpha[s], inv = s_inv[s];
+                double x = 0.0;
+                for (int i = 0; i < Mt; ++i) {
+                    x = al * x + double(X[up ? i : Mt - i]) * inv;  // P_s(up ? i : Mt - 1 - i)
+                    W[up ? i : Mt - 1 - i] = float(x);
+                }
+            }
+            const uint32_t dst = j == 0 ? od.x : j == 1 ? od.y : j == 2 ? od.z : od.w;
+            copy_out(m > 0 ? Mt : 0, dst, 1, poolD);
+            j += m > 0 ? 1 : 0;
         }
-        __syncthreads();
-        for (int b = int(lane); b < B; b += kWave) jl.Hb[b] = 0.0;
-        __syncthreads();
     }
 }
 
-// pmf buffers per lane group: the forward convolves in X and Y (in place)
-// in the 16-lane layout and in X, Y and Z in the whole-wave one; a
-// derivative pass without stored pmfs rebuilds them in X, Y and Z; with
-// stored pmfs the adjoint needs X and Z (the cotangent differences), the
-// Jacobian X alone.  Fewer buffers leave LDS for more resident waves.
-__host__ __device__ inline int rb_pmf_buffers(int mode, bool stored, bool small) {
-    if (stored && mode == kRbJac) return 1;
-    if (stored && mode == kRbGrad) return 2;
-    return mode == kRbFwdInPlace && small ? 2 : 3;
+// Bytes per wave of rb_unit_kernel's LDS: the round's units (slot, pos,
+// offP, F, counts; derivative passes: offD; adjoint: h; Jacobian: S - Sbar).
+__host__ __device__ inline size_t rb_unit_wave_bytes(int mode) {
+    size_t b = size_t(kWave) * (4 + 4 + 4 + 4 + 8);
+    if (mode != kRbFwd) b += size_t(kWave) * 16;
+    if (mode == kRbGrad) b += size_t(kWave) * 8;
+    if (mode == kRbJac) b += size_t(kWave) * 8 * kRbMaxPairs;
+    return b;
+}
+// Dynamic LDS of rb_unit_kernel: the 8-byte head (the forward's fixed-point
+// histogram, the adjoint's cotangent, the Jacobian's [B][kRbJacRow] rows),
+// then the waves' regions.
+// The head is rounded to 16 bytes: the waves' regions hold 16-byte entries.
+__host__ __device__ inline int rb_unit_head(int B, int mode) {
+    return ((mode == kRbJac ? B * kRbJacRow : B) + 1) & ~1;
+}
+__host__ __device__ inline size_t rb_unit_lds(int B, int mode) {
+    return sizeof(double) * size_t(rb_unit_head(B, mode)) +
+           size_t(kRbUnitWaves) * rb_unit_wave_bytes(mode);
 }
 
-// Float entries of the pmf buffers: 4 groups x nbuf x 256 (16-lane layout)
-// or nbuf x cap_big (whole-wave layout), whichever is larger.
-__host__ __device__ inline size_t rb_pmf_floats(int cap_big, int mode, bool stored) {
-    const size_t small = size_t(kWave / kRbSmallLanes) *
-                         size_t(rb_pmf_buffers(mode, stored, true)) * kRbSmallCap;
-    const size_t big = size_t(rb_pmf_buffers(mode, stored, false)) * size_t(cap_big);
-    return ((small > big ? small : big) + 1u) & ~size_t(1);  // even: doubles follow
-}
-
-// One wave per burst at a time (bursts from the counters after the walk's).
-// Dynamic LDS: B 8-byte entries (the forward's fixed-point histogram or the
-// adjoint's cotangent; none in the Jacobian) followed by the float buffers,
-// then (Jacobian) the JacLds arrays.  The in-place forward, whose products
-// sit in registers, is held to 128 VGPRs (4 waves per SIMD) so that its
-// smaller LDS can let more waves in; the
-// adjoint is held to 128 VGPRs (4 waves per SIMD instead of 3, a few values
-// spilled to scratch): its deconvolution scans wait on shuffles and LDS, and
-// the extra wave measured 8-14 % faster.  The Jacobian's LDS allows 3 waves
-// per SIMD whatever its registers.
+// Every unit u < a.nunits with Mt < kRbMemoCap.  A wave takes a.batch
+// consecutive units per grab and handles them 64 at a time: the units'
+// outcomes a = 0..Mt, concatenated, are spread over the lanes (wave_windows),
+// so each lane handles one (unit, outcome) pair per window whatever the
+// units' lengths; the pmf entries come from the memo.  No barrier after the
+// prologue: each wave works in its own LDS region and adds to the
+// workgroup's accumulators with atomics.
 template <int kMode>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
-    kMode == kRbGrad || kMode == kRbFwdInPlace ? 4 : 1))) void rb_pmf_kernel(
-    const RbModel* __restrict__ rbm, const double* __restrict__ lf,
-    const float* __restrict__ pmf_tab, const RunArgs a, const BurstDesc* __restrict__ bursts,
-    const uint2* __restrict__ cnt_in, const float* __restrict__ F_in,
-    const float* __restrict__ S_in, const double* __restrict__ g,
+__global__ __launch_bounds__(kWave * kRbUnitWaves) void rb_unit_kernel(
+    const RbModel* __restrict__ rbm, const RunArgs a, const uint2* __restrict__ cnt_in,
+    const float* __restrict__ F_in, const float* __restrict__ S_in,
+    const double* __restrict__ Sbar, const double* __restrict__ g,
+    const uint32_t* __restrict__ rank_offP, const uint4* __restrict__ rank_offD,
+    const float* __restrict__ poolP, const float* __restrict__ poolD,
     unsigned long long* __restrict__ hist_acc, double* __restrict__ grad,
-    uint32_t* __restrict__ work, int cap_big, float* __restrict__ pmf_io,
-    const uint64_t* __restrict__ poff) {
-    constexpr bool kGrad = kMode == kRbGrad || kMode == kRbJac;
+    uint32_t* __restrict__ work) {
+    extern __shared__ __align__(16) double s_unit[];
+    __shared__ RbModel sR;
+    const int tid = int(threadIdx.x), lane = tid % kWave, wave = tid / kWave;
+    const int B = a.B;
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(rbm);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&sR);
+        for (uint32_t i = uint32_t(tid); i < sizeof(RbModel) / 4; i += blockDim.x) dst[i] = src[i];
+    }
+    const int head = rb_unit_head(B, kMode);
+    unsigned long long* s_h = reinterpret_cast<unsigned long long*>(s_unit);
+    double* s_d = s_unit;
+    for (int i = tid; i < head; i += int(blockDim.x)) {
+        if (kMode == kRbGrad) s_d[i] = i < B ? g[i] : 0.0;
+        else if (kMode == kRbJac) s_d[i] = 0.0;
+        else s_h[i] = 0ull;
+    }
+    __syncthreads();
+    const int n = sR.n, K = sR.K;
+    uint8_t* wr = reinterpret_cast<uint8_t*>(s_unit + head) + size_t(wave) * rb_unit_wave_bytes(kMode);
+    int* w_slot = reinterpret_cast<int*>(wr);
+    int* w_pos = w_slot + kWave;
+    uint32_t* w_offP = reinterpret_cast<uint32_t*>(w_pos + kWave);
+    float* w_F = reinterpret_cast<float*>(w_offP + kWave);
+    uint2* w_cnt = reinterpret_cast<uint2*>(w_F + kWave);
+    uint4* w_offD = reinterpret_cast<uint4*>(w_cnt + kWave);   // derivative passes
+    double* w_x = reinterpret_cast<double*>(w_offD + kWave);   // adjoint: h; Jacobian: S - Sbar
+
+    const uint32_t nq = gridDim.x < kAuxQueues ? gridDim.x : kAuxQueues;
+    const uint32_t queue = blockIdx.x % nq;
+    const uint32_t q_begin = uint32_t(uint64_t(a.nunits) * queue / nq);
+    const uint32_t q_len = uint32_t(uint64_t(a.nunits) * (queue + 1u) / nq) - q_begin;
+    uint32_t* counter = &work[kRbUnitCounter + queue * kCounterStride];
+    double gp[kRbMaxStates] = {};  // adjoint: d h / d p_s over the lane's elements
+    double gk[kRbMaxPairs] = {};   // adjoint: h (S - Sbar) over the lane's units
+    while (true) {
+        uint32_t off = 0;
+        if (lane == 0) off = atomicAdd(counter, a.batch);
+        off = __shfl(off, 0);
+        if (off >= q_len) break;
+        const uint32_t end = q_len - off < a.batch ? q_len : off + a.batch;
+        for (uint32_t r0 = off; r0 < end; r0 += kWave) {
+            const uint32_t o = r0 + uint32_t(lane);
+            const bool valid = o < end;
+            const uint32_t u = q_begin + (valid ? o : r0);
+            const uint2 c = valid ? cnt_in[u] : make_uint2(0u, 0u);
+            int mm[kRbMaxStates];
+            int Mt = 0;
+#pragma unroll
+            for (int s = 0; s < kRbMaxStates; ++s) {
+                mm[s] = int(rb_count(c, s));
+                Mt += mm[s];
+            }
+            // units with Mt >= kRbMemoCap: listed by the walk for rb_big_kernel
+            const bool memo = valid && Mt < kRbMemoCap;
+            const int len = memo ? Mt + 1 : 0;
+            if (memo) {
+                const uint32_t rk = rb_rank(mm, n, kRbMemoCap - 1);
+                w_offP[lane] = rank_offP[rk];
+                w_F[lane] = F_in[u];
+                w_cnt[lane] = c;
+                if (kMode != kRbFwd) w_offD[lane] = rank_offD[rk];
+                if (kMode == kRbGrad) w_x[lane] = 0.0;
+                if (kMode == kRbJac) {
+                    const uint32_t bl = u / a.R;
+#pragma unroll
+                    for (int k = 0; k < kRbMaxPairs; ++k)
+                        if (k < K)
+                            w_x[lane * kRbMaxPairs + k] =
+                                double(S_in[size_t(u) * size_t(K) + size_t(k)]) -
+                                Sbar[size_t(bl) * size_t(K) + size_t(k)];
+                }
+            }
+            const int incl = wave_scan_sum(len, lane);
+            const int pos = incl - len;
+            const int total = __shfl(incl, kWave - 1);
+            w_pos[lane] = pos;
+            wave_windows(len, pos, total, lane, w_slot, [&](int j, int own) {
+                const bool act = j < total;
+                const int i = act ? own : 0;
+                const int ai = act ? j - w_pos[i] : 0;
+                const float Fi = act ? w_F[i] : 0.0f;
+                const float P = act ? poolP[w_offP[i] + uint32_t(ai)] : 0.0f;
+                const int b0 = act ? rb_bin(ai, Fi, B) : -1;
+                if (kMode == kRbFwd) {
+                    if (b0 >= 0) atomicAdd(&s_h[b0], (unsigned long long)rb_fixed(P));
+                    return;
+                }
+                const uint2 ci = w_cnt[i];
+                int Mi = 0;
+#pragma unroll
+                for (int s = 0; s < kRbMaxStates; ++s) Mi += int(rb_count(ci, s));
+                // the next outcome's bin: from the next lane, which holds it
+                // unless this lane is the window's last
+                int b1 = __shfl_down(b0, 1);
+                const bool inner = act && ai < Mi;
+                if (inner && lane == kWave - 1) b1 = rb_bin(ai + 1, Fi, B);
+                const uint4 od = w_offD[i];
+                if (kMode == kRbGrad) {
+                    const double G0 = b0 >= 0 ? s_d[b0] : 0.0;
+                    double hv = G0 * double(P);
+                    if (inner) {
+                        const double G1 = b1 >= 0 ? s_d[b1] : 0.0;
+                        const double dG = double(float(G1 - G0));
+                        int js = 0;
+#pragma unroll
+                        for (int s = 0; s < kRbMaxStates; ++s) {
+                            const int m = int(rb_count(ci, s));
+                            if (m > 0) {
+                                const uint32_t base = js == 0 ? od.x : js == 1 ? od.y : js == 2 ? od.z : od.w;
+                                gp[s] += double(m) * double(poolD[base + uint32_t(ai)]) * dG;
+                                ++js;
+                            }
+                        }
+                    }
+                    // h per unit: a sum over the unit's lanes of the window
+                    // (segments of consecutive lanes), added by its last lane
+                    const int s0 = act ? (w_pos[i] > j - lane ? w_pos[i] - (j - lane) : 0) : lane;
+#pragma unroll
+                    for (int d = 1; d < kWave; d <<= 1) {
+                        const double t = __shfl_up(hv, d);
+                        if (lane - d >= s0) hv += t;
+                    }
+                    if (act && (ai == Mi || lane == kWave - 1)) w_x[i] += hv;
+                } else {
+                    if (b0 >= 0) {
+                        double* row = s_d + b0 * kRbJacRow + kRbMaxStates;
+                        const double* sc = w_x + i * kRbMaxPairs;
+#pragma unroll
+                        for (int k = 0; k < kRbMaxPairs; ++k)
+                            if (k < K) atomicAdd(&row[k], double(P) * sc[k]);
+                    }
+                    if (inner && b0 != b1) {
+                        int js = 0;
+#pragma unroll
+                        for (int s = 0; s < kRbMaxStates; ++s) {
+                            const int m = int(rb_count(ci, s));
+                            if (m > 0) {
+                                const uint32_t base = js == 0 ? od.x : js == 1 ? od.y : js == 2 ? od.z : od.w;
+                                const double w = double(m) * double(poolD[base + uint32_t(ai)]);
+                                if (b1 >= 0) atomicAdd(&s_d[b1 * kRbJacRow + s], w);
+                                if (b0 >= 0) atomicAdd(&s_d[b0 * kRbJacRow + s], -w);
+                                ++js;
+                            }
+                        }
+                    }
+                }
+            });
+            if (kMode == kRbGrad && memo) {
+                const uint32_t bl = u / a.R;
+                const double h = w_x[lane];
+#pragma unroll
+                for (int k = 0; k < kRbMaxPairs; ++k)
+                    if (k < K)
+                        gk[k] += h * (double(S_in[size_t(u) * size_t(K) + size_t(k)]) -
+                                      Sbar[size_t(bl) * size_t(K) + size_t(k)]);
+            }
+        }
+    }
+    if (kMode == kRbFwd) {
+        __syncthreads();
+        unsigned long long* row = hist_acc + size_t(blockIdx.x & 7u) * kMaxBins;
+        for (int i = tid; i < B; i += int(blockDim.x))
+            if (s_h[i]) atomicAdd(&row[i], s_h[i]);
+    } else if (kMode == kRbGrad) {
+        const double f = a.R > 1 ? double(a.R) / double(a.R - 1) : 0.0;
+#pragma unroll
+        for (int s = 0; s < kRbMaxStates; ++s) {
+            const double v = wave_sum(gp[s]);
+            if (lane == 0 && s < n) atomicAdd(&grad[s], v);
+        }
+#pragma unroll
+        for (int k = 0; k < kRbMaxPairs; ++k) {
+            const double v = wave_sum(gk[k]);
+            if (lane == 0 && k < K) atomicAdd(&grad[n + k], f * v);
+        }
+    } else {
+        // grad holds J[b][i], i < n + K (efficiencies, then rates in args
+        // order), B rows of kRbJacCols
+        __syncthreads();
+        const double f = a.R > 1 ? double(a.R) / double(a.R - 1) : 0.0;
+        const int w = n + K;
+        for (int i = tid; i < B * w; i += int(blockDim.x)) {
+            const int b = i / w, col = i - b * w;
+            const double v = col < n ? s_d[b * kRbJacRow + col]
+                                     : f * s_d[b * kRbJacRow + kRbMaxStates + (col - n)];
+            if (v != 0.0) atomicAdd(&grad[size_t(b) * kRbJacCols + size_t(col)], v);
+        }
+    }
+}
+
+// Bytes of rb_big_kernel's dynamic LDS: B 8-byte entries (the forward's
+// histogram, the adjoint's cotangent) or the Jacobian's [B][kRbJacCols]
+// rows, then three pmf buffers (X, Y, Z) of cap floats.
+__host__ __device__ inline size_t rb_big_lds(int B, int cap, int mode) {
+    const size_t head = size_t(mode == kRbJac ? B * kRbJacCols : B);
+    return sizeof(double) * head + sizeof(float) * 3u * size_t(cap);
+}
+
+// The units the walk listed (Mt >= kRbMemoCap), one wave per unit, lanes
+// over acceptor totals; cap = the largest Mt + 1 of the run's bursts.
+template <int kMode>
+__global__ __launch_bounds__(kWave) void rb_big_kernel(
+    const RbModel* __restrict__ rbm, const double* __restrict__ lf,
+    const float* __restrict__ pmf_tab, const RunArgs a, const uint2* __restrict__ cnt_in,
+    const float* __restrict__ F_in, const float* __restrict__ S_in,
+    const double* __restrict__ Sbar, const double* __restrict__ g,
+    const uint32_t* __restrict__ big, const uint32_t* __restrict__ big_count,
+    unsigned long long* __restrict__ hist_acc, double* __restrict__ grad, int cap) {
     extern __shared__ double s_dyn[];
     __shared__ RbModel sR;
-    // B 8-byte entries first (the forward's histogram, the adjoint's
-    // cotangent; none for the Jacobian)
-    const int head = kMode == kRbJac ? 0 : a.B;
-    unsigned long long* s_h = reinterpret_cast<unsigned long long*>(s_dyn);
-    double* s_g = s_dyn;
-    float* s_buf = reinterpret_cast<float*>(s_dyn + head);
-    const int B = a.B;
-    JacLds jl;
-    const bool stored = pmf_io != nullptr;
-    jl.JE = s_dyn + head + rb_pmf_floats(cap_big, kMode, stored) / 2;
-    jl.JK = jl.JE + B * kRbMaxStates;
-    jl.hu = jl.JK + B * kRbMaxPairs;
-    jl.Hb = jl.hu + B * (kWave / kRbSmallLanes);
-    jl.Sk = jl.Hb + B;
-    jl.Su = jl.Sk + kRbMaxPairs;
     const uint32_t lane = threadIdx.x;
+    const int B = a.B;
+    const uint32_t count = *big_count;
+    if (blockIdx.x >= count) return;  // whole workgroup: no barrier below is skipped by part
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(rbm);
         uint32_t* dst = reinterpret_cast<uint32_t*>(&sR);
         for (uint32_t i = lane; i < sizeof(RbModel) / 4; i += kWave) dst[i] = src[i];
-        for (uint32_t i = lane; i < uint32_t(head); i += kWave) {
-            if (kMode == kRbGrad) s_g[i] = g[i];
-            else s_h[i] = 0ull;
-        }
-        if (kMode == kRbJac)
-            for (size_t i = lane; i < rb_jac_doubles(B); i += kWave) jl.JE[i] = 0.0;
+    }
+    const int head = kMode == kRbJac ? B * kRbJacCols : B;
+    unsigned long long* s_h = reinterpret_cast<unsigned long long*>(s_dyn);
+    double* s_d = s_dyn;
+    float* bufs = reinterpret_cast<float*>(s_dyn + head);
+    for (int i = int(lane); i < head; i += kWave) {
+        if (kMode == kRbGrad) s_d[i] = g[i];
+        else if (kMode == kRbJac) s_d[i] = 0.0;
+        else s_h[i] = 0ull;
     }
     __syncthreads();
-    const uint32_t nb = a.nunits / a.R;
-    // every counter must have a workgroup: fewer queues on a small grid
-    const uint32_t nq = gridDim.x < kAuxQueues ? gridDim.x : kAuxQueues;
-    const uint32_t waves_q = gridDim.x / nq;  // >= 1 workgroups per queue
-    uint32_t queue = blockIdx.x % nq, visited = 1;
-    uint32_t qb = uint32_t(uint64_t(nb) * queue / nq);
-    uint32_t qn = uint32_t(uint64_t(nb) * (queue + 1u) / nq) - qb;
-    uint32_t* counter = &work[kRbPmfCounter + queue * kCounterStride];
-    // The forward takes fixed batches (about 16 grabs per workgroup).  The
-    // derivative passes, several times dearer per burst, take guided batches
-    // (a quarter of the wave's share of what its queue had left at its last
-    // grab) and then single bursts from the other queues.
-    constexpr bool kGuided = kGrad;
-    uint32_t batch = kGuided ? qn / (waves_q * 4u) : nb / (gridDim.x * 16u);
-    batch = batch < 1u ? 1u : (batch > kRbMaxBurstBatch ? kRbMaxBurstBatch : batch);
-    double gp_acc = 0.0, gk_acc = 0.0;  // lane s: d/dp_s; lane i: d/dk_i
-    uint32_t next = 0, end = 0;  // the wave's current batch of bursts
-    while (true) {
-        if (next >= end) {
-            uint32_t off = 0;
-            if (lane == 0) off = atomicAdd(counter, batch);
-            off = __shfl(off, 0);
-            if (off >= qn) {
-                // queue drained: help the next one until every queue has
-                // been seen drained
-                if (!kGuided || visited == nq) break;
-                ++visited;
-                queue = queue + 1u == nq ? 0u : queue + 1u;
-                qb = uint32_t(uint64_t(nb) * queue / nq);
-                qn = uint32_t(uint64_t(nb) * (queue + 1u) / nq) - qb;
-                counter = &work[kRbPmfCounter + queue * kCounterStride];
-                batch = 1u;
+    const int n = sR.n, K = sR.K;
+    double gp_acc = 0.0, gk_acc = 0.0;  // adjoint: lane s holds d/dp_s, lane k d/dk_k
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const uint32_t u = big[i];
+        const uint2 c = cnt_in[u];
+        const float F = F_in[u];
+        int bX = 0, bY = 1, bZ = 2;
+        int xl = 1;
+        if (lane == 0) bufs[0] = 1.0f;
+#pragma unroll
+        for (int s = 0; s < kRbMaxStates; ++s) {
+            const int m = s < n ? int(rb_count(c, s)) : 0;
+            float* Y = bufs + bY * cap;
+            for (int k = int(lane); m > 0 && k <= m; k += kWave)
+                Y[k] = m < kRbPmfTable ? pmf_tab[rb_pmf_index(s, m, k)]
+                                       : float(rb_binom_pmf(m, k, lf, sR.p[s], sR.lp[s], sR.lq[s]));
+            __syncthreads();
+            if (m > 0) {
+                if (xl == 1) {  // X = [1]: the product is Y itself, bit for bit
+                    const int t = bX;
+                    bX = bY;
+                    bY = t;
+                    xl = m + 1;
+                } else {
+                    const float* X = bufs + bX * cap;
+                    float* Z = bufs + bZ * cap;
+                    for (int q = int(lane); q < xl + m; q += kWave)
+                        Z[q] = rb_conv_at(Y, m + 1, X, xl, q);
+                    const int t = bX;
+                    bX = bZ;
+                    bZ = t;
+                    xl += m;
+                }
+            }
+            __syncthreads();
+        }
+        const float* P = bufs + bX * cap;
+        if (kMode == kRbFwd) {
+            for (int q = int(lane); q < xl; q += kWave) {
+                const int bin = rb_bin(q, F, B);
+                if (bin >= 0) atomicAdd(&s_h[bin], (unsigned long long)rb_fixed(P[q]));
+            }
+            __syncthreads();
+            continue;
+        }
+        const int Mt = xl - 1;
+        const uint32_t bl = u / a.R;
+        double sc[kRbMaxPairs];  // Jacobian: the unit's score less its burst's mean
+#pragma unroll
+        for (int k = 0; k < kRbMaxPairs; ++k)
+            sc[k] = kMode == kRbJac && k < K
+                        ? double(S_in[size_t(u) * size_t(K) + size_t(k)]) -
+                              Sbar[size_t(bl) * size_t(K) + size_t(k)]
+                        : 0.0;
+        float* dG = bufs + bY * cap;  // adjoint: G(bin(c + 1)) - G(bin(c))
+        double hp = 0.0;
+        for (int q = int(lane); q < xl; q += kWave) {
+            const int b0 = rb_bin(q, F, B);
+            if (kMode == kRbJac) {
+                if (b0 >= 0) {
+                    const double w = double(P[q]);
+                    double* row = s_d + b0 * kRbJacCols + kRbMaxStates;
+#pragma unroll
+                    for (int k = 0; k < kRbMaxPairs; ++k)
+                        if (k < K) atomicAdd(&row[k], w * sc[k]);
+                }
                 continue;
             }
-            next = qb + off;
-            end = off + batch < qn ? next + batch : qb + qn;
-            if (kGuided) {
-                const uint32_t left = off + batch < qn ? qn - off - batch : 0u;
-                batch = left / (waves_q * 4u);
-                batch = batch < 1u ? 1u : (batch > kRbMaxBurstBatch ? kRbMaxBurstBatch : batch);
+            const double G0 = b0 >= 0 ? s_d[b0] : 0.0;
+            hp += G0 * double(P[q]);
+            if (q + 1 < xl) {
+                const int b1 = rb_bin(q + 1, F, B);
+                dG[q] = float((b1 >= 0 ? s_d[b1] : 0.0) - G0);
             }
         }
-        const uint32_t bl = next++;
-        const int bound = rb_total_bound(bursts[a.begin + bl].P, sR.coef);
-        if (bound < kRbSmallCap)
-            rb_burst<kRbSmallLanes, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h,
-                                           s_buf, rb_pmf_buffers(kMode, stored, true),
-                                           kRbSmallCap, lane, gp_acc, gk_acc, jl,
-                                           pmf_io, poff);
-        else
-            rb_burst<kWave, kMode>(sR, lf, pmf_tab, a, bl, cnt_in, F_in, S_in, s_g, s_h, s_buf,
-                                   rb_pmf_buffers(kMode, stored, false), cap_big, lane,
-                                   gp_acc, gk_acc, jl, pmf_io, poff);
-    }
-    if (kMode == kRbJac) {
-        // grad holds J[b][i], i < n + K (efficiencies, then rates in args
-        // order), B rows of kRbMaxStates + kRbMaxPairs
         __syncthreads();
-        const int n = sR.n, K = sR.K, w = n + K;
-        const double f = a.R > 1 ? double(a.R) / double(a.R - 1) : 0.0;
-        for (int i = int(lane); i < B * w; i += kWave) {
-            const int b = i / w, c = i - b * w;
-            const double v = c < n ? jl.JE[b * kRbMaxStates + c] : f * jl.JK[b * kRbMaxPairs + (c - n)];
-            if (v != 0.0) atomicAdd(&grad[size_t(b) * (kRbMaxStates + kRbMaxPairs) + size_t(c)], v);
-        }
-    } else if (!kGrad) {
-        __syncthreads();
-        unsigned long long* row = hist_acc + size_t(blockIdx.x & 7u) * kMaxBins;
-        for (uint32_t i = lane; i < uint32_t(a.B); i += kWave)
-            if (s_h[i]) atomicAdd(&row[i], s_h[i]);
-    } else {
-        // every group's lane s holds a share of d/dp_s
+        const double h = wave_sum(hp);
 #pragma unroll
-        for (int o = kRbSmallLanes; o < kWave; o <<= 1) gp_acc += __shfl_xor(gp_acc, o);
-        if (int(lane) < sR.n) atomicAdd(&grad[lane], gp_acc);
-        if (int(lane) < sR.K) atomicAdd(&grad[sR.n + int(lane)], gk_acc);
+        for (int s = 0; s < kRbMaxStates; ++s) {
+            const int m = s < n ? int(rb_count(c, s)) : 0;
+            if (m == 0) continue;  // uniform over the wave
+            const double p = double(sR.p[s]), q = 1.0 - p;
+            const bool up = p <= 0.5;
+            const double alpha = up ? -p / q : -q / p;
+            const double inv = up ? 1.0 / q : 1.0 / p;
+            double carry = 0.0, part = 0.0;
+            for (int j0 = 0; j0 < Mt; j0 += kWave) {
+                // lanes over 64 steps of the recurrence: a scan of the
+                // affine maps x -> alpha x + P(.) inv
+                const int j = j0 + int(lane);
+                const bool v = j < Mt;
+                double A = v ? alpha : 1.0;
+                double Bv = v ? double(P[up ? j : Mt - j]) * inv : 0.0;
+#pragma unroll
+                for (int d = 1; d < kWave; d <<= 1) {
+                    const double A2 = __shfl_up(A, d);
+                    const double B2 = __shfl_up(Bv, d);
+                    if (int(lane) >= d) {
+                        Bv = A * B2 + Bv;
+                        A = A * A2;
+                    }
+                }
+                const double x = A * carry + Bv;  // P_s(up ? j : Mt - 1 - j)
+                carry = __shfl(x, kWave - 1);
+                const int ci = up ? j : Mt - 1 - j;
+                if (kMode == kRbJac) {
+                    const int b0 = rb_bin(ci, F, B), b1 = rb_bin(ci + 1, F, B);
+                    if (v && b0 != b1) {
+                        const double w = double(m) * x;
+                        if (b1 >= 0) atomicAdd(&s_d[b1 * kRbJacCols + s], w);
+                        if (b0 >= 0) atomicAdd(&s_d[b0 * kRbJacCols + s], -w);
+                    }
+                } else if (v) {
+                    part += x * double(dG[ci]);
+                }
+            }
+            if (kMode == kRbGrad) {
+                part = wave_sum(part) * double(m);
+                if (int(lane) == s) gp_acc += part;
+            }
+        }
+        if (kMode == kRbGrad && int(lane) < K)
+            gk_acc += h * (double(S_in[size_t(u) * size_t(K) + lane]) -
+                           Sbar[size_t(bl) * size_t(K) + lane]);
+        __syncthreads();
+    }
+    if (kMode == kRbFwd) {
+        unsigned long long* row = hist_acc + size_t(blockIdx.x & 7u) * kMaxBins;
+        for (int i = int(lane); i < B; i += kWave)
+            if (s_h[i]) atomicAdd(&row[i], s_h[i]);
+    } else if (kMode == kRbGrad) {
+        const double f = a.R > 1 ? double(a.R) / double(a.R - 1) : 0.0;
+        if (int(lane) < n) atomicAdd(&grad[lane], gp_acc);
+        if (int(lane) < K) atomicAdd(&grad[n + int(lane)], f * gk_acc);
+    } else {
+        const double f = a.R > 1 ? double(a.R) / double(a.R - 1) : 0.0;
+        const int w = n + K;
+        for (int i = int(lane); i < B * w; i += kWave) {
+            const int b = i / w, col = i - b * w;
+            const double v = col < n ? s_d[b * kRbJacCols + col]
+                                     : f * s_d[b * kRbJacCols + kRbMaxStates + (col - n)];
+            if (v != 0.0) atomicAdd(&grad[size_t(b) * kRbJacCols + size_t(col)], v);
+        }
+    }
+}
+
+// Per burst bl and rate k: Sbar[bl K + k] = (1 / R) sum_r S[(bl R + r) K + k],
+// summed r ascending in double.  A workgroup stages the scores of a run of
+// whole bursts in LDS with coalesced loads, then one thread per (burst,
+// rate) sums them; bursts whose R K scores exceed the stage are summed
+// straight from global memory.
+constexpr uint32_t kRbSbarStage = 8192;  // floats
+__global__ __launch_bounds__(256) void rb_score_mean_kernel(const RunArgs a, int K,
+                                                            const float* __restrict__ S,
+                                                            double* __restrict__ Sbar) {
+    __shared__ float st[kRbSbarStage];
+    const uint64_t nb = a.nunits / a.R;
+    const uint32_t per = a.R * uint32_t(K);  // scores per burst
+    const uint32_t bpb = per <= kRbSbarStage ? kRbSbarStage / per : 0u;
+    if (bpb == 0) {
+        for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nb * uint64_t(K);
+             i += uint64_t(gridDim.x) * blockDim.x) {
+            const uint64_t bl = i / uint64_t(K), k = i - bl * uint64_t(K);
+            const float* src = S + bl * per + k;
+            double sum = 0.0;
+            for (uint32_t r = 0; r < a.R; ++r) sum += double(src[uint64_t(r) * uint64_t(K)]);
+            Sbar[i] = sum / double(a.R);
+        }
+        return;
+    }
+    for (uint64_t b0 = uint64_t(blockIdx.x) * bpb; b0 < nb; b0 += uint64_t(gridDim.x) * bpb) {
+        const uint32_t nbh = uint32_t(nb - b0 < bpb ? nb - b0 : bpb);
+        const float* src = S + b0 * per;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nbh * per; i += blockDim.x) st[i] = src[i];
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < nbh * uint32_t(K); t += blockDim.x) {
+            const uint32_t bl = t / uint32_t(K), k = t - bl * uint32_t(K);
+            const float* x = st + bl * per + k;
+            double sum = 0.0;
+            for (uint32_t r = 0; r < a.R; ++r) sum += double(x[r * uint32_t(K)]);
+            Sbar[(b0 + bl) * uint64_t(K) + k] = sum / double(a.R);
+        }
     }
 }
 
 // The binomial pmfs of every state for counts m < kRbPmfTable (one thread
-// per entry), read by rb_pmf_kernel instead of recomputing them per unit.
+// per entry), read by the pmf passes instead of recomputing them per unit.
 __global__ void rb_table_kernel(const RbModel* __restrict__ rbm, const double* __restrict__ lf,
                                 float* __restrict__ pmf_tab) {
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
@@ -632,34 +902,37 @@ __global__ __launch_bounds__(kWave) void rb_finish_kernel(
 }
 
 hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu) {
+    const size_t l = size_t(dyn_lds);
+    const int tu = kWave * kRbUnitWaves;
     switch (which) {
         case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_walk_kernel<false>, kWave, 0);
         case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_walk_kernel<true>, kWave, 0);
-        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbFwd>, kWave, size_t(dyn_lds));
-        case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbFwdInPlace>, kWave, size_t(dyn_lds));
-        case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbGrad>, kWave, size_t(dyn_lds));
-        default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_pmf_kernel<kRbJac>, kWave, size_t(dyn_lds));
+        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kRbFwd>, tu, l);
+        case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kRbGrad>, tu, l);
+        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kRbJac>, tu, l);
+        case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbFwd>, kWave, l);
+        case 6: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbGrad>, kWave, l);
+        default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbJac>, kWave, l);
     }
 }
 
 hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
                           const uint2* links, const ModelBlob* blob, const RbModel* rbm,
                           const RunArgs& a, uint2* cnt, float* F, float* S, uint32_t* work,
-                          hipStream_t stream) {
+                          uint32_t* big, uint32_t* big_count, hipStream_t stream) {
     if (scores)
         hipLaunchKernelGGL(rb_walk_kernel<true>, dim3(grid), dim3(kWave), 0, stream, bursts, ticks,
-                           links, blob, rbm, a, cnt, F, S, work);
+                           links, blob, rbm, a, cnt, F, S, work, big, big_count);
     else
         hipLaunchKernelGGL(rb_walk_kernel<false>, dim3(grid), dim3(kWave), 0, stream, bursts,
-                           ticks, links, blob, rbm, a, cnt, F, S, work);
+                           ticks, links, blob, rbm, a, cnt, F, S, work, big, big_count);
     return hipGetLastError();
 }
 
-size_t rb_pmf_lds(int B, int cap_big, int mode, bool stored) {
-    return sizeof(double) * size_t(mode == kRbJac ? 0 : B) +
-           sizeof(float) * rb_pmf_floats(cap_big, mode, stored) +
-           (mode == 2 ? sizeof(double) * rb_jac_doubles(B) : 0u);
-}
+int rb_unit_threads() { return kWave * kRbUnitWaves; }
+size_t rb_unit_lds_bytes(int B, int mode) { return rb_unit_lds(B, mode); }
+size_t rb_big_lds_bytes(int B, int cap, int mode) { return rb_big_lds(B, cap, mode); }
+size_t rb_memo_lds_bytes(int n, bool deriv) { return rb_memo_lds(n, deriv); }
 
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
                            hipStream_t stream) {
@@ -669,29 +942,66 @@ hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab,
     return hipGetLastError();
 }
 
-hipError_t launch_rb_pmf(int mode, uint32_t grid, int cap_big, const RbModel* rbm,
-                         const double* lf, const float* pmf_tab, const RunArgs& a,
-                         const BurstDesc* bursts, const uint2* cnt, const float* F,
-                         const float* S, const double* g, unsigned long long* hist_acc,
-                         double* grad, uint32_t* work, float* pmf_io, const uint64_t* poff,
-                         hipStream_t stream) {
-    const size_t lds = rb_pmf_lds(a.B, cap_big, mode, pmf_io != nullptr);
-    if (mode == kRbGrad)
-        hipLaunchKernelGGL(rb_pmf_kernel<kRbGrad>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
-                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big,
-                           pmf_io, poff);
-    else if (mode == kRbJac)
-        hipLaunchKernelGGL(rb_pmf_kernel<kRbJac>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
-                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big,
-                           pmf_io, poff);
-    else if (mode == kRbFwdInPlace)
-        hipLaunchKernelGGL(rb_pmf_kernel<kRbFwdInPlace>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
-                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big,
-                           pmf_io, poff);
+hipError_t launch_rb_memo(bool deriv, int n, const RbModel* rbm, const float* pmf_tab,
+                          const uint4* list, const uint4* listD, uint32_t nvec, float* poolP,
+                          float* poolD, hipStream_t stream) {
+    const uint32_t threads = kWave * kRbMemoWaves;
+    const dim3 grid((nvec + threads - 1) / threads);
+    const size_t lds = rb_memo_lds(n, deriv);
+    if (deriv)
+        hipLaunchKernelGGL(rb_memo_kernel<true>, grid, dim3(threads), lds, stream, rbm, pmf_tab,
+                           list, listD, nvec, poolP, poolD);
     else
-        hipLaunchKernelGGL(rb_pmf_kernel<kRbFwd>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
-                           pmf_tab, a, bursts, cnt, F, S, g, hist_acc, grad, work, cap_big,
-                           pmf_io, poff);
+        hipLaunchKernelGGL(rb_memo_kernel<false>, grid, dim3(threads), lds, stream, rbm, pmf_tab,
+                           list, listD, nvec, poolP, poolD);
+    return hipGetLastError();
+}
+
+hipError_t launch_rb_unit(int mode, uint32_t grid, const RbModel* rbm, const RunArgs& a,
+                          const uint2* cnt, const float* F, const float* S, const double* Sbar,
+                          const double* g, const uint32_t* rank_offP, const uint4* rank_offD,
+                          const float* poolP, const float* poolD, unsigned long long* hist_acc,
+                          double* grad, uint32_t* work, hipStream_t stream) {
+    const size_t lds = rb_unit_lds(a.B, mode);
+    const dim3 block(uint32_t(kWave * kRbUnitWaves));
+    if (mode == kRbGrad)
+        hipLaunchKernelGGL(rb_unit_kernel<kRbGrad>, dim3(grid), block, lds, stream, rbm, a, cnt, F,
+                           S, Sbar, g, rank_offP, rank_offD, poolP, poolD, hist_acc, grad, work);
+    else if (mode == kRbJac)
+        hipLaunchKernelGGL(rb_unit_kernel<kRbJac>, dim3(grid), block, lds, stream, rbm, a, cnt, F,
+                           S, Sbar, g, rank_offP, rank_offD, poolP, poolD, hist_acc, grad, work);
+    else
+        hipLaunchKernelGGL(rb_unit_kernel<kRbFwd>, dim3(grid), block, lds, stream, rbm, a, cnt, F,
+                           S, Sbar, g, rank_offP, rank_offD, poolP, poolD, hist_acc, grad, work);
+    return hipGetLastError();
+}
+
+hipError_t launch_rb_big(int mode, uint32_t grid, int cap, const RbModel* rbm, const double* lf,
+                         const float* pmf_tab, const RunArgs& a, const uint2* cnt, const float* F,
+                         const float* S, const double* Sbar, const double* g,
+                         const uint32_t* big, const uint32_t* big_count,
+                         unsigned long long* hist_acc, double* grad, hipStream_t stream) {
+    const size_t lds = rb_big_lds(a.B, cap, mode);
+    if (mode == kRbGrad)
+        hipLaunchKernelGGL(rb_big_kernel<kRbGrad>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
+                           pmf_tab, a, cnt, F, S, Sbar, g, big, big_count, hist_acc, grad, cap);
+    else if (mode == kRbJac)
+        hipLaunchKernelGGL(rb_big_kernel<kRbJac>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
+                           pmf_tab, a, cnt, F, S, Sbar, g, big, big_count, hist_acc, grad, cap);
+    else
+        hipLaunchKernelGGL(rb_big_kernel<kRbFwd>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
+                           pmf_tab, a, cnt, F, S, Sbar, g, big, big_count, hist_acc, grad, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_rb_score_mean(const RunArgs& a, int K, const float* S, double* Sbar,
+                                hipStream_t stream) {
+    const uint64_t nb = a.nunits / a.R;
+    const uint64_t per = uint64_t(a.R) * uint64_t(K);
+    const uint64_t bpb = per <= kRbSbarStage ? kRbSbarStage / per : 0;
+    const uint64_t want = bpb ? (nb + bpb - 1) / bpb : (nb * uint64_t(K) + 255) / 256;
+    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(want, 65536)));
+    hipLaunchKernelGGL(rb_score_mean_kernel, dim3(grid), dim3(256), 0, stream, a, K, S, Sbar);
     return hipGetLastError();
 }
 
