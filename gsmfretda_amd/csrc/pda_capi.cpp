@@ -264,6 +264,12 @@ int pda_destroy(pda_ctx* ctx) {
         free_population(s.pop);
         free_rb(s.rb);
         for (auto& r : s.rbpop) free_rb(r);
+        if (s.rb_aux) {
+            (void)hipStreamSynchronize(s.rb_aux);
+            (void)hipStreamDestroy(s.rb_aux);
+        }
+        if (s.rb_fork) (void)hipEventDestroy(s.rb_fork);
+        if (s.rb_join) (void)hipEventDestroy(s.rb_join);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
     free_data(ctx);

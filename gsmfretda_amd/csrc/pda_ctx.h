@@ -121,6 +121,7 @@ struct RbState {
     float* d_poolP = nullptr;
     float* d_poolD = nullptr;
     size_t cap_poolP = 0, cap_poolD = 0;
+    bool sbar_valid = false;           // d_Sbar holds the last forward's means
     double* h_grad = nullptr;          // pinned
     hipEvent_t ev0 = nullptr, ev_walk = nullptr, ev1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;
     bool ready = false;                // every fixed-size buffer above allocated
@@ -163,6 +164,10 @@ struct Slot {
     hipEvent_t ev0 = nullptr, ev_mid = nullptr, ev1 = nullptr;
     Population pop;
     RbState rb;
+    // RB forward: the pmf table and memo are built on rb_aux while the walk
+    // runs on stream (fork after the model copy, join before the pmf pass)
+    hipStream_t rb_aux = nullptr;
+    hipEvent_t rb_fork = nullptr, rb_join = nullptr;
     std::vector<RbState> rbpop;  // RB population: one state per set
     int rbpop_sets = 0;          // sets of the last RB population forward
 };
@@ -284,7 +289,7 @@ hipError_t launch_rb_big(int mode, uint32_t grid, int cap, const RbModel* rbm, c
                          const uint32_t* big, const uint32_t* big_count,
                          unsigned long long* hist_acc, double* grad, hipStream_t stream);
 hipError_t launch_rb_score_mean(const RunArgs& a, int K, const float* S, double* Sbar,
-                                hipStream_t stream);
+                                int num_cu, hipStream_t stream);
 hipError_t launch_rb_finish(const RunArgs& a, const float* obs, unsigned long long* hist_acc,
                             uint32_t* work, void* res, hipStream_t stream);
 

@@ -405,25 +405,33 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
         r.cap_poolD = ML.poolD;
     }
 
+    if (!s.rb_aux) HIP_TRY(hipStreamCreateWithFlags(&s.rb_aux, hipStreamNonBlocking));
+    if (!s.rb_fork) HIP_TRY(hipEventCreateWithFlags(&s.rb_fork, hipEventDisableTiming));
+    if (!s.rb_join) HIP_TRY(hipEventCreateWithFlags(&s.rb_join, hipEventDisableTiming));
     HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
     HIP_TRY(hipMemsetAsync(r.d_big_count, 0, sizeof(uint32_t), s.stream));
     HIP_TRY(hipMemcpyAsync(r.d_rbm, r.h_rbm, sizeof(RbModel), hipMemcpyHostToDevice, s.stream));
     const RbModel* d_rbm = static_cast<const RbModel*>(r.d_rbm);
     HIP_TRY(hipEventRecord(r.ev0, s.stream));
+    // the pmf table and memo depend on the model only: built on the side
+    // stream while the walk runs (the memo's few, long waves hide under it)
+    HIP_TRY(hipEventRecord(s.rb_fork, s.stream));
+    HIP_TRY(hipStreamWaitEvent(s.rb_aux, s.rb_fork, 0));
+    HIP_TRY(launch_rb_table(d_rbm, ctx->d_lf, r.d_pmf_tab, n, s.rb_aux));
+    HIP_TRY(launch_rb_memo(scores, n, d_rbm, r.d_pmf_tab, ML.d_list, ML.d_listD, ML.nvec,
+                           r.d_poolP, r.d_poolD, s.rb_aux));
+    HIP_TRY(hipEventRecord(s.rb_join, s.rb_aux));
     HIP_TRY(launch_rb_walk(scores, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, d_blob,
                            d_rbm, a, r.d_cnt, r.d_F, r.d_S, r.d_work, r.d_big, r.d_big_count,
                            s.stream));
     HIP_TRY(hipEventRecord(r.ev_walk, s.stream));
-    HIP_TRY(launch_rb_table(d_rbm, ctx->d_lf, r.d_pmf_tab, n, s.stream));
-    HIP_TRY(launch_rb_memo(scores, n, d_rbm, r.d_pmf_tab, ML.d_list, ML.d_listD, ML.nvec,
-                           r.d_poolP, r.d_poolD, s.stream));
+    HIP_TRY(hipStreamWaitEvent(s.stream, s.rb_join, 0));
     HIP_TRY(launch_rb_unit(0, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, nullptr, nullptr, nullptr,
                            ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, r.d_poolD, r.d_hist,
                            nullptr, r.d_work, s.stream));
     HIP_TRY(launch_rb_big(0, ps.big_grid, cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt, r.d_F,
                           nullptr, nullptr, nullptr, r.d_big, r.d_big_count, r.d_hist, nullptr,
                           s.stream));
-    if (sbar) HIP_TRY(launch_rb_score_mean(a, n * (n - 1), r.d_S, r.d_Sbar, s.stream));
     HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s.stream));
     HIP_TRY(hipEventRecord(r.ev1, s.stream));
     HIP_TRY(hipMemcpyAsync(r.h_res, r.d_res, sizeof(RbResultHeaderH) + sizeof(uint64_t) * size_t(a.B),
@@ -431,6 +439,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     r.ran = true;
     r.adj_ran = false;
     r.scores = scores;
+    r.sbar_valid = false;  // the derivative passes compute the means first
     r.R = R;
     r.B = a.B;
     r.left_pad = a.left_pad;
@@ -529,6 +538,10 @@ int enqueue_derivative(pda_ctx* ctx, Slot& s, RbState& r, int mode, double* out)
     const double* g = mode == 1 ? r.d_g : nullptr;
     HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
     HIP_TRY(hipEventRecord(r.ev_a0, s.stream));
+    if (!r.sbar_valid) {
+        HIP_TRY(launch_rb_score_mean(a, r.K, r.d_S, r.d_Sbar, ctx->num_cu, s.stream));
+        r.sbar_valid = true;
+    }
     HIP_TRY(launch_rb_unit(mode, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, r.d_S, r.d_Sbar, g,
                            ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, r.d_poolD, nullptr, out,
                            r.d_work, s.stream));

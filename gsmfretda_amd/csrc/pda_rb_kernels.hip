@@ -995,12 +995,14 @@ hipError_t launch_rb_big(int mode, uint32_t grid, int cap, const RbModel* rbm, c
 }
 
 hipError_t launch_rb_score_mean(const RunArgs& a, int K, const float* S, double* Sbar,
-                                hipStream_t stream) {
+                                int num_cu, hipStream_t stream) {
     const uint64_t nb = a.nunits / a.R;
     const uint64_t per = uint64_t(a.R) * uint64_t(K);
     const uint64_t bpb = per <= kRbSbarStage ? kRbSbarStage / per : 0;
     const uint64_t want = bpb ? (nb + bpb - 1) / bpb : (nb * uint64_t(K) + 255) / 256;
-    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(want, 65536)));
+    // a few workgroups per CU, each looping over its share
+    const uint64_t cap = uint64_t(num_cu > 0 ? num_cu : 1) * 4u;
+    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(want, cap)));
     hipLaunchKernelGGL(rb_score_mean_kernel, dim3(grid), dim3(256), 0, stream, a, K, S, Sbar);
     return hipGetLastError();
 }
