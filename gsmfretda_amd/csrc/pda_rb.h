@@ -108,31 +108,56 @@ PDA_HD int rb_pmf_index(int s, int m, int k) { return s * kRbPmfStride + m * (m 
 // units; units with larger totals compute their own.
 constexpr int kRbMemoCap = 48;
 
-// C(x, y) for 0 <= y <= kRbMaxStates and x >= 0 (0 when x < y); every
-// partial product C(x, i) is an integer, so the division is exact.
+// C(x, y) for x >= 0 (0 when x < y); y <= kRbMaxStates in closed form with
+// constant divisors (each product of y consecutive integers is divisible
+// by y!, and stays below 2^32 for the x < 64 used here).
 PDA_HD uint32_t rb_choose(int x, int y) {
-    uint32_t c = 1;
-    for (int i = 0; i < y; ++i) c = c * uint32_t(x - i) / uint32_t(i + 1);
-    return x < y ? 0u : c;
+    const uint32_t v = uint32_t(x);
+    switch (y) {
+        case 0: return 1u;
+        case 1: return v;
+        case 2: return v * (v - 1u) / 2u;
+        case 3: return v * (v - 1u) * (v - 2u) / 6u;
+        case 4: return v * (v - 1u) * (v - 2u) * (v - 3u) / 24u;
+        default: {
+            uint32_t c = 1;
+            for (int i = 0; i < y; ++i) c = c * uint32_t(x - i) / uint32_t(i + 1);
+            return x < y ? 0u : c;
+        }
+    }
 }
 
 // Lexicographic rank of the count vector (m_0 .. m_{n-1}) among all
 // n-vectors of counts >= 0 with sum <= L: for each position i, the vectors
 // that agree before i and have a smaller count at i, counted by the hockey
 // stick identity sum_{w <= W} C(w + k, k) = C(W + k + 1, k + 1).
-PDA_HD uint32_t rb_rank(const int* m, int n, int L) {
+template <int N>
+PDA_HD uint32_t rb_rank_n(const int* m, int L) {
     uint32_t r = 0;
     int left = L;
-    for (int i = 0; i < n; ++i) {
-        const int k = n - 1 - i;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const int k = N - 1 - i;
         r += rb_choose(left + k + 1, k + 1) - rb_choose(left - m[i] + k + 1, k + 1);
         left -= m[i];
     }
     return r;
 }
+PDA_HD uint32_t rb_rank(const int* m, int n, int L) {
+    switch (n) {
+        case 1: return rb_rank_n<1>(m, L);
+        case 2: return rb_rank_n<2>(m, L);
+        case 3: return rb_rank_n<3>(m, L);
+        default: return rb_rank_n<4>(m, L);
+    }
+}
 
-// Fixed-point weight of a probability (2^-32 units), rounded half up.
-PDA_HD uint64_t rb_fixed(float P) { return uint64_t(double(P) * 4294967296.0 + 0.5); }
+// Fixed-point weight of a probability (2^-32 units), rounded half up; the
+// 32-bit conversion whenever the value fits (P < 1), the same result.
+PDA_HD uint64_t rb_fixed(float P) {
+    const double x = double(P) * 4294967296.0 + 0.5;
+    return x < 4294967296.0 ? uint64_t(uint32_t(x)) : uint64_t(x);
+}
 
 // Summary of one unit's walk: per-state photon counts (16 bits each), F.
 PDA_HD uint2 rb_pack(const uint32_t* cnt) {

@@ -386,36 +386,64 @@ __global__ __launch_bounds__(kWave * kRbMemoWaves) void rb_memo_kernel(
     }
 }
 
-// Bytes per wave of rb_unit_kernel's LDS: the round's units (slot, pos,
-// offP, F, counts; derivative passes: offD; adjoint: h; Jacobian: S - Sbar).
-__host__ __device__ inline size_t rb_unit_wave_bytes(int mode) {
-    size_t b = size_t(kWave) * (4 + 4 + 4 + 4 + 8);
-    if (mode != kRbFwd) b += size_t(kWave) * 16;
-    if (mode == kRbGrad) b += size_t(kWave) * 8;
-    if (mode == kRbJac) b += size_t(kWave) * 8 * kRbMaxPairs;
-    return b;
+// Owner of position j of the wave's concatenation: the last lane l with
+// pos_l <= j (pos is nondecreasing over the lanes and pos_0 = 0; a lane with
+// len 0 shares its pos with the next lane, so for j < total the last such
+// lane has len > 0).  Binary search over the lanes with shuffles; *pown
+// gets the owner's pos.
+__device__ __forceinline__ int wave_owner(int pos, int j, int* pown) {
+    int idx = 0, pbest = 0;
+#pragma unroll
+    for (int step = kWave / 2; step > 0; step >>= 1) {
+        const int t = idx + step;
+        const int p = __shfl(pos, t);
+        if (p <= j) {
+            idx = t;
+            pbest = p;
+        }
+    }
+    *pown = pbest;
+    return idx;
+}
+
+// Per-wave LDS of rb_unit_kernel (doubles), derivative passes: S - Sbar per
+// unit and rate.
+__host__ __device__ inline int rb_unit_wave_doubles(int mode) {
+    return mode == kRbFwd ? 0 : kWave * kRbMaxPairs;
 }
 // Dynamic LDS of rb_unit_kernel: the 8-byte head (the forward's fixed-point
 // histogram, the adjoint's cotangent, the Jacobian's [B][kRbJacRow] rows),
-// then the waves' regions.
-// The head is rounded to 16 bytes: the waves' regions hold 16-byte entries.
+// rounded to 16 bytes, then the waves' regions.
 __host__ __device__ inline int rb_unit_head(int B, int mode) {
     return ((mode == kRbJac ? B * kRbJacRow : B) + 1) & ~1;
 }
 __host__ __device__ inline size_t rb_unit_lds(int B, int mode) {
-    return sizeof(double) * size_t(rb_unit_head(B, mode)) +
-           size_t(kRbUnitWaves) * rb_unit_wave_bytes(mode);
+    return sizeof(double) * (size_t(rb_unit_head(B, mode)) +
+                             size_t(kRbUnitWaves) * size_t(rb_unit_wave_doubles(mode)));
 }
+
+// One (unit, outcome) element of a window: its unit's lane, outcome, and
+// what the pass reads for it (prefetched one window ahead).
+struct RbElem {
+    bool act;
+    int own, pown, ai, Mi;
+    float F, P;
+    uint32_t cx, cy;
+    float pd[kRbMaxStates];  // P_s(ai) of the unit's counted states (derivative passes)
+};
 
 // Every unit u < a.nunits with Mt < kRbMemoCap.  A wave takes a.batch
 // consecutive units per grab and handles them 64 at a time: the units'
-// outcomes a = 0..Mt, concatenated, are spread over the lanes (wave_windows),
-// so each lane handles one (unit, outcome) pair per window whatever the
-// units' lengths; the pmf entries come from the memo.  No barrier after the
-// prologue: each wave works in its own LDS region and adds to the
-// workgroup's accumulators with atomics.
+// outcomes a = 0..Mt, concatenated, are spread over the lanes in windows of
+// 64, so each lane handles one (unit, outcome) pair per window whatever the
+// units' lengths.  A lane finds its pair's unit by a binary search over the
+// lanes' start positions and takes the unit's data from that lane by
+// shuffles; the pmf entries come from the memo, each window's loads issued
+// while the previous window is processed.  No barrier after the prologue:
+// each wave adds to the workgroup's LDS accumulators with atomics.
 template <int kMode>
-__global__ __launch_bounds__(kWave * kRbUnitWaves) void rb_unit_kernel(
+__global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_per_eu(
+    kMode == kRbFwd ? 1 : 4))) void rb_unit_kernel(
     const RbModel* __restrict__ rbm, const RunArgs a, const uint2* __restrict__ cnt_in,
     const float* __restrict__ F_in, const float* __restrict__ S_in,
     const double* __restrict__ Sbar, const double* __restrict__ g,
@@ -423,6 +451,7 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) void rb_unit_kernel(
     const float* __restrict__ poolP, const float* __restrict__ poolD,
     unsigned long long* __restrict__ hist_acc, double* __restrict__ grad,
     uint32_t* __restrict__ work) {
+    constexpr bool kDeriv = kMode != kRbFwd;
     extern __shared__ __align__(16) double s_unit[];
     __shared__ RbModel sR;
     const int tid = int(threadIdx.x), lane = tid % kWave, wave = tid / kWave;
@@ -442,14 +471,7 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) void rb_unit_kernel(
     }
     __syncthreads();
     const int n = sR.n, K = sR.K;
-    uint8_t* wr = reinterpret_cast<uint8_t*>(s_unit + head) + size_t(wave) * rb_unit_wave_bytes(kMode);
-    int* w_slot = reinterpret_cast<int*>(wr);
-    int* w_pos = w_slot + kWave;
-    uint32_t* w_offP = reinterpret_cast<uint32_t*>(w_pos + kWave);
-    float* w_F = reinterpret_cast<float*>(w_offP + kWave);
-    uint2* w_cnt = reinterpret_cast<uint2*>(w_F + kWave);
-    uint4* w_offD = reinterpret_cast<uint4*>(w_cnt + kWave);   // derivative passes
-    double* w_x = reinterpret_cast<double*>(w_offD + kWave);   // adjoint: h; Jacobian: S - Sbar
+    double* w_x = s_unit + head + wave * rb_unit_wave_doubles(kMode);
 
     const uint32_t nq = gridDim.x < kAuxQueues ? gridDim.x : kAuxQueues;
     const uint32_t queue = blockIdx.x % nq;
@@ -457,7 +479,7 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) void rb_unit_kernel(
     const uint32_t q_len = uint32_t(uint64_t(a.nunits) * (queue + 1u) / nq) - q_begin;
     uint32_t* counter = &work[kRbUnitCounter + queue * kCounterStride];
     double gp[kRbMaxStates] = {};  // adjoint: d h / d p_s over the lane's elements
-    double gk[kRbMaxPairs] = {};   // adjoint: h (S - Sbar) over the lane's units
+    double gk[kRbMaxPairs] = {};   // adjoint: G P (S - Sbar) over the lane's elements
     while (true) {
         uint32_t off = 0;
         if (lane == 0) off = atomicAdd(counter, a.batch);
@@ -465,6 +487,7 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) void rb_unit_kernel(
         if (off >= q_len) break;
         const uint32_t end = q_len - off < a.batch ? q_len : off + a.batch;
         for (uint32_t r0 = off; r0 < end; r0 += kWave) {
+            // this lane's unit of the round
             const uint32_t o = r0 + uint32_t(lane);
             const bool valid = o < end;
             const uint32_t u = q_begin + (valid ? o : r0);
@@ -479,107 +502,121 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) void rb_unit_kernel(
             // units with Mt >= kRbMemoCap: listed by the walk for rb_big_kernel
             const bool memo = valid && Mt < kRbMemoCap;
             const int len = memo ? Mt + 1 : 0;
+            uint32_t offP = 0;
+            uint4 offD = make_uint4(0u, 0u, 0u, 0u);
+            float F = 0.0f;
+            // derivative passes: the unit's score less its burst's mean, in
+            // the wave's LDS
+            double* w_sc = w_x + lane * kRbMaxPairs;
             if (memo) {
                 const uint32_t rk = rb_rank(mm, n, kRbMemoCap - 1);
-                w_offP[lane] = rank_offP[rk];
-                w_F[lane] = F_in[u];
-                w_cnt[lane] = c;
-                if (kMode != kRbFwd) w_offD[lane] = rank_offD[rk];
-                if (kMode == kRbGrad) w_x[lane] = 0.0;
-                if (kMode == kRbJac) {
-                    const uint32_t bl = u / a.R;
+                offP = rank_offP[rk];
+                F = F_in[u];
+                if (kDeriv) offD = rank_offD[rk];
+            }
+            if (kDeriv) {
+                const uint32_t bl = u / a.R;
 #pragma unroll
-                    for (int k = 0; k < kRbMaxPairs; ++k)
-                        if (k < K)
-                            w_x[lane * kRbMaxPairs + k] =
-                                double(S_in[size_t(u) * size_t(K) + size_t(k)]) -
-                                Sbar[size_t(bl) * size_t(K) + size_t(k)];
-                }
+                for (int k = 0; k < kRbMaxPairs; ++k)
+                    w_sc[k] = memo && k < K ? double(S_in[size_t(u) * size_t(K) + size_t(k)]) -
+                                                  Sbar[size_t(bl) * size_t(K) + size_t(k)]
+                                            : 0.0;
             }
             const int incl = wave_scan_sum(len, lane);
             const int pos = incl - len;
             const int total = __shfl(incl, kWave - 1);
-            w_pos[lane] = pos;
-            wave_windows(len, pos, total, lane, w_slot, [&](int j, int own) {
-                const bool act = j < total;
-                const int i = act ? own : 0;
-                const int ai = act ? j - w_pos[i] : 0;
-                const float Fi = act ? w_F[i] : 0.0f;
-                const float P = act ? poolP[w_offP[i] + uint32_t(ai)] : 0.0f;
-                const int b0 = act ? rb_bin(ai, Fi, B) : -1;
+            if (kDeriv) wave_lds_sync();
+
+            auto fetch = [&](int wb, RbElem& e) {
+                const int j = wb + lane;
+                e.act = j < total;
+                e.own = wave_owner(pos, j, &e.pown);
+                e.ai = j - e.pown;
+                e.F = __shfl(F, e.own);
+                const uint32_t op = __shfl(offP, e.own);
+                e.P = e.act ? poolP[op + uint32_t(e.ai)] : 0.0f;
+                if (kDeriv) {
+                    e.cx = __shfl(c.x, e.own);
+                    e.cy = __shfl(c.y, e.own);
+                    const uint2 ci = make_uint2(e.cx, e.cy);
+                    const uint32_t d0 = __shfl(offD.x, e.own), d1 = __shfl(offD.y, e.own);
+                    const uint32_t d2 = __shfl(offD.z, e.own), d3 = __shfl(offD.w, e.own);
+                    int Mi = 0;
+#pragma unroll
+                    for (int s = 0; s < kRbMaxStates; ++s) Mi += int(rb_count(ci, s));
+                    e.Mi = Mi;
+                    const bool inner = e.act && e.ai < Mi;
+                    int js = 0;
+#pragma unroll
+                    for (int s = 0; s < kRbMaxStates; ++s) {
+                        const bool counted = rb_count(ci, s) > 0u;
+                        const uint32_t base = js == 0 ? d0 : js == 1 ? d1 : js == 2 ? d2 : d3;
+                        e.pd[s] = inner && counted ? poolD[base + uint32_t(e.ai)] : 0.0f;
+                        js += counted ? 1 : 0;
+                    }
+                }
+            };
+            auto process = [&](const RbElem& e, int wb) {
+                const int b0 = e.act ? rb_bin(e.ai, e.F, B) : -1;
                 if (kMode == kRbFwd) {
-                    if (b0 >= 0) atomicAdd(&s_h[b0], (unsigned long long)rb_fixed(P));
+                    if (b0 >= 0) atomicAdd(&s_h[b0], (unsigned long long)rb_fixed(e.P));
                     return;
                 }
-                const uint2 ci = w_cnt[i];
-                int Mi = 0;
-#pragma unroll
-                for (int s = 0; s < kRbMaxStates; ++s) Mi += int(rb_count(ci, s));
                 // the next outcome's bin: from the next lane, which holds it
                 // unless this lane is the window's last
                 int b1 = __shfl_down(b0, 1);
-                const bool inner = act && ai < Mi;
-                if (inner && lane == kWave - 1) b1 = rb_bin(ai + 1, Fi, B);
-                const uint4 od = w_offD[i];
+                const bool inner = e.act && e.ai < e.Mi;
+                if (inner && lane == kWave - 1) b1 = rb_bin(e.ai + 1, e.F, B);
+                const uint2 ci = make_uint2(e.cx, e.cy);
                 if (kMode == kRbGrad) {
                     const double G0 = b0 >= 0 ? s_d[b0] : 0.0;
-                    double hv = G0 * double(P);
                     if (inner) {
                         const double G1 = b1 >= 0 ? s_d[b1] : 0.0;
                         const double dG = double(float(G1 - G0));
-                        int js = 0;
 #pragma unroll
-                        for (int s = 0; s < kRbMaxStates; ++s) {
-                            const int m = int(rb_count(ci, s));
-                            if (m > 0) {
-                                const uint32_t base = js == 0 ? od.x : js == 1 ? od.y : js == 2 ? od.z : od.w;
-                                gp[s] += double(m) * double(poolD[base + uint32_t(ai)]) * dG;
-                                ++js;
-                            }
-                        }
+                        for (int s = 0; s < kRbMaxStates; ++s)
+                            gp[s] += double(rb_count(ci, s)) * double(e.pd[s]) * dG;
                     }
-                    // h per unit: a sum over the unit's lanes of the window
-                    // (segments of consecutive lanes), added by its last lane
-                    const int s0 = act ? (w_pos[i] > j - lane ? w_pos[i] - (j - lane) : 0) : lane;
+                    // h (S - Sbar) of the unit, one element at a time (h is
+                    // the sum of G(bin(a)) P(a) over the unit's outcomes)
+                    if (e.act && b0 >= 0) {
+                        const double hv = G0 * double(e.P);
+                        const double* scu = w_x + e.own * kRbMaxPairs;
 #pragma unroll
-                    for (int d = 1; d < kWave; d <<= 1) {
-                        const double t = __shfl_up(hv, d);
-                        if (lane - d >= s0) hv += t;
+                        for (int k = 0; k < kRbMaxPairs; ++k)
+                            if (k < K) gk[k] += hv * scu[k];
                     }
-                    if (act && (ai == Mi || lane == kWave - 1)) w_x[i] += hv;
                 } else {
                     if (b0 >= 0) {
                         double* row = s_d + b0 * kRbJacRow + kRbMaxStates;
-                        const double* sc = w_x + i * kRbMaxPairs;
+                        const double* scu = w_x + e.own * kRbMaxPairs;
 #pragma unroll
                         for (int k = 0; k < kRbMaxPairs; ++k)
-                            if (k < K) atomicAdd(&row[k], double(P) * sc[k]);
+                            if (k < K) atomicAdd(&row[k], double(e.P) * scu[k]);
                     }
                     if (inner && b0 != b1) {
-                        int js = 0;
 #pragma unroll
                         for (int s = 0; s < kRbMaxStates; ++s) {
-                            const int m = int(rb_count(ci, s));
-                            if (m > 0) {
-                                const uint32_t base = js == 0 ? od.x : js == 1 ? od.y : js == 2 ? od.z : od.w;
-                                const double w = double(m) * double(poolD[base + uint32_t(ai)]);
+                            const uint32_t m = rb_count(ci, s);
+                            if (m > 0u) {
+                                const double w = double(m) * double(e.pd[s]);
                                 if (b1 >= 0) atomicAdd(&s_d[b1 * kRbJacRow + s], w);
                                 if (b0 >= 0) atomicAdd(&s_d[b0 * kRbJacRow + s], -w);
-                                ++js;
                             }
                         }
                     }
                 }
-            });
-            if (kMode == kRbGrad && memo) {
-                const uint32_t bl = u / a.R;
-                const double h = w_x[lane];
-#pragma unroll
-                for (int k = 0; k < kRbMaxPairs; ++k)
-                    if (k < K)
-                        gk[k] += h * (double(S_in[size_t(u) * size_t(K) + size_t(k)]) -
-                                      Sbar[size_t(bl) * size_t(K) + size_t(k)]);
+            };
+            if (total > 0) {
+                RbElem cur, nxt;
+                fetch(0, cur);
+                for (int wb = 0; wb < total; wb += kWave) {
+                    if (wb + kWave < total) fetch(wb + kWave, nxt);
+                    process(cur, wb);
+                    cur = nxt;
+                }
             }
+            if (kDeriv) wave_lds_sync();  // the round's LDS reads before the next round's writes
         }
     }
     if (kMode == kRbFwd) {
