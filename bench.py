@@ -265,26 +265,37 @@ def rb_leg(pda, synth, steps):
            "adjoint_gpu_ms": float(np.mean(adj)), "jacobian_gpu_ms": float(np.mean(jac)),
            "forward_plus_adjoint_units_per_s": units / ((np.mean(fwd) + np.mean(adj)) * 1e-3),
            "chi2": c2}
-    # rooflines of the pmf passes: the algorithm's multiply-adds per unit
-    # (from each unit's photon counts per state, frozen in the cost model)
-    # over the pass time; a float multiply-add is two lane-ops (no
-    # contraction) and two 4-byte LDS reads, a double one two lane-ops
+    # rooflines of the pmf passes, in the algorithm's work per unit (frozen
+    # in the cost model from each unit's photon counts per state): every
+    # pass bins and weights each of the unit's Mt + 1 outcomes (the
+    # microbenchmarked outcome weight); the adjoint adds a double
+    # multiply-add per leave-one-out term and per outcome and rate, the
+    # Jacobian an LDS update per outcome and rate and two per leave-one-out
+    # term (bin boundaries, at most); a multiply-add is two lane-ops (no
+    # contraction).  The pmfs themselves come from the per-count-vector
+    # memo, built beside the walk: the forward's pmf time is the forward's
+    # time after the walk.
     from gsmfretda_amd import cost_model
     cm = cost_model.load()
     w = cm["rb_work_per_unit"]
+    wo = cm["weights_lane_cycles"].get("rb_outcome")
     peak = cm["valu_peak_lane_ops_per_s"]
     lds_peak = 256 * 128 * 2.4e9  # bytes/s: 128 B per clock per CU
+    K = n * (n - 1)
+    o, dt = w["outcomes"], w["deriv_terms"]
     pmf_fwd_ms = float(np.mean(fwd)) - float(np.mean(walk))
-    for name, ms, macs, lds_bytes in (
-            ("forward_pmf", pmf_fwd_ms, w["conv_macs"], 8.0 * w["conv_macs"]),
-            ("adjoint", float(np.mean(adj)), w["deconv_macs"], 4.0 * w["deconv_macs"]),
-            ("jacobian", float(np.mean(jac)), w["deconv_macs"], 4.0 * w["deconv_macs"])):
-        t = ms * 1e-3
-        out[f"roofline_{name}"] = {
-            "bound": "valu", "achieved": 2.0 * macs * units / t / 1e12, "peak": peak / 1e12,
-            "unit": "TOP/s", "frac": 2.0 * macs * units / t / peak, "traffic": None,
-            "lds_frac": lds_bytes * units / t / lds_peak, "macs_per_unit": macs,
-            "kernel_ms": ms}
+    if wo is not None:
+        for name, ms, ops, lds_bytes in (
+                ("forward_pmf", pmf_fwd_ms, o * wo, 8.0 * o),
+                ("adjoint", float(np.mean(adj)), o * wo + 2.0 * (dt + o * K), 8.0 * o * (K + 2)),
+                ("jacobian", float(np.mean(jac)), o * wo + 2.0 * (o * K + 2.0 * dt),
+                 8.0 * (o * K + 2.0 * dt))):
+            t = ms * 1e-3
+            out[f"roofline_{name}"] = {
+                "bound": "valu", "achieved": ops * units / t / 1e12, "peak": peak / 1e12,
+                "unit": "TOP/s", "frac": ops * units / t / peak, "traffic": None,
+                "lds_frac": lds_bytes * units / t / lds_peak, "ops_per_unit": ops,
+                "outcomes_per_unit": o, "kernel_ms": ms}
     return out
 
 

@@ -584,7 +584,7 @@ int pda_last_pass_ms(pda_ctx* ctx, int sid, float* walk_ms, float* finish_ms) {
 
 int pda_debug_microbench(pda_ctx* ctx, int op, int iters, float param, float* ms,
                          int64_t* events, int64_t* inner) {
-    if (!ctx || op < 0 || op > 5 || iters < 1 || iters > (1 << 24))
+    if (!ctx || op < 0 || op > 6 || iters < 1 || iters > (1 << 24))
         return fail(PDA_E_ARG, "bad argument");
     if ((op == 4) && !(param >= 0.0f && param < float(kBinomTable)))
         return fail(PDA_E_ARG, "binomial microbenchmark: n = %g outside [0, %d)", double(param),

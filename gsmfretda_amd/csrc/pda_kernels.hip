@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include "pda_math.h"
+#include "pda_rb.h"
 #include "pda_types.h"
 #include "pda_walk.h"
 
@@ -558,8 +559,10 @@ enum MicroOp : int {
     kMbBinom = 4,    // one binomial inversion draw from the q^m table (and its
                      // iterations)
     kMbUnit = 5,     // a unit's initial state draw and its tally into the histogram
+    kMbOutcome = 6,  // one outcome of the Rao-Blackwellised spread: the bin of E = a / F
+                     // and the 2^-32 fixed-point weight of its probability
 };
-constexpr int kMicroOps = 6;
+constexpr int kMicroOps = 7;
 constexpr uint32_t kMbThreads = 256;
 
 __device__ __forceinline__ uint32_t lcg(uint32_t x) { return x * 1664525u + 1013904223u; }
@@ -692,6 +695,21 @@ __global__ __launch_bounds__(kMbThreads) void microbench_kernel(int iters, float
         }
         h ^= fbits(acc);
         events = uint64_t(iters);
+    } else if (kOp == kMbOutcome) {
+        // rb_bin and rb_fixed of pda_rb.h on fresh (a, F, P) per event; the
+        // sums carry the results out
+        uint64_t fx = 0;
+        int32_t bins = 0;
+        for (int i = 0; i < iters; ++i) {
+            h = lcg(h);
+            const int a = int(h & 31u);
+            const float F = 8.0f + float(h >> 27) + param;
+            const float P = uniform24(h ^ 0x5bd1e995u);
+            bins += rb_bin(a, F, 50);
+            fx += rb_fixed(P);
+        }
+        h ^= uint32_t(fx) ^ uint32_t(fx >> 32) ^ uint32_t(bins);
+        events = uint64_t(iters);
     }
     out[3u * gid] = h;
     out[3u * gid + 1u] = events;
@@ -800,6 +818,7 @@ hipError_t microbench_shape(int op, uint32_t* grid, uint32_t* threads) {
         case kMbPoisson: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, microbench_kernel<kMbPoisson>, kMbThreads, 0); break;
         case kMbBinom: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, microbench_kernel<kMbBinom>, kMbThreads, 0); break;
         case kMbUnit: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, microbench_kernel<kMbUnit>, kMbThreads, 0); break;
+        case kMbOutcome: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, microbench_kernel<kMbOutcome>, kMbThreads, 0); break;
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
@@ -823,6 +842,7 @@ hipError_t launch_microbench(int op, uint32_t grid, int iters, float param, cons
         case kMbPoisson: hipLaunchKernelGGL(microbench_kernel<kMbPoisson>, g, b, 0, stream, iters, param, qn, bound, out); break;
         case kMbBinom: hipLaunchKernelGGL(microbench_kernel<kMbBinom>, g, b, 0, stream, iters, param, qn, bound, out); break;
         case kMbUnit: hipLaunchKernelGGL(microbench_kernel<kMbUnit>, g, b, 0, stream, iters, param, qn, bound, out); break;
+        case kMbOutcome: hipLaunchKernelGGL(microbench_kernel<kMbOutcome>, g, b, 0, stream, iters, param, qn, bound, out); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

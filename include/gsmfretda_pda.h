@@ -322,7 +322,9 @@ int pda_debug_primitive(pda_ctx *ctx, int op, const void *in, void *out, int64_t
  * dependence: op 0 a Philox4x32-10 block, 1 a dwell window's scalar work,
  * 2 a window-end binary-search step, 3 a capped-Poisson draw of mean
  * `param` (0..64), 4 a binomial inversion draw of n = `param` (0..127)
- * trials at p = 0.3 from a q^m table, 5 a unit's start and tally.  Writes
+ * trials at p = 0.3 from a q^m table, 5 a unit's start and tally, 6 one
+ * outcome of the Rao-Blackwellised spread (its E bin and fixed-point
+ * weight; `param` shifts F).  Writes
  * the GPU milliseconds of one launch (after an untimed one), the events run
  * and the inner iterations (Poisson / inversion iterations; 0 otherwise). */
 int pda_debug_microbench(pda_ctx *ctx, int op, int iters, float param, float *ms,

@@ -23,6 +23,7 @@ This script only writes the model; run from the repository root:
 
     python tests/golden/make_cost_model.py --events            # CPU
     python tests/golden/make_cost_model.py --weights           # MI355X
+    python tests/golden/make_cost_model.py --rb-weight         # MI355X
 """
 import argparse
 import json
@@ -91,25 +92,27 @@ def events_model(procs):
 
 
 def rb_model(bursts=2000):
-    """Per-unit work of the Rao-Blackwellised kernels on the 3-state workload
+    """Per-unit work of the Rao-Blackwellised passes on the 3-state workload
     (100,000 bursts x 64; the first `bursts` bursts), from each unit's
-    photon counts per state m_s (the oracle's rb_unit):
-      forward: the pmf of the acceptor total is the chain of convolutions of
-        Binomial(m_s) over the states with m_s > 0 -- the first is a copy,
-        each later one of a pmf of length l with one of length m_s + 1 costs
-        l (m_s + 1) multiply-adds (float, two LDS reads each); then one
-        histogram update per outcome (Mt + 1 of them, Mt = sum m_s);
-      adjoint / Jacobian: per state with m_s > 0 the pmf without one trial of
-        that state (a first-order recurrence over the Mt outcomes) and its
-        contraction with the cotangent differences (Mt more), plus the
-        cotangent-weighted sum of the pmf (Mt + 1): multiply-adds in double."""
+    photon counts per state m_s (the oracle's rb_unit), Mt = sum m_s:
+      outcomes: Mt + 1 per unit -- each is binned (E = a / F) and weighted
+        (its probability in 2^-32 fixed point) by every pass;
+      deriv_terms: Mt per state with m_s > 0 -- the leave-one-trial-out
+        probabilities P_s(c), c < Mt, each a multiply-add against the
+        cotangent differences (adjoint) or a pair of updates at a bin
+        boundary (Jacobian);
+      conv_macs: the multiply-adds of building the unit's pmf by convolution
+        (the first state a copy, each later one l (m_s + 1) for a pmf of
+        length l): the work a per-unit build would do, which the memo of
+        per-count-vector pmfs replaces by one build per vector;
+      memo_share: the units whose total is below the memo's cap (48)."""
     from gsmfretda_amd import synth
     from oracle import oracle as O
     cfg = synth.config(3)
     od = O.Data.from_bursts(cfg["data"])
     p = O.make_params(cfg["n"], cfg["args"], *cfg["photophysics"])
     R, seed = cfg["R"], cfg["seeds"]["mc"]
-    macs = outcomes = dmacs = entries = 0
+    macs = outcomes = dterms = memo = 0
     units = 0
     for b in range(bursts):
         for r in range(R):
@@ -118,15 +121,15 @@ def rb_model(bursts=2000):
             mt = sum(ms)
             length = 1
             for i, m in enumerate(ms):
-                entries += m + 1
                 if i > 0:
                     macs += length * (m + 1)
                 length += m
             outcomes += mt + 1
-            dmacs += (mt + 1) + 2 * mt * len(ms)
+            dterms += mt * len(ms)
+            memo += 1 if mt < 48 else 0
             units += 1
-    return {"conv_macs": macs / units, "pmf_entries": entries / units,
-            "outcomes": outcomes / units, "deconv_macs": dmacs / units,
+    return {"outcomes": outcomes / units, "deriv_terms": dterms / units,
+            "conv_macs": macs / units, "memo_share": memo / units,
             "sample": f"bursts [0, {bursts}) of 100000 x {R}, the true 3-state set, {units} units"}
 
 
@@ -162,11 +165,21 @@ def weights_model():
     return w, raw
 
 
+def rb_weight_model():
+    """Lane-issue-cycles of one Rao-Blackwellised outcome (op 6)."""
+    from gsmfretda_amd import pda
+    with pda.MC(0, 1) as m:
+        s, _, ms = _slope(m, 6, 8000)
+    return s, dict(op=6, iters=[8000, 16000], ms_long=ms)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--events", action="store_true")
     ap.add_argument("--weights", action="store_true")
     ap.add_argument("--rb", action="store_true")
+    ap.add_argument("--rb-weight", action="store_true",
+                    help="measure only the Rao-Blackwellised outcome weight (MI355X)")
     ap.add_argument("--procs", type=int, default=8)
     ap.add_argument("--print", action="store_true", help="print instead of writing the model")
     a = ap.parse_args()
@@ -188,6 +201,10 @@ def main():
                                 "event = GPU time x 256 CU x 64 lanes x 2.4 GHz / events, "
                                 "difference of two launch lengths: python "
                                 "tests/golden/make_cost_model.py --weights")
+    if a.rb_weight:
+        s, raw = rb_weight_model()
+        cm["weights_lane_cycles"]["rb_outcome"] = s
+        cm["weights_raw"]["rb_outcome"] = raw
     if a.print:
         print(json.dumps(cm, indent=2))
     else:

@@ -63,3 +63,26 @@ def test_ops_per_unit():
         b = cost_model.breakdown(w, cm)
         assert cost_model.ops_per_unit(w, cm) == pytest.approx(sum(b.values()))
         assert all(v > 0 for v in b.values())
+
+
+def test_frozen_rb_work_matches_the_oracle(oracle):
+    """The Rao-Blackwellised work counts (outcomes Mt + 1 and leave-one-out
+    terms Mt per counted state, per unit) recounted on other bursts of the
+    same workload agree with the frozen ones within the sampling spread, and
+    almost every unit's pmf comes from the memo (Mt < 48)."""
+    cfg = synth.config(3, n_bursts=60_000)
+    od = oracle.Data.from_bursts(cfg["data"])
+    p = oracle.make_params(cfg["n"], cfg["args"], *cfg["photophysics"])
+    o = dt = units = 0
+    for b in range(50_000, 50_150):
+        for r in range(cfg["R"]):
+            cnt, _, _ = oracle.rb_unit(od, p, b, r, cfg["seeds"]["mc"], 0)
+            mt = int(np.sum(cnt))
+            o += mt + 1
+            dt += mt * int(np.count_nonzero(cnt))
+            units += 1
+    w = cost_model.load()["rb_work_per_unit"]
+    assert abs(o / units - w["outcomes"]) <= 0.05 * w["outcomes"], (o / units, w["outcomes"])
+    assert abs(dt / units - w["deriv_terms"]) <= 0.08 * w["deriv_terms"], (dt / units, w)
+    assert w["memo_share"] > 0.99
+    assert cost_model.load()["weights_lane_cycles"]["rb_outcome"] > 0

@@ -10,9 +10,10 @@ autograd Function, through the C ABI.
   1e-3 relative; the efficiency gradient equals central finite differences
   of the GPU forward (h = 1e-4 relative) within 1e-3; the rate gradient
   agrees with central differences over independent seeds.
-* Jacobian: g @ J equals the adjoint for random cotangents (both lane
-  layouts), and its rows equal the oracle's FP64 gradient for indicator
-  cotangents within 1e-3.
+* Jacobian: g @ J equals the adjoint for random cotangents (units whose
+  pmf comes from the per-count-vector memo, and units with acceptor totals
+  of 48 or more, built one wave per unit), and its rows equal the oracle's
+  FP64 gradient for indicator cotangents within 1e-3.
 * torch: the autograd Function's chi2 and gradient match the library's.
 """
 import numpy as np
@@ -78,10 +79,8 @@ def test_rb_forward_bit_exact(gpu_mc, oracle, n, bg, rng_):
 
 @pytest.mark.parametrize("base,geom", [(200, 60), (400, 150)])
 def test_rb_forward_bit_exact_long_bursts(gpu_mc, oracle, base, geom):
-    """Bursts with acceptor totals of 256 and more take the whole-wave layout:
-    mixed with shorter ones (the in-place forward, both layouts in one
-    launch) and all long (the three-buffer forward, chosen when the long
-    bursts set the LDS)."""
+    """Long bursts: units with acceptor totals of 48 and more (listed by the
+    walk, one wave per unit) mixed with memo units, and nearly all long."""
     d = synth.generate(120, synth.make_args([0.2, 0.55, 0.85], [900, 2500, 700, 3000, 1500, 5000]),
                        3, base_photons=base, mean_geom=geom, seed=61)
     _load(gpu_mc, d)
@@ -94,8 +93,8 @@ def test_rb_forward_bit_exact_long_bursts(gpu_mc, oracle, base, geom):
 
 def test_rb_forward_full_workload_bit_exact(gpu_mc, oracle):
     """The Rao-Blackwellised workload at full size (3 states, 100,000 bursts
-    x 64 resamples, with scores and the pmf store): the fixed-point
-    histogram equals the oracle's."""
+    x 64 resamples, with scores): the fixed-point histogram equals the
+    oracle's."""
     cfg = synth.config(3)
     d = cfg["data"]
     gpu_mc.load(d, SgDivSr=np.zeros(d.nbursts, np.float32), nbins=50, left_pad=2)
@@ -191,7 +190,7 @@ def test_rb_at_200_bins(gpu_mc, oracle):
 
 @pytest.mark.parametrize("long_bursts", [False, True])
 def test_rb_jacobian_is_the_adjoint_of_every_cotangent(gpu_mc, oracle, long_bursts):
-    # long bursts (acceptor totals >= 256) take the one-unit-per-wave layout
+    # long bursts: many units with acceptor totals >= 48 (one wave per unit)
     if long_bursts:
         d = synth.generate(60, synth.make_args([0.2, 0.55, 0.85], [900, 2500, 700, 3000, 1500, 5000]),
                            3, base_photons=400, mean_geom=150, seed=57)
