@@ -19,6 +19,10 @@ fitters run under the same wall-clock limit:
                resamples per burst (timed)
   lm-rb        Levenberg-Marquardt on the same chi2 with the Jacobian of the
                expected histogram (pda_rb_jacobian), same start
+  lm-rb-batched, adam-rb-batched
+               the same with four chains in lockstep through the RB
+               population passes (pda_rb_forward_population and its adjoint
+               and Jacobian), started from the four best initial sets
 
 Each is timed to reach chi2 <= 1.01 chi2(truth), and to reach within 1 % of
 the chi2 the per-set genetic algorithm ends with.
@@ -77,7 +81,9 @@ def main(argv=None):
                     help="resamples per burst of the batched evaluation that ranks the starting "
                          "sets for the gradient fitters")
     ap.add_argument("--fitters", default="ga-per-set,ga-batched,adam-rb,lm-rb",
-                    help="comma-separated subset of ga-per-set, ga-batched, adam-rb, lm-rb")
+                    help="comma-separated subset of ga-per-set, ga-batched, adam-rb, lm-rb, "
+                         "lm-rb-batched, adam-rb-batched")
+    ap.add_argument("--chains", type=int, default=4, help="chains of the batched fitters")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
 
@@ -121,13 +127,17 @@ def main(argv=None):
     grad_specs = [s for s in [
         ("adam-rb", lambda: fit.AdamFitter(lr=a.lr, R_grad=a.r_grad, check_every=5,
                                            plateau=a.plateau, lr_decay=a.lr_decay)),
-        ("lm-rb", lambda: fit.LMFitter(R_grad=a.r_grad))] if s[0] in fitters]
+        ("lm-rb", lambda: fit.LMFitter(R_grad=a.r_grad)),
+        ("lm-rb-batched", lambda: fit.BatchedLMFitter(chains=a.chains, R_grad=a.r_grad)),
+        ("adam-rb-batched", lambda: fit.BatchedAdamFitter(chains=a.chains, lr=a.lr,
+                                                          R_grad=a.r_grad, check_every=5))]
+        if s[0] in fitters]
     ranking_seconds = {}
     for name, make in grad_specs:
         prob.evaluations = 0
         h, t_init = fit.fit_from_population(make(), prob, pop0, a.seconds, target=target,
                                             rank_resamples=a.rank_resamples,
-                                            restarts=name == "lm-rb")
+                                            restarts=name != "adam-rb")
         results[name] = h
         ranking_seconds[name] = t_init
         log(f"{name}: best chi2 {h.chi2[-1]:.5f} after {h.times[-1]:.1f} s "
