@@ -177,6 +177,7 @@ int upload_model(Slot& s) {
     s.any_v = M.any_v != 0;
     HIP_TRY(hipMemcpyAsync(s.d_blob, s.h_blob, sizeof(ModelBlob), hipMemcpyHostToDevice, s.stream));
     HIP_TRY(hipEventRecord(s.model_copied, s.stream));
+    if (!s.any_v) HIP_TRY(launch_guide(s.d_blob, M.n, s.d_thr, s.d_guide, s.stream));
     s.model_pending = true;
     return PDA_OK;
 }
@@ -214,6 +215,8 @@ int pda_create(int device, int nstreams, pda_ctx** out) {
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
             hipHostMalloc(reinterpret_cast<void**>(&s.h_blob), sizeof(ModelBlob), hipHostMallocDefault) != hipSuccess ||
             hipMalloc(&s.d_blob, sizeof(ModelBlob)) != hipSuccess ||
+            hipMalloc(&s.d_thr, sizeof(ThrPair) * kGuideThrPairs) != hipSuccess ||
+            hipMalloc(&s.d_guide, kGuideBytes) != hipSuccess ||
             hipMalloc(&s.d_replay_count, sizeof(uint32_t)) != hipSuccess ||
             hipMemset(s.d_replay_count, 0, sizeof(uint32_t)) != hipSuccess ||
             hipMalloc(&s.d_work, sizeof(uint32_t) * kWorkWords) != hipSuccess ||
@@ -246,6 +249,8 @@ int pda_destroy(pda_ctx* ctx) {
     for (auto& s : ctx->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         (void)hipFree(s.d_blob);
+        (void)hipFree(s.d_thr);
+        (void)hipFree(s.d_guide);
         (void)hipHostFree(s.h_blob);
         (void)hipFree(s.d_replay);
         (void)hipFree(s.d_replay_count);
@@ -461,7 +466,8 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
     }
     a.replay_cap = uint32_t(s->replay_cap);
     HIP_TRY(hipEventRecord(s->ev0, s->stream));
-    HIP_TRY(launch_walk(pass, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, a,
+    const BinomGuide guide{s->d_thr, s->d_guide};
+    HIP_TRY(launch_walk(pass, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, guide, a,
                         s->d_hist_acc, s->d_replay, s->d_replay_count, s->d_work, outE, outAcc,
                         outF, s->stream));
     HIP_TRY(hipEventRecord(s->ev_mid, s->stream));
@@ -475,7 +481,8 @@ int pda_run(pda_ctx* ctx, int sid, int R, uint32_t seed, uint32_t set_id, int fl
         const uint32_t rgrid =
             uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(want, uint64_t(ctx->num_cu) * per_cu)));
         HIP_TRY(launch_walk(kPassReplay, rgrid, ctx->d_bursts, ctx->d_ticks, ctx->d_links,
-                            s->d_blob, a, s->d_hist_acc, s->d_replay, s->d_replay_count, s->d_work,
+                            s->d_blob, guide, a, s->d_hist_acc, s->d_replay, s->d_replay_count,
+                            s->d_work,
                             outE, outAcc, outF, s->stream));
     }
     HIP_TRY(launch_finish(a, ctx->d_obs, s->d_hist_acc, s->d_replay_count, s->d_work, s->d_result,

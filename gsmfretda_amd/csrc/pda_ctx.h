@@ -137,6 +137,10 @@ struct Slot {
     hipStream_t stream = nullptr;
     ModelBlob* h_blob = nullptr;  // pinned staging of the model tables
     ModelBlob* d_blob = nullptr;
+    // threshold rows of the fast walk's binomial draws for the uploaded set
+    // (guide_kernel, enqueued with every model upload)
+    ThrPair* d_thr = nullptr;     // kGuideThrPairs
+    uint8_t* d_guide = nullptr;   // kGuideBytes
     bool any_v = false;           // broadening: the general pass walks every unit
     uint32_t* d_replay = nullptr;  // units the fast pass abandoned
     uint32_t* d_replay_count = nullptr;
@@ -234,10 +238,12 @@ Launch plan_launch(const pda_ctx* ctx, int pass, bool population, uint64_t units
 hipError_t walk_occupancy(int pass, int* blocks_per_cu);
 hipError_t pop_walk_occupancy(int pass, int B, int* blocks_per_cu);
 hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
-                       const uint2* links, const ModelBlob* blob, const RunArgs& a,
-                       int32_t* hist_acc, uint32_t* replay, uint32_t* replay_count,
-                       uint32_t* work, float* outE, int32_t* outAcc, float* outF,
-                       hipStream_t stream);
+                       const uint2* links, const ModelBlob* blob, const BinomGuide& guide,
+                       const RunArgs& a, int32_t* hist_acc, uint32_t* replay,
+                       uint32_t* replay_count, uint32_t* work, float* outE, int32_t* outAcc,
+                       float* outF, hipStream_t stream);
+hipError_t launch_guide(const ModelBlob* blob, int nstates, ThrPair* thr, uint8_t* guide,
+                        hipStream_t stream);
 hipError_t launch_finish(const RunArgs& a, const float* obs, int32_t* hist_acc,
                          uint32_t* replay_count, uint32_t* work, ResultHeader* result,
                          hipStream_t stream);

@@ -72,8 +72,8 @@ template <int kPass>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     kPass == kPassFast ? kFastWavesPerEU : 1))) void walk_kernel(
     const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
-    const uint2* __restrict__ links, const ModelBlob* __restrict__ blob, const RunArgs a,
-    int32_t* __restrict__ hist_acc, uint32_t* __restrict__ replay,
+    const uint2* __restrict__ links, const ModelBlob* __restrict__ blob, const BinomGuide guide,
+    const RunArgs a, int32_t* __restrict__ hist_acc, uint32_t* __restrict__ replay,
     uint32_t* __restrict__ replay_count, uint32_t* __restrict__ work, float* __restrict__ outE,
     int32_t* __restrict__ outAcc, float* __restrict__ outF) {
     constexpr bool kFastForm = kPass == kPassFast;
@@ -118,7 +118,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     // whole batch, so the batch sets the grid's tail.  Photons are read
     // through L1/L2.  A replay pass takes its units the same way from the
     // replay list.
-    const BinomTable* tab = &blob->tab;
     const uint32_t queue = blockIdx.x % queues;
     const uint32_t q_begin = uint32_t(uint64_t(nunits) * queue / queues);
     const uint32_t q_end = uint32_t(uint64_t(nunits) * (queue + 1u) / queues);
@@ -167,8 +166,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
         }
         if (!__any(busy)) break;
         if (busy) {
-            const int r = step_window<kFastForm>(sM, tab, key, d.P, d.stop_rel, d.T,
-                                                 ticks + d.off, links + d.off, a, st);
+            const int r = step_window<kFastForm, false, BinomGuide>(
+                sM, &guide, key, d.P, d.stop_rel, d.T, ticks + d.off, links + d.off, a, st);
             if (r != kStepMore) {
                 if (r == kStepReplay) {
                     const uint32_t slot = atomicAdd(replay_count, 1u);
@@ -186,6 +185,21 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
         const int32_t v = s_hist[i];
         if (v) atomicAdd(&row[i], v);
     }
+}
+
+// Threshold rows of the fast walk's binomial draws (build_guide_row), one
+// lane per (state, photon count) row of the slot's parameter set; runs on
+// the slot's stream after the model upload, so every later walk of that set
+// reads finished rows.
+__global__ __launch_bounds__(kWave) void guide_kernel(const ModelBlob* __restrict__ blob,
+                                                      ThrPair* __restrict__ thr,
+                                                      uint8_t* __restrict__ guide) {
+    const uint32_t row = blockIdx.x * kWave + threadIdx.x;
+    const int s = int(row / kBinomTable);
+    const int m = int(row % kBinomTable);
+    if (s >= blob->m.n) return;
+    build_guide_row(blob->m, blob->tab, s, m, thr + size_t(row) * kGuideRow,
+                    guide + size_t(row) * kGuideBuckets);
 }
 
 constexpr uint32_t kFinishWaves = 16;
@@ -730,22 +744,30 @@ hipError_t walk_occupancy(int pass, int* blocks_per_cu) {
 }
 
 hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
-                       const uint2* links, const ModelBlob* blob, const RunArgs& a,
-                       int32_t* hist_acc, uint32_t* replay, uint32_t* replay_count,
-                       uint32_t* work, float* outE, int32_t* outAcc, float* outF,
-                       hipStream_t stream) {
+                       const uint2* links, const ModelBlob* blob, const BinomGuide& guide,
+                       const RunArgs& a, int32_t* hist_acc, uint32_t* replay,
+                       uint32_t* replay_count, uint32_t* work, float* outE, int32_t* outAcc,
+                       float* outF, hipStream_t stream) {
     if (pass == kPassFast)
         hipLaunchKernelGGL(walk_kernel<kPassFast>, dim3(grid), dim3(kWave), 0, stream, bursts,
-                           ticks, links, blob, a, hist_acc, replay, replay_count, work, outE,
+                           ticks, links, blob, guide, a, hist_acc, replay, replay_count, work, outE,
                            outAcc, outF);
     else if (pass == kPassReplay)
         hipLaunchKernelGGL(walk_kernel<kPassReplay>, dim3(grid), dim3(kWave), 0, stream, bursts,
-                           ticks, links, blob, a, hist_acc, replay, replay_count, work, outE,
+                           ticks, links, blob, guide, a, hist_acc, replay, replay_count, work, outE,
                            outAcc, outF);
     else
         hipLaunchKernelGGL(walk_kernel<kPassGeneral>, dim3(grid), dim3(kWave), 0, stream, bursts,
-                           ticks, links, blob, a, hist_acc, replay, replay_count, work, outE,
+                           ticks, links, blob, guide, a, hist_acc, replay, replay_count, work, outE,
                            outAcc, outF);
+    return hipGetLastError();
+}
+
+hipError_t launch_guide(const ModelBlob* blob, int nstates, ThrPair* thr, uint8_t* guide,
+                        hipStream_t stream) {
+    const uint32_t rows = uint32_t(nstates) * kBinomTable;
+    hipLaunchKernelGGL(guide_kernel, dim3((rows + kWave - 1) / kWave), dim3(kWave), 0, stream,
+                       blob, thr, guide);
     return hipGetLastError();
 }
 

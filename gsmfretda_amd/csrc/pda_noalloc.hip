@@ -139,8 +139,9 @@ extern "C" int pda_run_noalloc(pda_ctx* ctx, int sid, int R, uint32_t seed, uint
     hipLaunchKernelGGL(clear_scratch_kernel, dim3(1), dim3(1024), 0, s->stream, work, count,
                        hist_acc);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(launch_walk(pass, L.grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, a,
-                        hist_acc, replay, count, work, nullptr, nullptr, nullptr, s->stream));
+    const BinomGuide guide{s->d_thr, s->d_guide};
+    HIP_TRY(launch_walk(pass, L.grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, s->d_blob, guide,
+                        a, hist_acc, replay, count, work, nullptr, nullptr, nullptr, s->stream));
     if (pass == kPassFast && nunits > 0) {
         int per_cu = 0;
         if (walk_occupancy(kPassReplay, &per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
@@ -148,8 +149,8 @@ extern "C" int pda_run_noalloc(pda_ctx* ctx, int sid, int R, uint32_t seed, uint
         const uint32_t rgrid = uint32_t(
             std::max<uint64_t>(1, std::min<uint64_t>(want, uint64_t(ctx->num_cu) * per_cu)));
         HIP_TRY(launch_walk(kPassReplay, rgrid, ctx->d_bursts, ctx->d_ticks, ctx->d_links,
-                            s->d_blob, a, hist_acc, replay, count, work, nullptr, nullptr, nullptr,
-                            s->stream));
+                            s->d_blob, guide, a, hist_acc, replay, count, work, nullptr, nullptr,
+                            nullptr, s->stream));
     }
     HIP_TRY(launch_finish(a, ctx->d_obs, hist_acc, count, work,
                           static_cast<ResultHeader*>(d_result), s->stream));

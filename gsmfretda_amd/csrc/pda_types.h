@@ -71,6 +71,25 @@ struct BinomTable {
     int32_t bound[kMaxStates][kBinomTable];
 };
 
+// Thresholds of the fast walk's binomial inversion on the 53-bit uniform
+// grid (binomial_guided, build_guide_row in pda_walk.h): row (s, m) holds,
+// for X = 0 .. its last entry, a pair (up, lo) of grid indices -- u >= up
+// proves that the inversion goes past X, u <= lo that it stops at or before
+// X -- followed by a sentinel, and a guide of kGuideBuckets entries: the
+// first X that a uniform in each 1/64 of [0, 1) can end at.
+constexpr int kGuideBuckets = 64;
+constexpr int kGuideShift = 47;            // 53-bit grid index >> 47 = bucket
+constexpr int kGuideRow = kBinomTable + 1;  // pairs per row, sentinel included
+struct ThrPair {
+    int64_t up, lo;
+};
+struct BinomGuide {
+    const ThrPair* thr;    // [kMaxStates][kBinomTable][kGuideRow]
+    const uint8_t* guide;  // [kMaxStates][kBinomTable][kGuideBuckets]
+};
+constexpr size_t kGuideThrPairs = size_t(kMaxStates) * kBinomTable * kGuideRow;
+constexpr size_t kGuideBytes = size_t(kMaxStates) * kBinomTable * kGuideBuckets;
+
 // What a stream slot uploads per parameter set.
 struct ModelBlob {
     Model m;

@@ -67,6 +67,94 @@ PDA_HD int64_t binomial_table(int n, int s, const Model& M, const BinomTable& ta
     return M.flip[s] ? int64_t(n - X) : int64_t(X);
 }
 
+static_assert(kInversionMaxN >= kBinomTable, "every tabulated count draws by inversion");
+
+constexpr int64_t kU53Max = (int64_t(1) << 53) - 1;  // largest 53-bit grid index
+
+// Row (s, m) of the guided inversion: binomial_table's X as a function of
+// the uniform's grid index u (U = u 2^-53 exactly, uniform53).
+//
+// The loop keeps U_0 = U, U_{i+1} = fl(U_i - px_i) and goes past i while
+// U_i > px_i, where px_i does not depend on U.  Rounding is monotone, so
+// every U_i is a non-decreasing function of U, and so is X(U): X >= k + 1
+// exactly for U at or above some grid point.  Locating it: every subtraction
+// rounds a value in (0, 1), so U_i is within i 2^-54 of U - S_{i-1}, with
+// S_k = px_0 + ... + px_k exact; hence U > S_k + k 2^-54 gives X >= k + 1
+// and U <= S_k - k 2^-54 gives X <= k.  The running sum
+// C_k = fl(C_{k-1} + px_k) of the loop's own px_k (same operations) stays
+// within k 2^-53 of S_k (its values are below 2), so with d_k = 2k + 2:
+//   up_k = floor(C_k 2^53) + d_k + 1:  u >= up_k  =>  U > C_k + d_k 2^-53  =>  X >= k + 1
+//   lo_k = ceil(C_k 2^53) - d_k - 1:   u <= lo_k  =>  U < C_k - d_k 2^-53  =>  X <= k
+// The row ends at the bound (then a sentinel makes X = bound + 1 a replay,
+// as the loop's "X > bound") or at the first k whose lo_k covers the whole
+// grid (X <= k for every uniform).  guide[j] is the number of leading
+// entries with up <= j 2^47, the least X of any u in bucket j.
+PDA_HD int build_guide_row(const Model& M, const BinomTable& tab, int s, int m, ThrPair* thr,
+                           uint8_t* guide) {
+    const double pp = M.pp[s];
+    const double q = M.qq[s];
+    const int bound = tab.bound[s][m];
+    double px = tab.qn[s][m];
+    double C = px;
+    int k = 0;
+    bool covered = false;
+    for (;; ++k) {
+        const double cs = C * 0x1p53;  // exact scaling, below 2^54
+        const int64_t d = 2 * int64_t(k) + 2;
+        const int64_t lo = int64_t(ceil(cs)) - d - 1;
+        thr[k] = ThrPair{int64_t(floor(cs)) + d + 1, lo};
+        if (lo >= kU53Max) {
+            covered = true;
+            break;
+        }
+        if (k >= bound) break;
+        px = (double(m - k) * pp * px) / (double(k + 1) * q);
+        C = C + px;
+    }
+    thr[k + 1] = ThrPair{INT64_MAX, covered ? INT64_MAX : int64_t(-1)};
+    int x = 0;
+    for (int j = 0; j < kGuideBuckets; ++j) {
+        const int64_t u0 = int64_t(j) << kGuideShift;
+        while (x <= k && thr[x].up <= u0) ++x;
+        guide[j] = uint8_t(x);
+    }
+    return k + 1;
+}
+
+// binomial_table by the thresholds: the same X for every uniform, or -1
+// where binomial_table returns -1 and where u falls between lo_X and up_X
+// (about (4X + 6) grid points per row: the general form then runs the loop).
+// Two dependent loads instead of X + 1 loop passes whose count differs from
+// lane to lane.
+PDA_HD int64_t binomial_guided(int n, int s, const Model& M, const BinomGuide& G, uint32_t hi,
+                               uint32_t lo) {
+    if (n <= 0) return 0;
+    if (n >= kBinomTable) return -1;
+    const int64_t u = (int64_t(hi >> 5) << 26) | int64_t(lo >> 6);
+    const size_t row = size_t(s) * kBinomTable + size_t(n);
+    int X = G.guide[row * kGuideBuckets + size_t(u >> kGuideShift)];
+    const ThrPair* t = G.thr + row * kGuideRow;
+    ThrPair e = t[X];
+    while (u >= e.up) e = t[++X];
+    if (u > e.lo) return -1;
+    return M.flip[s] ? int64_t(n - X) : int64_t(X);
+}
+
+PDA_HD int64_t binomial_fast(int n, int s, const Model& M, const BinomTable& tab, uint32_t hi,
+                             uint32_t lo) {
+    return binomial_table(n, s, M, tab, hi, lo);
+}
+
+PDA_HD int64_t binomial_fast(int n, int s, const Model& M, const BinomGuide& g, uint32_t hi,
+                             uint32_t lo) {
+    return binomial_guided(n, s, M, g, hi, lo);
+}
+
+template <class T>
+struct NoDeduce {
+    using type = T;
+};
+
 // background_count() for means <= 64 (one inversion chunk); -1 otherwise
 // (also for caps of 2^30 photons or more, which a 32-bit counter could pass).
 PDA_HD float background_single(float lam, float cap, float u_first, float u_cap) {
@@ -140,8 +228,11 @@ PDA_HD int begin_unit(const Model& M, const UnitKey& key, uint32_t P, float T,
 // tick/link point at the burst's photons in LDS or in global memory.
 // kCounts (general form only): no acceptor draw; the window is described in
 // *wi instead, for the Rao-Blackwellised form that sums the binomial pmf.
-template <bool kFast, bool kCounts = false>
-PDA_HD int step_window(const Model& M, const BinomTable* tab, const UnitKey& key, uint32_t P,
+// Tab: BinomTable (inversion loop) or BinomGuide (thresholds) for the fast
+// form's binomial draw; unused by the general form.
+template <bool kFast, bool kCounts = false, class Tab = BinomTable>
+PDA_HD int step_window(const Model& M, const typename NoDeduce<Tab>::type* tab, const UnitKey& key,
+                       uint32_t P,
                        uint32_t stop_rel, float T, const uint32_t* tick, const uint2* link,
                        const RunArgs& a, WalkState<kFast>& st, WindowInfo* wi = nullptr) {
     static_assert(!(kFast && kCounts), "the counts form walks in the general form");
@@ -231,7 +322,7 @@ PDA_HD int step_window(const Model& M, const BinomTable* tab, const UnitKey& key
         wi->tau = tau;
         wi->m = int32_t(m);
     } else if (kFast) {
-        x = binomial_table(int(m), s, M, *tab, A.w[2], A.w[3]);
+        x = binomial_fast(int(m), s, M, *tab, A.w[2], A.w[3]);
         if (x < 0) return kStepReplay;
     } else {
         // broadened efficiency (drawE, src/gen_rand.cuh:63-68)
@@ -258,13 +349,14 @@ PDA_HD int step_window(const Model& M, const BinomTable* tab, const UnitKey& key
 }
 
 // One (burst, resample) unit: the body of mc_kernel (src/mc.cu:92-196).
-template <bool kFast>
-PDA_HD WalkOut walk_unit(const Model& M, const BinomTable* tab, const UnitKey& key, uint32_t P,
-                         uint32_t stop_rel, float T, const uint32_t* tick, const uint2* link,
-                         const RunArgs& a) {
+template <bool kFast, class Tab = BinomTable>
+PDA_HD WalkOut walk_unit(const Model& M, const typename NoDeduce<Tab>::type* tab, const UnitKey& key,
+                         uint32_t P, uint32_t stop_rel, float T, const uint32_t* tick,
+                         const uint2* link, const RunArgs& a) {
     WalkState<kFast> st;
     int r = begin_unit<kFast>(M, key, P, T, st);
-    while (r == kStepMore) r = step_window<kFast>(M, tab, key, P, stop_rel, T, tick, link, a, st);
+    while (r == kStepMore)
+        r = step_window<kFast, false, Tab>(M, tab, key, P, stop_rel, T, tick, link, a, st);
     return WalkOut{st.acc, st.F, r == kStepReplay};
 }
 

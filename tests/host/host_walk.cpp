@@ -5,6 +5,7 @@
 // Test infrastructure; the product never runs this build.
 #include <cstdint>
 #include <cstring>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -15,8 +16,114 @@ using namespace pda;
 
 static std::string g_err;
 static int64_t replays = 0;
+static int guided = 1;  // fast form draws binomials by the threshold rows
 
 extern "C" int64_t hw_replays() { return replays; }
+
+extern "C" void hw_set_guided(int on) { guided = on; }
+
+// Threshold rows of every (state, count) row of a model, as the guide kernel
+// builds them on the device.
+static void build_guide(const Model& M, const BinomTable& tab, std::vector<ThrPair>& thr,
+                        std::vector<uint8_t>& guide) {
+    thr.assign(kGuideThrPairs, ThrPair{0, 0});
+    guide.assign(kGuideBytes, 0);
+    for (int s = 0; s < M.n; ++s)
+        for (int m = 0; m < kBinomTable; ++m) {
+            const size_t row = size_t(s) * kBinomTable + size_t(m);
+            build_guide_row(M, tab, s, m, thr.data() + row * kGuideRow,
+                            guide.data() + row * kGuideBuckets);
+        }
+}
+
+static uint64_t splitmix(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// The guided draw against the inversion loop of binomial_table on one
+// model: for every state and count 1..127, at every threshold pair +- 3 grid
+// steps and at n_rand random grid indices.  out: [0] draws compared, [1]
+// draws where both give an X and they differ, [2] guided -1 where the loop
+// gives an X (inside a band), [3] loop -1 where the guided draw gives an X,
+// [4] thresholds the loop's exact crossing lies outside (lo_k, up_k] of,
+// [5] largest row length, [6] rows whose guide is not monotone, [7] random
+// draws that fell inside a band.
+extern "C" int hw_guide_check(int n, const float* args, const float* photophysics, int n_rand,
+                              uint64_t seed, int64_t* out) {
+    Photophysics ph;
+    ph.gamma = photophysics[0];
+    ph.dexdir = photophysics[1];
+    ph.crosstalk = photophysics[2];
+    ph.r0 = photophysics[3];
+    Model M;
+    int rc = build_model(ph, n, args, n * (n + 1), M, g_err);
+    if (rc) return rc;
+    BinomTable tab;
+    build_binom_table(M, tab);
+    std::vector<ThrPair> thr;
+    std::vector<uint8_t> guide;
+    build_guide(M, tab, thr, guide);
+    const BinomGuide G{thr.data(), guide.data()};
+    for (int i = 0; i < 8; ++i) out[i] = 0;
+    auto draw_pair = [&](int s, int m, int64_t u) {
+        const uint32_t hi = uint32_t(u >> 26) << 5;
+        const uint32_t lo = uint32_t(u & ((int64_t(1) << 26) - 1)) << 6;
+        const int64_t a = binomial_table(m, s, M, tab, hi, lo);
+        const int64_t b = binomial_guided(m, s, M, G, hi, lo);
+        return std::make_pair(a, b);
+    };
+    auto check = [&](int s, int m, int64_t u, bool rnd) {
+        if (u < 0 || u > kU53Max) return;
+        const auto ab = draw_pair(s, m, u);
+        ++out[0];
+        if (ab.first >= 0 && ab.second >= 0 && ab.first != ab.second) ++out[1];
+        if (ab.first >= 0 && ab.second < 0) ++out[rnd ? 7 : 2];
+        if (ab.first < 0 && ab.second >= 0) ++out[3];
+    };
+    // the loop's X as a function of the grid index, before the flip
+    auto loop_x = [&](int s, int m, int64_t u) {
+        const int64_t x = draw_pair(s, m, u).first;
+        return x < 0 ? int64_t(kBinomTable + 1) : (M.flip[s] ? int64_t(m) - x : x);
+    };
+    uint64_t st = seed;
+    for (int s = 0; s < M.n; ++s)
+        for (int m = 1; m < kBinomTable; ++m) {
+            const size_t row = size_t(s) * kBinomTable + size_t(m);
+            const ThrPair* t = thr.data() + row * kGuideRow;
+            const uint8_t* g = guide.data() + row * kGuideBuckets;
+            int len = 0;
+            while (t[len].up != INT64_MAX) ++len;
+            if (len > out[5]) out[5] = len;
+            for (int j = 1; j < kGuideBuckets; ++j)
+                if (g[j] < g[j - 1]) {
+                    ++out[6];
+                    break;
+                }
+            for (int k = 0; k < len; ++k) {
+                for (int d = -3; d <= 3; ++d) {
+                    check(s, m, t[k].up + d, false);
+                    check(s, m, t[k].lo + d, false);
+                }
+                // exact crossing of X >= k + 1 by bisection on the grid
+                // (X is non-decreasing in u): it must lie in (lo_k, up_k]
+                if (t[k].up <= kU53Max && loop_x(s, m, kU53Max) >= k + 1) {
+                    int64_t a = 0, b = kU53Max;  // loop_x(b) >= k + 1
+                    if (loop_x(s, m, 0) >= k + 1) b = 0;
+                    while (b - a > 1) {
+                        const int64_t mid = a + (b - a) / 2;
+                        if (loop_x(s, m, mid) >= k + 1) b = mid;
+                        else a = mid;
+                    }
+                    if (!(b > t[k].lo && b <= t[k].up)) ++out[4];
+                }
+            }
+            for (int i = 0; i < n_rand; ++i) check(s, m, int64_t(splitmix(st) >> 11), true);
+        }
+    return 0;
+}
 
 extern "C" const char* hw_error() { return g_err.c_str(); }
 
@@ -50,6 +157,10 @@ extern "C" int hw_run(const int64_t* start, const int64_t* stop, const int64_t* 
     if (rc) return rc;
     BinomTable tab;
     build_binom_table(M, tab);
+    std::vector<ThrPair> thr;
+    std::vector<uint8_t> guide;
+    build_guide(M, tab, thr, guide);
+    const BinomGuide G{thr.data(), guide.data()};
     RunArgs a{};
     a.R = uint32_t(R);
     a.seed = seed;
@@ -64,7 +175,10 @@ extern "C" int hw_run(const int64_t* start, const int64_t* stop, const int64_t* 
             const UnitKey key{seed, set, uint32_t(r), uint32_t(b)};
             // fast form unless the model broadens; abandoned units replay in full
             WalkOut o{0.0f, 0.0f, true};
-            if (!M.any_v)
+            if (!M.any_v && guided)
+                o = walk_unit<true, BinomGuide>(M, &G, key, d.P, d.stop_rel, d.T,
+                                                ticks.data() + d.off, links.data() + d.off, a);
+            else if (!M.any_v)
                 o = walk_unit<true>(M, &tab, key, d.P, d.stop_rel, d.T, ticks.data() + d.off,
                                     links.data() + d.off, a);
             if (o.replay) {
