@@ -648,6 +648,21 @@ int pda_debug_microbench(pda_ctx* ctx, int op, int iters, float param, float* ms
     return PDA_OK;
 }
 
+int pda_debug_guide(pda_ctx* ctx, int sid, void* thr, int64_t thr_bytes, void* guide,
+                    int64_t guide_bytes) {
+    Slot* s = slot_of(ctx, sid);
+    if (!s) return fail(PDA_E_ARG, "bad stream id %d", sid);
+    if (!thr || !guide || thr_bytes < int64_t(sizeof(ThrPair) * kGuideThrPairs) ||
+        guide_bytes < int64_t(kGuideBytes))
+        return fail(PDA_E_ARG, "buffers below %zu / %zu bytes", sizeof(ThrPair) * kGuideThrPairs,
+                    kGuideBytes);
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(thr, s->d_thr, sizeof(ThrPair) * kGuideThrPairs, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(guide, s->d_guide, kGuideBytes, hipMemcpyDeviceToHost));
+    return PDA_OK;
+}
+
 int pda_debug_primitive(pda_ctx* ctx, int op, const void* in, void* out, int64_t n) {
     // input / output bytes per item for each op
     static const size_t in_sz[10] = {24, 4, 4, 8, 8, 4, 8, 48, 40, 4};

@@ -187,19 +187,69 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
-// Threshold rows of the fast walk's binomial draws (build_guide_row), one
-// lane per (state, photon count) row of the slot's parameter set; runs on
-// the slot's stream after the model upload, so every later walk of that set
-// reads finished rows.
+// Threshold rows of the fast walk's binomial draws, one wave per (state,
+// photon count) row of the slot's parameter set (the operations and their
+// order are those of build_guide_row_host); runs on the slot's stream after
+// the model upload, so every later walk of that set reads finished rows.
 __global__ __launch_bounds__(kWave) void guide_kernel(const ModelBlob* __restrict__ blob,
                                                       ThrPair* __restrict__ thr,
                                                       uint8_t* __restrict__ guide) {
-    const uint32_t row = blockIdx.x * kWave + threadIdx.x;
-    const int s = int(row / kBinomTable);
-    const int m = int(row % kBinomTable);
-    if (s >= blob->m.n) return;
-    build_guide_row(blob->m, blob->tab, s, m, thr + size_t(row) * kGuideRow,
-                    guide + size_t(row) * kGuideBuckets);
+    __shared__ int64_t s_up[kBinomTable + 1];
+    const int row = int(blockIdx.x);
+    const int s = row / kBinomTable;
+    const int m = row % kBinomTable;
+    if (s >= blob->m.n) return;  // uniform over the workgroup
+    const int lane = int(threadIdx.x);
+    const double pp = blob->m.pp[s];
+    const double q = blob->m.qq[s];
+    const int bound = blob->tab.bound[s][m];
+    const double qn = blob->tab.qn[s][m];
+    ThrPair* t = thr + guide_row_offset(s, m);
+    double carryP = 1.0, carryC = 0.0;
+    int end = -1;
+    bool covered = false;
+    for (int c = 0; c * kWave < kBinomTable && end < 0; ++c) {
+        const int k = c * kWave + lane;
+        double P = k == 0 ? 1.0 : (double(m - k + 1) * pp) / (double(k) * q);
+        for (int off = 1; off < kWave; off <<= 1) {
+            const double o = __shfl_up(P, off);
+            if (lane >= off) P = P * o;
+        }
+        const double Pf = P * carryP;
+        double C = qn * Pf;
+        for (int off = 1; off < kWave; off <<= 1) {
+            const double o = __shfl_up(C, off);
+            if (lane >= off) C = C + o;
+        }
+        const double Cf = C + carryC;
+        const double cs = Cf * 0x1p53;
+        const int64_t up = int64_t(floor(cs)) + kGuideBand + 1;
+        const int64_t lo = int64_t(ceil(cs)) - kGuideBand - 1;
+        const bool full = lo >= kU53Max;
+        const uint64_t stop = __ballot(full || k >= bound);
+        const int first = stop ? int(__builtin_ctzll(stop)) : kWave;
+        if (lane <= first && lane < kWave) {
+            t[k] = ThrPair{up, lo};
+            s_up[k] = up;
+        }
+        if (stop) {
+            end = c * kWave + first;
+            covered = __shfl(full ? 1 : 0, first) != 0;
+        }
+        carryP = __shfl(Pf, kWave - 1);
+        carryC = __shfl(Cf, kWave - 1);
+    }
+    if (lane == 0) t[end + 1] = ThrPair{INT64_MAX, covered ? INT64_MAX : int64_t(-1)};
+    __syncthreads();
+    // guide: entries with up <= j 2^47, by binary search over the row
+    const int64_t u0 = int64_t(lane) << kGuideShift;
+    int lo_i = 0, hi_i = end + 1;  // count in [lo_i, hi_i]
+    while (lo_i < hi_i) {
+        const int mid = (lo_i + hi_i) >> 1;
+        if (s_up[mid] <= u0) lo_i = mid + 1;
+        else hi_i = mid;
+    }
+    guide[size_t(row) * kGuideBuckets + size_t(lane)] = uint8_t(lo_i);
 }
 
 constexpr uint32_t kFinishWaves = 16;
@@ -766,8 +816,7 @@ hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const u
 hipError_t launch_guide(const ModelBlob* blob, int nstates, ThrPair* thr, uint8_t* guide,
                         hipStream_t stream) {
     const uint32_t rows = uint32_t(nstates) * kBinomTable;
-    hipLaunchKernelGGL(guide_kernel, dim3((rows + kWave - 1) / kWave), dim3(kWave), 0, stream,
-                       blob, thr, guide);
+    hipLaunchKernelGGL(guide_kernel, dim3(rows), dim3(kWave), 0, stream, blob, thr, guide);
     return hipGetLastError();
 }
 

@@ -72,22 +72,28 @@ struct BinomTable {
 };
 
 // Thresholds of the fast walk's binomial inversion on the 53-bit uniform
-// grid (binomial_guided, build_guide_row in pda_walk.h): row (s, m) holds,
+// grid (binomial_guided and build_guide_row_host in pda_walk.h, guide_kernel):
+// row (s, m) holds,
 // for X = 0 .. its last entry, a pair (up, lo) of grid indices -- u >= up
 // proves that the inversion goes past X, u <= lo that it stops at or before
 // X -- followed by a sentinel, and a guide of kGuideBuckets entries: the
 // first X that a uniform in each 1/64 of [0, 1) can end at.
 constexpr int kGuideBuckets = 64;
 constexpr int kGuideShift = 47;            // 53-bit grid index >> 47 = bucket
-constexpr int kGuideRow = kBinomTable + 1;  // pairs per row, sentinel included
 struct ThrPair {
     int64_t up, lo;
 };
+// Row (s, m) holds at most m + 2 pairs (X = 0 .. bound <= m, the sentinel):
+// rows of a state packed one after another, m (m + 3) / 2 pairs before row m.
+constexpr size_t kGuideStatePairs = size_t(kBinomTable) * (kBinomTable + 3) / 2;
+constexpr size_t kGuideThrPairs = size_t(kMaxStates) * kGuideStatePairs;
+constexpr size_t guide_row_offset(int s, int m) {
+    return size_t(s) * kGuideStatePairs + size_t(m) * size_t(m + 3) / 2;
+}
 struct BinomGuide {
-    const ThrPair* thr;    // [kMaxStates][kBinomTable][kGuideRow]
+    const ThrPair* thr;    // kGuideThrPairs, rows at guide_row_offset
     const uint8_t* guide;  // [kMaxStates][kBinomTable][kGuideBuckets]
 };
-constexpr size_t kGuideThrPairs = size_t(kMaxStates) * kBinomTable * kGuideRow;
 constexpr size_t kGuideBytes = size_t(kMaxStates) * kBinomTable * kGuideBuckets;
 
 // What a stream slot uploads per parameter set.

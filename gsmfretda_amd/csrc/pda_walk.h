@@ -71,8 +71,8 @@ static_assert(kInversionMaxN >= kBinomTable, "every tabulated count draws by inv
 
 constexpr int64_t kU53Max = (int64_t(1) << 53) - 1;  // largest 53-bit grid index
 
-// Row (s, m) of the guided inversion: binomial_table's X as a function of
-// the uniform's grid index u (U = u 2^-53 exactly, uniform53).
+// Rows of the guided inversion: binomial_table's X as a function of the
+// uniform's grid index u (U = u 2^-53 exactly, uniform53).
 //
 // The loop keeps U_0 = U, U_{i+1} = fl(U_i - px_i) and goes past i while
 // U_i > px_i, where px_i does not depend on U.  Rounding is monotone, so
@@ -80,50 +80,82 @@ constexpr int64_t kU53Max = (int64_t(1) << 53) - 1;  // largest 53-bit grid inde
 // exactly for U at or above some grid point.  Locating it: every subtraction
 // rounds a value in (0, 1), so U_i is within i 2^-54 of U - S_{i-1}, with
 // S_k = px_0 + ... + px_k exact; hence U > S_k + k 2^-54 gives X >= k + 1
-// and U <= S_k - k 2^-54 gives X <= k.  The running sum
-// C_k = fl(C_{k-1} + px_k) of the loop's own px_k (same operations) stays
-// within k 2^-53 of S_k (its values are below 2), so with d_k = 2k + 2:
-//   up_k = floor(C_k 2^53) + d_k + 1:  u >= up_k  =>  U > C_k + d_k 2^-53  =>  X >= k + 1
-//   lo_k = ceil(C_k 2^53) - d_k - 1:   u <= lo_k  =>  U < C_k - d_k 2^-53  =>  X <= k
-// The row ends at the bound (then a sentinel makes X = bound + 1 a replay,
-// as the loop's "X > bound") or at the first k whose lo_k covers the whole
-// grid (X <= k for every uniform).  guide[j] is the number of leading
-// entries with up <= j 2^47, the least X of any u in bucket j.
-PDA_HD int build_guide_row(const Model& M, const BinomTable& tab, int s, int m, ThrPair* thr,
-                           uint8_t* guide) {
+// and U <= S_k - k 2^-54 gives X <= k.
+//
+// The rows hold C_k, an approximation of S_k computed in parallel (below),
+// with |C_k - S_k| + k 2^-54 < 2^-42 = kGuideBand grid steps, and
+//   up_k = floor(C_k 2^53) + kGuideBand + 1:  u >= up_k  =>  X >= k + 1
+//   lo_k = ceil(C_k 2^53) - kGuideBand - 1:   u <= lo_k  =>  X <= k
+// The error bound, with eps = 2^-53 and k <= 127: the loop's px_k is
+// qn rho_0 ... rho_{k-1} (rho_i = (m - i) p / ((i + 1) q), qn = q^m as
+// tabulated) times 1 + t with |t| <= gamma(4k) (four roundings a step); here
+// each ratio is rounded three times and the prefix product of k of them and
+// qn k times more, also gamma(4k); both are within 2 gamma(4k + 1) < 1019 eps
+// relative of the same exact product, whose sum over k is qn / q^m <= 1.0001.
+// Any order of summing k + 1 such terms adds at most gamma(k) < 128 eps.
+// With k 2^-54 <= 64 eps that is < 1211 eps < 2048 eps.
+//
+// A row ends at the bound (then a sentinel makes X = bound + 1 a replay, as
+// the loop's "X > bound") or at the first k whose lo_k covers the whole grid
+// (X <= k for every uniform).  guide[j] is the number of leading entries
+// with up <= j 2^47, the least X of any u in bucket j.
+constexpr int64_t kGuideBand = 2048;
+
+// The device builds a row with one wave (guide_kernel): chunks of 64
+// entries, lane l of chunk c holding k = 64 c + l; ratio r_k (r_0 = 1),
+// Hillis-Steele inclusive scans (offsets 1, 2, ..., 32) of the ratios'
+// product and of px_k = qn P_k, each chunk continuing from the last lane of
+// the one before.  This host function performs the same operations in the
+// same order, so its rows equal the device's bit for bit.
+inline int build_guide_row_host(const Model& M, const BinomTable& tab, int s, int m, ThrPair* thr,
+                                uint8_t* guide) {
     const double pp = M.pp[s];
     const double q = M.qq[s];
     const int bound = tab.bound[s][m];
-    double px = tab.qn[s][m];
-    double C = px;
-    int k = 0;
+    const double qn = tab.qn[s][m];
+    double carryP = 1.0, carryC = 0.0;
+    int end = -1;
     bool covered = false;
-    for (;; ++k) {
-        const double cs = C * 0x1p53;  // exact scaling, below 2^54
-        const int64_t d = 2 * int64_t(k) + 2;
-        const int64_t lo = int64_t(ceil(cs)) - d - 1;
-        thr[k] = ThrPair{int64_t(floor(cs)) + d + 1, lo};
-        if (lo >= kU53Max) {
-            covered = true;
-            break;
+    for (int c = 0; c * kWave < kBinomTable && end < 0; ++c) {
+        double P[kWave], C[kWave], t[kWave];
+        for (int l = 0; l < kWave; ++l) {
+            const int k = c * kWave + l;
+            P[l] = k == 0 ? 1.0 : (double(m - k + 1) * pp) / (double(k) * q);
         }
-        if (k >= bound) break;
-        px = (double(m - k) * pp * px) / (double(k + 1) * q);
-        C = C + px;
+        for (int off = 1; off < kWave; off <<= 1) {
+            for (int l = 0; l < kWave; ++l) t[l] = l >= off ? P[l] * P[l - off] : P[l];
+            for (int l = 0; l < kWave; ++l) P[l] = t[l];
+        }
+        for (int l = 0; l < kWave; ++l) C[l] = qn * (P[l] * carryP);
+        for (int off = 1; off < kWave; off <<= 1) {
+            for (int l = 0; l < kWave; ++l) t[l] = l >= off ? C[l] + C[l - off] : C[l];
+            for (int l = 0; l < kWave; ++l) C[l] = t[l];
+        }
+        for (int l = 0; l < kWave && end < 0; ++l) {
+            const int k = c * kWave + l;
+            const double cs = (C[l] + carryC) * 0x1p53;
+            const int64_t lo = int64_t(ceil(cs)) - kGuideBand - 1;
+            thr[k] = ThrPair{int64_t(floor(cs)) + kGuideBand + 1, lo};
+            if (lo >= kU53Max) covered = true;
+            if (covered || k >= bound) end = k;
+        }
+        carryP = P[kWave - 1] * carryP;
+        carryC = C[kWave - 1] + carryC;
     }
-    thr[k + 1] = ThrPair{INT64_MAX, covered ? INT64_MAX : int64_t(-1)};
-    int x = 0;
+    thr[end + 1] = ThrPair{INT64_MAX, covered ? INT64_MAX : int64_t(-1)};
     for (int j = 0; j < kGuideBuckets; ++j) {
         const int64_t u0 = int64_t(j) << kGuideShift;
-        while (x <= k && thr[x].up <= u0) ++x;
+        int x = 0;
+        while (x <= end && thr[x].up <= u0) ++x;
         guide[j] = uint8_t(x);
     }
-    return k + 1;
+    return end + 1;
 }
 
 // binomial_table by the thresholds: the same X for every uniform, or -1
 // where binomial_table returns -1 and where u falls between lo_X and up_X
-// (about (4X + 6) grid points per row: the general form then runs the loop).
+// (2 kGuideBand + 2 grid points around each threshold, ~5e-13 of the grid:
+// the general form then runs the loop).
 // Two dependent loads instead of X + 1 loop passes whose count differs from
 // lane to lane.
 PDA_HD int64_t binomial_guided(int n, int s, const Model& M, const BinomGuide& G, uint32_t hi,
@@ -133,7 +165,7 @@ PDA_HD int64_t binomial_guided(int n, int s, const Model& M, const BinomGuide& G
     const int64_t u = (int64_t(hi >> 5) << 26) | int64_t(lo >> 6);
     const size_t row = size_t(s) * kBinomTable + size_t(n);
     int X = G.guide[row * kGuideBuckets + size_t(u >> kGuideShift)];
-    const ThrPair* t = G.thr + row * kGuideRow;
+    const ThrPair* t = G.thr + guide_row_offset(s, n);
     ThrPair e = t[X];
     while (u >= e.up) e = t[++X];
     if (u > e.lo) return -1;

@@ -42,7 +42,7 @@ EXPORTS = (
     "pda_rb_jacobian", "pda_rb_stats", "pda_rb_forward_population",
     "pda_rb_get_population_result", "pda_rb_adjoint_population", "pda_rb_jacobian_population",
     "pda_run_noalloc", "pda_debug_primitive",
-    "pda_debug_microbench",
+    "pda_debug_microbench", "pda_debug_guide",
 )
 
 
@@ -117,6 +117,7 @@ def load_library(path=LIB_PATH):
                                        ctypes.POINTER(ctypes.c_float),
                                        ctypes.POINTER(ctypes.c_int64),
                                        ctypes.POINTER(ctypes.c_int64)]
+    L.pda_debug_guide.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int64, _vp, ctypes.c_int64]
     _ip = ctypes.POINTER(ctypes.c_int)
     L.pda_set_launch_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.pda_get_launch_shape.argtypes = [_vp, _ip, _ip, _ip]
@@ -519,6 +520,16 @@ class MC:
         out = np.zeros(n * {0: 4}.get(op, 1), out_dtype)
         _check(self._L.pda_debug_primitive(self._h, op, inp.ctypes.data, out.ctypes.data, n))
         return out
+
+    def debug_guide(self, sid=0):
+        """The slot's binomial threshold rows as the GPU built them: (thr int64
+        [8, 8384, 2] of (up, lo), row m of a state at m (m + 3) / 2, guide uint8
+        [8, 128, 64])."""
+        thr = np.zeros((8, 8384, 2), np.int64)
+        guide = np.zeros((8, 128, 64), np.uint8)
+        _check(self._L.pda_debug_guide(self._h, sid, thr.ctypes.data, thr.nbytes,
+                                       guide.ctypes.data, guide.nbytes))
+        return thr, guide
 
     def debug_microbench(self, op, iters, param=0.0):
         """(GPU ms, events, inner iterations) of one event microbenchmark launch."""

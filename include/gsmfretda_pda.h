@@ -330,6 +330,17 @@ int pda_debug_primitive(pda_ctx *ctx, int op, const void *in, void *out, int64_t
 int pda_debug_microbench(pda_ctx *ctx, int op, int iters, float param, float *ms,
                          int64_t *events, int64_t *inner);
 
+/* Diagnostics (no reference counterpart): copies slot `sid`'s threshold rows
+ * of the fast walk's binomial draws, as its last parameter upload built them
+ * on the GPU, to host memory: `thr` receives 8 * 8,384 pairs of int64
+ * (up, lo), the rows of each state packed with m (m + 3) / 2 pairs before the
+ * row of photon count m, m + 2 pairs per row (thr_bytes at least 1,073,152),
+ * `guide` 8 * 128 * 64 bytes (guide_bytes at least 65,536).  Rows of states
+ * beyond the set's are not written.  Blocks until the slot's stream is
+ * idle. */
+int pda_debug_guide(pda_ctx *ctx, int sid, void *thr, int64_t thr_bytes, void *guide,
+                    int64_t guide_bytes);
+
 #ifdef __cplusplus
 }
 #endif

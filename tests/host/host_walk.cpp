@@ -31,9 +31,31 @@ static void build_guide(const Model& M, const BinomTable& tab, std::vector<ThrPa
     for (int s = 0; s < M.n; ++s)
         for (int m = 0; m < kBinomTable; ++m) {
             const size_t row = size_t(s) * kBinomTable + size_t(m);
-            build_guide_row(M, tab, s, m, thr.data() + row * kGuideRow,
+            build_guide_row_host(M, tab, s, m, thr.data() + guide_row_offset(s, m),
                             guide.data() + row * kGuideBuckets);
         }
+}
+
+// The rows of one model as build_guide_row_host makes them (the layout of
+// pda_debug_guide).
+extern "C" int hw_build_guide(int n, const float* args, const float* photophysics, ThrPair* thr,
+                              uint8_t* guide) {
+    Photophysics ph;
+    ph.gamma = photophysics[0];
+    ph.dexdir = photophysics[1];
+    ph.crosstalk = photophysics[2];
+    ph.r0 = photophysics[3];
+    Model M;
+    int rc = build_model(ph, n, args, n * (n + 1), M, g_err);
+    if (rc) return rc;
+    BinomTable tab;
+    build_binom_table(M, tab);
+    std::vector<ThrPair> t;
+    std::vector<uint8_t> g;
+    build_guide(M, tab, t, g);
+    std::memcpy(thr, t.data(), sizeof(ThrPair) * t.size());
+    std::memcpy(guide, g.data(), g.size());
+    return 0;
 }
 
 static uint64_t splitmix(uint64_t& x) {
@@ -92,7 +114,7 @@ extern "C" int hw_guide_check(int n, const float* args, const float* photophysic
     for (int s = 0; s < M.n; ++s)
         for (int m = 1; m < kBinomTable; ++m) {
             const size_t row = size_t(s) * kBinomTable + size_t(m);
-            const ThrPair* t = thr.data() + row * kGuideRow;
+            const ThrPair* t = thr.data() + guide_row_offset(s, m);
             const uint8_t* g = guide.data() + row * kGuideBuckets;
             int len = 0;
             while (t[len].up != INT64_MAX) ++len;

@@ -61,7 +61,7 @@ def test_guided_draw_is_the_inversion_loop(hw, name):
     assert guide_bad == 0
     assert rnd_band == 0          # a random uniform falls in a band with probability ~1e-14
     assert 0 < band < compared    # the probes inside the bands are replays
-    assert longest <= 129
+    assert longest <= 128
 
 
 def test_host_walk_guided_equals_loop(hw):
@@ -83,3 +83,38 @@ def test_host_walk_guided_equals_loop(hw):
     assert np.array_equal(f1.view(np.uint32), f0.view(np.uint32))
     assert np.array_equal(e1.view(np.uint32), e0.view(np.uint32))
     assert r1 == r0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(MODELS))
+def test_device_rows_equal_host_rows(hw, name):
+    """The rows guide_kernel builds on the GPU equal build_guide_row_host's
+    bit for bit (same operations, same order), so the equivalence checks
+    above hold for the rows the GPU walk reads."""
+    from gsmfretda_amd import pda
+    n, args = MODELS[name]
+    a = np.ascontiguousarray(args, np.float32)
+    ph = np.asarray(synth.PHOTOPHYSICS, np.float32)
+    h_thr = np.zeros((8, 8384, 2), np.int64)
+    h_guide = np.zeros((8, 128, 64), np.uint8)
+    assert hw.hw_build_guide(n, a.ctypes.data_as(_P(ctypes.c_float)), ph.ctypes.data_as(_P(ctypes.c_float)),
+                             h_thr.ctypes.data_as(_P(ctypes.c_int64)),
+                             h_guide.ctypes.data_as(_P(ctypes.c_uint8))) == 0
+    m = pda.MC(0, 1)
+    try:
+        m.set_photophysics(0, *synth.PHOTOPHYSICS)
+        assert m.set_params(n, 0, a)
+        d_thr, d_guide = m.debug_guide(0)
+    finally:
+        m.close()
+    rows = 0
+    for s in range(n):
+        for mm in range(128):
+            off = mm * (mm + 3) // 2
+            h_row, d_row = h_thr[s, off:off + mm + 2], d_thr[s, off:off + mm + 2]
+            end = int(np.argmax(h_row[:, 0] == np.iinfo(np.int64).max))
+            assert end > 0
+            assert np.array_equal(d_row[:end + 1], h_row[:end + 1]), (s, mm)
+            assert np.array_equal(d_guide[s, mm], h_guide[s, mm]), (s, mm)
+            rows += 1
+    assert rows == n * 128
