@@ -177,7 +177,7 @@ int upload_model(Slot& s) {
     s.any_v = M.any_v != 0;
     HIP_TRY(hipMemcpyAsync(s.d_blob, s.h_blob, sizeof(ModelBlob), hipMemcpyHostToDevice, s.stream));
     HIP_TRY(hipEventRecord(s.model_copied, s.stream));
-    if (!s.any_v) HIP_TRY(launch_guide(s.d_blob, M.n, s.d_thr, s.d_guide, s.stream));
+    if (!s.any_v) HIP_TRY(launch_guide(s.d_blob, 1, M.n, s.d_thr, s.d_guide, s.stream));
     s.model_pending = true;
     return PDA_OK;
 }

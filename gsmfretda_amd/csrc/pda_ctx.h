@@ -50,6 +50,11 @@ struct PopGraphKey {
     const void* links = nullptr;
     uint64_t data_gen = 0;
     const void* obs = nullptr;
+    // binomial threshold rows: whether the walk reads them, the guide
+    // kernel's state count and the table's buffer
+    bool guided = false;
+    int max_states = 0;
+    const void* thr = nullptr;
     bool operator==(const PopGraphKey& o) const;
 };
 
@@ -71,6 +76,13 @@ struct Population {
     uint32_t* d_replay_count = nullptr;
     size_t replay_cap = 0;
     uint32_t* d_work = nullptr;      // kWorkWords: queue counters, one per line
+    // binomial threshold rows of every set (fast form; set i's rows start at
+    // its model's guide_base), when the table fits kPopGuideBudget
+    ThrPair* d_thr = nullptr;
+    uint8_t* d_guide = nullptr;
+    size_t cap_guide_states = 0;
+    bool guided = false;
+    int max_states = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     PopGraphKey key;
@@ -242,16 +254,17 @@ hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const u
                        const RunArgs& a, int32_t* hist_acc, uint32_t* replay,
                        uint32_t* replay_count, uint32_t* work, float* outE, int32_t* outAcc,
                        float* outF, hipStream_t stream);
-hipError_t launch_guide(const ModelBlob* blob, int nstates, ThrPair* thr, uint8_t* guide,
-                        hipStream_t stream);
+hipError_t launch_guide(const ModelBlob* blobs, int nsets, int max_states, ThrPair* thr,
+                        uint8_t* guide, hipStream_t stream);
 hipError_t launch_finish(const RunArgs& a, const float* obs, int32_t* hist_acc,
                          uint32_t* replay_count, uint32_t* work, ResultHeader* result,
                          hipStream_t stream);
+// guide.thr == nullptr: the fast form draws binomials by the inversion loop
 hipError_t launch_pop_walk(int pass, uint32_t grid, const BurstDesc* bursts,
                            const uint32_t* ticks, const uint2* links, const PopControl* ctl,
-                           const ModelBlob* blobs, const RunArgs& a, int32_t* hist_acc,
-                           uint64_t* replay, uint32_t* replay_count, uint32_t* work,
-                           hipStream_t stream);
+                           const ModelBlob* blobs, const BinomGuide& guide, const RunArgs& a,
+                           int32_t* hist_acc, uint64_t* replay, uint32_t* replay_count,
+                           uint32_t* work, hipStream_t stream);
 hipError_t launch_pop_replay(const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
                              const PopControl* ctl, const ModelBlob* blobs, const RunArgs& a,
                              int32_t* hist_acc, const uint64_t* replay,

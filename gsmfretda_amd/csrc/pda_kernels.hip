@@ -188,23 +188,26 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
 }
 
 // Threshold rows of the fast walk's binomial draws, one wave per (state,
-// photon count) row of the slot's parameter set (the operations and their
-// order are those of build_guide_row_host); runs on the slot's stream after
-// the model upload, so every later walk of that set reads finished rows.
-__global__ __launch_bounds__(kWave) void guide_kernel(const ModelBlob* __restrict__ blob,
+// photon count) row of a parameter set (the operations and their order are
+// those of build_guide_row_host): blockIdx.x = state * kBinomTable + count,
+// blockIdx.y = set; set i's state s takes table row blobs[i].m.guide_base + s.
+// Runs on the stream after the model upload, so the walks that follow read
+// finished rows.
+__global__ __launch_bounds__(kWave) void guide_kernel(const ModelBlob* __restrict__ blobs,
                                                       ThrPair* __restrict__ thr,
                                                       uint8_t* __restrict__ guide) {
     __shared__ int64_t s_up[kBinomTable + 1];
-    const int row = int(blockIdx.x);
-    const int s = row / kBinomTable;
-    const int m = row % kBinomTable;
+    const ModelBlob* blob = blobs + blockIdx.y;
+    const int s = int(blockIdx.x) / kBinomTable;
+    const int m = int(blockIdx.x) % kBinomTable;
     if (s >= blob->m.n) return;  // uniform over the workgroup
+    const int srow = blob->m.guide_base + s;
     const int lane = int(threadIdx.x);
     const double pp = blob->m.pp[s];
     const double q = blob->m.qq[s];
     const int bound = blob->tab.bound[s][m];
     const double qn = blob->tab.qn[s][m];
-    ThrPair* t = thr + guide_row_offset(s, m);
+    ThrPair* t = thr + guide_row_offset(srow, m);
     double carryP = 1.0, carryC = 0.0;
     int end = -1;
     bool covered = false;
@@ -249,7 +252,7 @@ __global__ __launch_bounds__(kWave) void guide_kernel(const ModelBlob* __restric
         if (s_up[mid] <= u0) lo_i = mid + 1;
         else hi_i = mid;
     }
-    guide[size_t(row) * kGuideBuckets + size_t(lane)] = uint8_t(lo_i);
+    guide[(size_t(srow) * kBinomTable + size_t(m)) * kGuideBuckets + size_t(lane)] = uint8_t(lo_i);
 }
 
 constexpr uint32_t kFinishWaves = 16;
@@ -348,12 +351,15 @@ __device__ __forceinline__ void flush_hist(int32_t* h, int32_t* __restrict__ his
 // workgroup keeps two LDS model slots with an LDS histogram each, and a wave
 // moves to a third set only after the older slot's lanes have finished, so it
 // never drains to change sets.
-template <int kPass>
+// kGuided (fast form): binomial draws from the sets' threshold rows (guide),
+// else by the inversion loop from each set's BinomTable.
+template <int kPass, bool kGuided>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     kPass == kPassFast ? kFastWavesPerEU : 1))) void pop_walk_kernel(
     const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
     const uint2* __restrict__ links, const PopControl* __restrict__ ctl,
-    const ModelBlob* __restrict__ blobs, const RunArgs a, int32_t* __restrict__ hist_acc,
+    const ModelBlob* __restrict__ blobs, const BinomGuide guide, const RunArgs a,
+    int32_t* __restrict__ hist_acc,
     uint64_t* __restrict__ replay, uint32_t* __restrict__ replay_count,
     uint32_t* __restrict__ work) {
     __shared__ Model sM[2];
@@ -437,9 +443,18 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
         }
         if (!__any(busy)) break;
         if (busy) {
-            const int r = step_window<kFastForm>(sM[my_slot], &blobs[my_set].tab, key, d.P,
-                                                 d.stop_rel, d.T, ticks + d.off, links + d.off,
-                                                 a, st);
+            int r;
+            if (kGuided) {
+                const size_t base = size_t(sM[my_slot].guide_base);
+                const BinomGuide g{guide.thr + base * kGuideStatePairs,
+                                   guide.guide + base * kBinomTable * kGuideBuckets};
+                r = step_window<kFastForm, false, BinomGuide>(sM[my_slot], &g, key, d.P,
+                                                              d.stop_rel, d.T, ticks + d.off,
+                                                              links + d.off, a, st);
+            } else {
+                r = step_window<kFastForm>(sM[my_slot], &blobs[my_set].tab, key, d.P, d.stop_rel,
+                                           d.T, ticks + d.off, links + d.off, a, st);
+            }
             if (r != kStepMore) {
                 if (r == kStepReplay) {
                     const uint32_t slot = atomicAdd(replay_count, 1u);
@@ -813,10 +828,10 @@ hipError_t launch_walk(int pass, uint32_t grid, const BurstDesc* bursts, const u
     return hipGetLastError();
 }
 
-hipError_t launch_guide(const ModelBlob* blob, int nstates, ThrPair* thr, uint8_t* guide,
-                        hipStream_t stream) {
-    const uint32_t rows = uint32_t(nstates) * kBinomTable;
-    hipLaunchKernelGGL(guide_kernel, dim3(rows), dim3(kWave), 0, stream, blob, thr, guide);
+hipError_t launch_guide(const ModelBlob* blobs, int nsets, int max_states, ThrPair* thr,
+                        uint8_t* guide, hipStream_t stream) {
+    hipLaunchKernelGGL(guide_kernel, dim3(uint32_t(max_states) * kBinomTable, uint32_t(nsets)),
+                       dim3(kWave), 0, stream, blobs, thr, guide);
     return hipGetLastError();
 }
 
@@ -831,26 +846,30 @@ hipError_t launch_finish(const RunArgs& a, const float* obs, int32_t* hist_acc,
 hipError_t pop_walk_occupancy(int pass, int B, int* blocks_per_cu) {
     const size_t lds = sizeof(int32_t) * 2u * size_t(B);
     if (pass == kPassFast)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
-                                                            pop_walk_kernel<kPassFast>, kWave, lds);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
-                                                        pop_walk_kernel<kPassGeneral>, kWave, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            blocks_per_cu, pop_walk_kernel<kPassFast, true>, kWave, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, pop_walk_kernel<kPassGeneral, false>, kWave, lds);
 }
 
 hipError_t launch_pop_walk(int pass, uint32_t grid, const BurstDesc* bursts,
                            const uint32_t* ticks, const uint2* links, const PopControl* ctl,
-                           const ModelBlob* blobs, const RunArgs& a, int32_t* hist_acc,
-                           uint64_t* replay, uint32_t* replay_count, uint32_t* work,
-                           hipStream_t stream) {
+                           const ModelBlob* blobs, const BinomGuide& guide, const RunArgs& a,
+                           int32_t* hist_acc, uint64_t* replay, uint32_t* replay_count,
+                           uint32_t* work, hipStream_t stream) {
     const size_t lds = sizeof(int32_t) * 2u * size_t(a.B);
-    if (pass == kPassFast)
-        hipLaunchKernelGGL(pop_walk_kernel<kPassFast>, dim3(grid), dim3(kWave), lds, stream,
-                           bursts, ticks, links, ctl, blobs, a, hist_acc, replay, replay_count,
-                           work);
+    if (pass == kPassFast && guide.thr)
+        hipLaunchKernelGGL((pop_walk_kernel<kPassFast, true>), dim3(grid), dim3(kWave), lds,
+                           stream, bursts, ticks, links, ctl, blobs, guide, a, hist_acc, replay,
+                           replay_count, work);
+    else if (pass == kPassFast)
+        hipLaunchKernelGGL((pop_walk_kernel<kPassFast, false>), dim3(grid), dim3(kWave), lds,
+                           stream, bursts, ticks, links, ctl, blobs, guide, a, hist_acc, replay,
+                           replay_count, work);
     else
-        hipLaunchKernelGGL(pop_walk_kernel<kPassGeneral>, dim3(grid), dim3(kWave), lds, stream,
-                           bursts, ticks, links, ctl, blobs, a, hist_acc, replay, replay_count,
-                           work);
+        hipLaunchKernelGGL((pop_walk_kernel<kPassGeneral, false>), dim3(grid), dim3(kWave), lds,
+                           stream, bursts, ticks, links, ctl, blobs, guide, a, hist_acc, replay,
+                           replay_count, work);
     return hipGetLastError();
 }
 

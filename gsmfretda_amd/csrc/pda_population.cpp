@@ -33,7 +33,8 @@ bool PopGraphKey::operator==(const PopGraphKey& o) const {
            bg_on == o.bg_on && b0 == o.b0 && b1 == o.b1 && grid == o.grid && batch == o.batch &&
            queues == o.queues && replay_cap == o.replay_cap && clk_p == o.clk_p &&
            bg_ad == o.bg_ad && bg_dd == o.bg_dd && bursts == o.bursts && ticks == o.ticks &&
-           links == o.links && data_gen == o.data_gen && obs == o.obs;
+           links == o.links && data_gen == o.data_gen && obs == o.obs && guided == o.guided &&
+           max_states == o.max_states && thr == o.thr;
 }
 
 void drop_graph(Population& p) {
@@ -47,6 +48,11 @@ void free_population(Population& p) {
     drop_graph(p);
     (void)hipHostFree(p.h_in);
     (void)hipFree(p.d_in);
+    (void)hipFree(p.d_thr);
+    (void)hipFree(p.d_guide);
+    p.d_thr = nullptr;
+    p.d_guide = nullptr;
+    p.cap_guide_states = 0;
     (void)hipFree(p.d_hist_acc);
     (void)hipFree(p.d_out);
     (void)hipHostFree(p.h_out);
@@ -66,6 +72,10 @@ using namespace pda;
 namespace {
 
 constexpr int kMaxSets = 1 << 16;
+// Largest threshold table a population keeps for the fast form's guided
+// binomial draws (134 KB per state: 1 GiB covers about 8,000 states); a
+// larger population draws by the inversion loop.
+constexpr size_t kPopGuideBudget = size_t(1) << 30;
 
 size_t in_bytes(size_t nsets) { return sizeof(PopControl) + sizeof(ModelBlob) * nsets; }
 
@@ -129,8 +139,14 @@ int enqueue_population(pda_ctx* ctx, Slot& s, int pass, uint32_t grid, const Run
                                                 sizeof(SetResult) * size_t(p.nsets));
     HIP_TRY(hipMemcpyAsync(p.d_in, p.h_in, in_bytes(size_t(p.nsets)), hipMemcpyHostToDevice,
                            s.stream));
-    HIP_TRY(launch_pop_walk(pass, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, ctl, blobs, a,
-                            p.d_hist_acc, p.d_replay, p.d_replay_count, p.d_work, s.stream));
+    BinomGuide guide{nullptr, nullptr};
+    if (pass == 0 && p.guided) {
+        HIP_TRY(launch_guide(blobs, p.nsets, p.max_states, p.d_thr, p.d_guide, s.stream));
+        guide = BinomGuide{p.d_thr, p.d_guide};
+    }
+    HIP_TRY(launch_pop_walk(pass, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, ctl, blobs,
+                            guide, a, p.d_hist_acc, p.d_replay, p.d_replay_count, p.d_work,
+                            s.stream));
     if (pass == 0)
         HIP_TRY(launch_pop_replay(ctx->d_bursts, ctx->d_ticks, ctx->d_links, ctl, blobs, a,
                                   p.d_hist_acc, p.d_replay, p.d_replay_count,
@@ -232,6 +248,9 @@ int run_population(pda_ctx* ctx, Slot& s, int R, uint32_t seed, uint32_t set_bas
         k.links = ctx->d_links;
         k.data_gen = ctx->data_gen;
         k.obs = ctx->d_obs;
+        k.guided = pass == 0 && p.guided;
+        k.max_states = p.max_states;
+        k.thr = p.d_thr;
         if (!p.exec || !(k == p.key)) {
             drop_graph(p);
             HIP_TRY(hipStreamSynchronize(s.stream));
@@ -323,6 +342,29 @@ int pda_set_population(pda_ctx* ctx, int sid, int nsets, const int32_t* nstates,
         p.cap_sets = size_t(nsets);
         std::memset(p.h_in, 0, in_bytes(size_t(nsets)));
     }
+    // threshold rows of the fast form's binomial draws: set i's states take
+    // rows guide_base .. guide_base + n_i - 1 of one table
+    size_t states = 0;
+    int max_states = 0;
+    for (int i = 0; i < nsets; ++i) {
+        models[size_t(i)].guide_base = int32_t(states);
+        states += size_t(nstates[i]);
+        max_states = std::max(max_states, nstates[i]);
+    }
+    const bool guided = !any_v && states * kGuideStatePairs * sizeof(ThrPair) <= kPopGuideBudget;
+    if (guided && states > p.cap_guide_states) {
+        drop_graph(p);
+        (void)hipFree(p.d_thr);
+        (void)hipFree(p.d_guide);
+        p.d_thr = nullptr;
+        p.d_guide = nullptr;
+        p.cap_guide_states = 0;
+        HIP_TRY(hipMalloc(&p.d_thr, states * kGuideStatePairs * sizeof(ThrPair)));
+        HIP_TRY(hipMalloc(&p.d_guide, states * size_t(kBinomTable) * kGuideBuckets));
+        p.cap_guide_states = states;
+    }
+    p.guided = guided;
+    p.max_states = max_states;
     ModelBlob* hb = reinterpret_cast<ModelBlob*>(p.h_in + sizeof(PopControl));
     for (int i = 0; i < nsets; ++i) {
         hb[i].m = models[size_t(i)];

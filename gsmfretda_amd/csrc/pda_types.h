@@ -50,7 +50,10 @@ struct Model {
     float gmx;                    // gamma - Dch2Ach
     float omdd;                   // 1 - DexDirAem
     float r0;
-    float pad0[2];
+    // first state row of this set in its threshold table (population runs:
+    // the sets' rows follow one another; 0 for a slot's single set)
+    int32_t guide_base;
+    float pad0;
     float p0cum[kMaxStates];                      // running sums of genMatP's inflow
     float nscum[kMaxStates][kMaxStates];          // [s][jj] running sums of P_i2j column s
     float krate[kMaxStates][kMaxStates];          // [s][j] = K(j, s), rate s -> j

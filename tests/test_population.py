@@ -214,6 +214,26 @@ def test_population_up_to_eight_states(gpu_mc, oracle, data):
         assert c32[i] == np.float32(c_o), f"set {i}"
 
 
+def test_population_beyond_the_guide_budget(gpu_mc, data):
+    """A population whose binomial threshold table would exceed its 1 GiB
+    budget (8,240 states of 134,144 bytes) walks with the inversion loop; every
+    set checked equals its single-set run (which draws by the table)."""
+    d, obs = data
+    m = gpu_mc
+    _load(m, d, obs)
+    sets = _population(1030, 21, states=(8,))
+    assert len(sets) * 8 * 8384 * 16 > 1 << 30
+    m.set_population([n for n, _ in sets], [a for _, a in sets])
+    R, seed = 2, 515
+    m.run_population(R=R, seed=seed, set_base=7, set_stride=1)
+    hist, c64, c32, _ = m.get_population_result()
+    for i in (0, 1, 514, 1029):
+        n, a = sets[i]
+        h1, s64, s32, _ = pda.evaluate(m, n, a, R, seed, set_id=7 + i)
+        assert np.array_equal(hist[i], h1), f"set {i}"
+        assert c64[i] == s64 and c32[i] == s32
+
+
 def test_population_at_200_bins(gpu_mc, oracle, data):
     """The reference's default of 200 E bins: every set equal to the oracle."""
     d, obs = data
