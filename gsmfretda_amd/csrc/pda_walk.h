@@ -189,14 +189,39 @@ struct NoDeduce {
 
 // background_count() for means <= 64 (one inversion chunk); -1 otherwise
 // (also for caps of 2^30 photons or more, which a 32-bit counter could pass).
+// The inversion's first two steps run for every lane without a branch: their
+// divisions by 1 and 2 are exact (the identity and a power-of-two scaling),
+// so pk * lam and (pk * lam) * 0.5 round to the loop's values; the loop then
+// continues from k = 3 only for the lanes whose uniform lies beyond the
+// cumulative probability of 2 photons, so a wave no longer runs the loop as
+// often as its longest draw needs.
 PDA_HD float background_single(float lam, float cap, float u_first, float u_cap) {
     if (lam > 64.0f || !(cap < 0x1p30f)) return -1.0f;
     int32_t k = 0;
     bool over = false;
     if (lam > 0.0f) {
-        float pk = fexp(-lam);
-        float cdf = pk;
-        while (u_first > cdf) {
+        const float pk0 = fexp(-lam);
+        const float pk1 = pk0 * lam;
+        const float cdf1 = pk0 + pk1;
+        const float pk2 = (pk1 * lam) * 0.5f;
+        const float cdf2 = cdf1 + pk2;
+        float pk = pk0, cdf = pk0;
+        bool go = u_first > pk0;
+        if (go) {  // k = 1
+            k = 1;
+            over = 1.0f > cap;
+            pk = pk1;
+            cdf = cdf1;
+            go = !over && u_first > cdf1;
+        }
+        if (go) {  // k = 2
+            k = 2;
+            over = 2.0f > cap;
+            pk = pk2;
+            cdf = cdf2;
+            go = !over && u_first > cdf2;
+        }
+        while (go) {
             ++k;
             if (float(k) > cap) {
                 over = true;
@@ -204,6 +229,7 @@ PDA_HD float background_single(float lam, float cap, float u_first, float u_cap)
             }
             pk = (pk * lam) / float(k);
             cdf = cdf + pk;
+            go = u_first > cdf;
         }
     }
     if (!over && float(k) <= cap) return float(k);

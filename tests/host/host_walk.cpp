@@ -4,6 +4,7 @@
 // exit) can be compared with the oracle's literal photon scan without a GPU.
 // Test infrastructure; the product never runs this build.
 #include <cstdint>
+#include <cmath>
 #include <cstring>
 #include <utility>
 #include <string>
@@ -64,6 +65,69 @@ static uint64_t splitmix(uint64_t& x) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
+
+// The capped-Poisson draw as a plain inversion loop (the form of
+// background_count's single chunk, src/mc.cu:153-166), for comparison.
+static float poisson_loop(float lam, float cap, float u_first, float u_cap) {
+    if (lam > 64.0f || !(cap < 0x1p30f)) return -1.0f;
+    int32_t k = 0;
+    bool over = false;
+    if (lam > 0.0f) {
+        float pk = fexp(-lam);
+        float cdf = pk;
+        while (u_first > cdf) {
+            ++k;
+            if (float(k) > cap) {
+                over = true;
+                break;
+            }
+            pk = (pk * lam) / float(k);
+            cdf = cdf + pk;
+        }
+    }
+    if (!over && float(k) <= cap) return float(k);
+    return floorf(cap * u_cap);
+}
+
+// background_single against poisson_loop: out[0] draws compared, out[1]
+// results that differ (bitwise).  Means from [0, 64] on several scales, caps
+// of 0, 1/2, 1, 3/2, 2, 5/2, 3, 10 and random, uniforms on the 2^-24 grid at
+// random and at every cumulative sum of the first 8 steps and its neighbours.
+extern "C" int hw_poisson_check(int n_lam, uint64_t seed, int64_t* out) {
+    uint64_t st = seed;
+    auto unif = [&]() { return float(double(splitmix(st) >> 11) * 0x1p-53); };
+    const float caps[] = {0.0f, 0.5f, 1.0f, 1.5f, 2.0f, 2.5f, 3.0f, 10.0f};
+    out[0] = out[1] = 0;
+    auto cmp = [&](float lam, float cap, float u1, float u2) {
+        const float a = background_single(lam, cap, u1, u2);
+        const float b = poisson_loop(lam, cap, u1, u2);
+        ++out[0];
+        uint32_t ba, bb;
+        std::memcpy(&ba, &a, 4);
+        std::memcpy(&bb, &b, 4);
+        if (ba != bb) ++out[1];
+    };
+    for (int i = 0; i < n_lam; ++i) {
+        const int scale = i % 4;
+        float lam = scale == 0 ? unif() * 3.0f
+                  : scale == 1 ? unif() * 64.0f
+                  : scale == 2 ? std::ldexp(unif(), -int(splitmix(st) % 100))
+                               : float(splitmix(st) % 6) * 0.5f;
+        const float cap = (i / 4) % 9 < 8 ? caps[(i / 4) % 9] : unif() * 50.0f;
+        const float u2 = uniform24(uint32_t(splitmix(st)));
+        for (int r = 0; r < 8; ++r) cmp(lam, cap, uniform24(uint32_t(splitmix(st))), u2);
+        // the uniforms at the first cumulative sums and their neighbours
+        float pk = fexp(-lam), cdf = pk;
+        for (int k = 1; k <= 8 && lam > 0.0f; ++k) {
+            for (float u : {cdf, std::nextafter(cdf, 0.0f), std::nextafter(cdf, 2.0f)})
+                if (u > 0.0f && u <= 1.0f) cmp(lam, cap, u, u2);
+            pk = (pk * lam) / float(k);
+            cdf = cdf + pk;
+        }
+    }
+    return 0;
+}
+
 
 // The guided draw against the inversion loop of binomial_table on one
 // model: for every state and count 1..127, at every threshold pair +- 3 grid

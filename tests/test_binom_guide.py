@@ -118,3 +118,13 @@ def test_device_rows_equal_host_rows(hw, name):
             assert np.array_equal(d_guide[s, mm], h_guide[s, mm]), (s, mm)
             rows += 1
     assert rows == n * 128
+
+
+def test_poisson_peeled_steps_equal_the_loop(hw):
+    """background_single (first two inversion steps without a branch) equals
+    the plain inversion loop bitwise, at random uniforms and at every
+    cumulative sum of the first steps and its float neighbours."""
+    out = np.zeros(2, np.int64)
+    assert hw.hw_poisson_check(200_000, ctypes.c_uint64(3), out.ctypes.data_as(_P(ctypes.c_int64))) == 0
+    assert out[0] > 3_000_000
+    assert out[1] == 0
