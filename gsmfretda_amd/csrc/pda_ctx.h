@@ -125,6 +125,10 @@ struct RbState {
     // by the walk (capacity cap_units), and their count
     uint32_t* d_big = nullptr;
     uint32_t* d_big_count = nullptr;
+    // units the fast-form walk abandoned (capacity cap_units), and their
+    // count, for the general-form replay
+    uint32_t* d_replay = nullptr;
+    uint32_t* d_replay_count = nullptr;
     double* d_Sbar = nullptr;          // per burst: mean score of its R units
     size_t cap_sbar = 0;
     // pmf memo of the last forward (layout: the context's RbMemoLayout of n
@@ -280,13 +284,17 @@ hipError_t microbench_shape(int op, uint32_t* grid, uint32_t* threads);
 hipError_t launch_microbench(int op, uint32_t grid, int iters, float param, const double* qn,
                              const int32_t* bound, uint64_t* out, hipStream_t stream);
 struct RbModel;
-// which: 0/1 RB walk without/with scores; 2/3/4 rb_unit_kernel forward,
-// adjoint, Jacobian; 5/6/7 rb_big_kernel
+// which: 2/3/4 rb_unit_kernel forward, adjoint, Jacobian; 5/6/7 rb_big_kernel
 hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu);
-hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
-                          const uint2* links, const ModelBlob* blob, const RbModel* rbm,
-                          const RunArgs& a, uint2* cnt, float* F, float* S, uint32_t* work,
-                          uint32_t* big, uint32_t* big_count, hipStream_t stream);
+// the RB walk of n states (2..4), with or without scores
+hipError_t rb_walk_occupancy(bool scores, int n, int* per_cu);
+// the fast-form walk and, behind it, the general-form replay of the units it
+// abandoned (replay_grid workgroups; those beyond the list's need exit)
+hipError_t launch_rb_walk(bool scores, int n, uint32_t grid, uint32_t replay_grid,
+                          const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
+                          const ModelBlob* blob, const RbModel* rbm, const RunArgs& a, uint2* cnt,
+                          float* F, float* S, uint32_t* work, uint32_t* big, uint32_t* big_count,
+                          uint32_t* replay, uint32_t* replay_count, hipStream_t stream);
 // dynamic LDS bytes (mode: 0 forward, 1 adjoint, 2 Jacobian)
 int rb_unit_threads();
 size_t rb_unit_lds_bytes(int B, int mode);

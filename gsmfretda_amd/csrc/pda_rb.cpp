@@ -46,6 +46,8 @@ void free_rb(RbState& r) {
     (void)hipHostFree(r.h_jac);
     (void)hipFree(r.d_big);
     (void)hipFree(r.d_big_count);
+    (void)hipFree(r.d_replay);
+    (void)hipFree(r.d_replay_count);
     (void)hipFree(r.d_Sbar);
     (void)hipFree(r.d_poolP);
     (void)hipFree(r.d_poolD);
@@ -90,6 +92,7 @@ int ensure_rb(pda_ctx* ctx, RbState& r) {
         HIP_TRY(hipMemset(r.d_hist, 0, sizeof(unsigned long long) * 8u * kMaxBins));
         if (!r.d_work) HIP_TRY(hipMalloc(&r.d_work, sizeof(uint32_t) * kRbCounters));
         if (!r.d_big_count) HIP_TRY(hipMalloc(&r.d_big_count, sizeof(uint32_t)));
+        if (!r.d_replay_count) HIP_TRY(hipMalloc(&r.d_replay_count, sizeof(uint32_t)));
         HIP_TRY(hipMemset(r.d_work, 0, sizeof(uint32_t) * kRbCounters));
         if (!r.d_res) HIP_TRY(hipMalloc(&r.d_res, kRbResBytes));
         if (!r.h_res)
@@ -317,14 +320,17 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
         (void)hipFree(r.d_cnt);
         (void)hipFree(r.d_F);
         (void)hipFree(r.d_big);
+        (void)hipFree(r.d_replay);
         r.d_cnt = nullptr;
         r.d_F = nullptr;
         r.d_big = nullptr;
+        r.d_replay = nullptr;
         r.cap_units = 0;
         const size_t nu = size_t(std::max<int64_t>(units, 1));
         HIP_TRY(hipMalloc(&r.d_cnt, sizeof(uint2) * nu));
         HIP_TRY(hipMalloc(&r.d_F, sizeof(float) * nu));
         HIP_TRY(hipMalloc(&r.d_big, sizeof(uint32_t) * nu));
+        HIP_TRY(hipMalloc(&r.d_replay, sizeof(uint32_t) * nu));
         r.cap_units = size_t(units);
     }
     const size_t nsc = scores ? size_t(units) * size_t(n * (n - 1)) : 0;
@@ -367,7 +373,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     a.nsets = 1;
     // walk grid: persistent, batches of about six grabs per wave
     int per_cu = 0;
-    HIP_TRY(rb_occupancy(scores ? 1 : 0, 0, &per_cu));
+    HIP_TRY(rb_walk_occupancy(scores, n, &per_cu));
     per_cu = std::max(per_cu, 1);
     if (ctx->shape.waves_per_cu > 0) per_cu = std::min(per_cu, int(ctx->shape.waves_per_cu));
     const uint64_t max_grid = uint64_t(ctx->num_cu) * uint64_t(per_cu);
@@ -410,6 +416,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     if (!s.rb_join) HIP_TRY(hipEventCreateWithFlags(&s.rb_join, hipEventDisableTiming));
     HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
     HIP_TRY(hipMemsetAsync(r.d_big_count, 0, sizeof(uint32_t), s.stream));
+    HIP_TRY(hipMemsetAsync(r.d_replay_count, 0, sizeof(uint32_t), s.stream));
     HIP_TRY(hipMemcpyAsync(r.d_rbm, r.h_rbm, sizeof(RbModel), hipMemcpyHostToDevice, s.stream));
     const RbModel* d_rbm = static_cast<const RbModel*>(r.d_rbm);
     HIP_TRY(hipEventRecord(r.ev0, s.stream));
@@ -421,9 +428,9 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     HIP_TRY(launch_rb_memo(scores, n, d_rbm, r.d_pmf_tab, ML.d_list, ML.d_listD, ML.nvec,
                            r.d_poolP, r.d_poolD, s.rb_aux));
     HIP_TRY(hipEventRecord(s.rb_join, s.rb_aux));
-    HIP_TRY(launch_rb_walk(scores, grid, ctx->d_bursts, ctx->d_ticks, ctx->d_links, d_blob,
-                           d_rbm, a, r.d_cnt, r.d_F, r.d_S, r.d_work, r.d_big, r.d_big_count,
-                           s.stream));
+    HIP_TRY(launch_rb_walk(scores, n, grid, uint32_t(ctx->num_cu) * 4u, ctx->d_bursts, ctx->d_ticks,
+                           ctx->d_links, d_blob, d_rbm, a, r.d_cnt, r.d_F, r.d_S, r.d_work,
+                           r.d_big, r.d_big_count, r.d_replay, r.d_replay_count, s.stream));
     HIP_TRY(hipEventRecord(r.ev_walk, s.stream));
     HIP_TRY(hipStreamWaitEvent(s.stream, s.rb_join, 0));
     HIP_TRY(launch_rb_unit(0, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, nullptr, nullptr, nullptr,

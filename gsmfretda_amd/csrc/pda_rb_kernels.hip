@@ -5,7 +5,9 @@
 //                        but no acceptor draw; per unit it stores the photon
 //                        counts per state, F, and (for rate gradients) the
 //                        score of the walk's draws, and lists the units whose
-//                        acceptor total is too large for the memo
+//                        acceptor total is too large for the memo; fast form,
+//                        the rare units beyond it listed for
+//   rb_replay_kernel     the general form over those units
 //   rb_table_kernel      the binomial pmfs of every state, counts < 256
 //   rb_memo_kernel       the pmf (a convolution of one binomial per state)
 //                        of every count vector with a small acceptor total,
@@ -63,17 +65,99 @@ constexpr int kRbJacRow = kRbJacCols + 1;
 
 constexpr int kRbFwd = 0, kRbGrad = 1, kRbJac = 2;
 
-template <bool kScores>
-__global__ __launch_bounds__(kWave) void rb_walk_kernel(
+// What an RB walk accumulates over a unit's windows, in registers (kN
+// states, known at compile time, so every index below is a constant): the
+// photon counts per state and the score of the walk's draws per rate pair,
+//   d/dK(j', s) log[P(j | s) K(j, s) exp(-K(j, s) tau)]
+//     = [j' = j] (2 / K(j, s) - tau) - 1 / sum_{j''} K(j'', s)
+// summed over the windows in float, in the order of the oracle's
+// restatement: per window, the taken pair's 2 / K - tau first, then
+// 1 / sum K off every pair leaving the window's state.
+template <int kN, bool kScores>
+struct RbAcc {
+    static constexpr int kK = kN * (kN - 1);
+    uint32_t cnt[kN];
+    float S[kScores ? kK : 1];
+
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int s = 0; s < kN; ++s) cnt[s] = 0u;
+#pragma unroll
+        for (int i = 0; i < (kScores ? kK : 1); ++i) S[i] = 0.0f;
+    }
+
+    __device__ __forceinline__ void absorb(const RbModel& R, const WindowInfo& wi) {
+        const uint32_t m = wi.m > 0 ? uint32_t(wi.m) : 0u;
+#pragma unroll
+        for (int s = 0; s < kN; ++s) cnt[s] += s == wi.s ? m : 0u;
+        if (kScores) {
+            const int p = rate_pair(kN, wi.s, wi.j);
+            const float v = R.two_over_k[wi.s][wi.j] - wi.tau;
+            const float inv = R.inv_sum[wi.s];
+#pragma unroll
+            for (int i = 0; i < kK; ++i) {
+                // pair i = rate_pair(kN, from, to): to = i / (kN - 1)
+                const int to = i / (kN - 1), r = i % (kN - 1);
+                const int from = r < to ? r : r + 1;
+                const float t = i == p ? S[i] + v : S[i];
+                S[i] = from == wi.s ? t - inv : S[i];
+            }
+        }
+    }
+
+    // Stores the unit: its packed counts, F and scores; a unit too long for
+    // the memo joins the big list (one atomic per wave and call site: the
+    // lanes storing now, ranked among themselves).
+    __device__ __forceinline__ void store(uint32_t unit, float F, uint32_t lane,
+                                          uint2* __restrict__ cnt_out, float* __restrict__ F_out,
+                                          float* __restrict__ S_out, uint32_t* __restrict__ big,
+                                          uint32_t* __restrict__ big_count) const {
+        uint32_t c[kRbMaxStates] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int s = 0; s < kN; ++s) c[s] = cnt[s];
+        uint2 packed;  // rb_pack of the four counts (states >= n hold 0)
+        packed.x = (c[0] & 0xFFFFu) | (c[1] << 16);
+        packed.y = (c[2] & 0xFFFFu) | (c[3] << 16);
+        cnt_out[unit] = packed;
+        F_out[unit] = F;
+        if (kScores) {
+            // kK (2, 6 or 12) floats per unit: 8-byte aligned rows, two floats
+            // a store
+            float2* row = reinterpret_cast<float2*>(S_out + size_t(unit) * size_t(kK));
+#pragma unroll
+            for (int i = 0; i < kK / 2; ++i) row[i] = make_float2(S[2 * i], S[2 * i + 1]);
+        }
+        const uint32_t mt = c[0] + c[1] + c[2] + c[3];
+        const bool longer = mt >= uint32_t(kRbMemoCap);
+        const uint64_t mk = __ballot(longer);
+        if (mk) {
+            const int leader = __ffsll(static_cast<long long>(mk)) - 1;
+            uint32_t base = 0;
+            if (int(lane) == leader) base = atomicAdd(big_count, uint32_t(__popcll(mk)));
+            base = __shfl(base, leader);
+            if (longer)
+                big[base + __builtin_amdgcn_mbcnt_hi(uint32_t(mk >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(uint32_t(mk), 0u))] = unit;
+        }
+    }
+};
+
+// The walk of every unit in the fast form (32-bit window ends, background
+// means <= 64: the plain walk's common case) with the work stealing of
+// walk_kernel; a unit that leaves the common case is listed for
+// rb_replay_kernel, which walks it again from its start in the general form
+// with the same keyed draws, so every unit's stored summary is the general
+// form's.
+template <bool kScores, int kN>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(7))) void rb_walk_kernel(
     const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
     const uint2* __restrict__ links, const ModelBlob* __restrict__ blob,
     const RbModel* __restrict__ rbm, const RunArgs a, uint2* __restrict__ cnt_out,
     float* __restrict__ F_out, float* __restrict__ S_out, uint32_t* __restrict__ work,
-    uint32_t* __restrict__ big, uint32_t* __restrict__ big_count) {
+    uint32_t* __restrict__ big, uint32_t* __restrict__ big_count, uint32_t* __restrict__ replay,
+    uint32_t* __restrict__ replay_count) {
     __shared__ Model sM;
     __shared__ RbModel sR;
-    __shared__ uint32_t s_cnt[kRbMaxStates][kWave];
-    __shared__ float s_S[kRbMaxPairs][kWave];
     const uint32_t lane = threadIdx.x;
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(&blob->m);
@@ -84,42 +168,18 @@ __global__ __launch_bounds__(kWave) void rb_walk_kernel(
         for (uint32_t i = lane; i < sizeof(RbModel) / 4; i += kWave) dst2[i] = src2[i];
     }
     __syncthreads();
-    const int n = sR.n, K = sR.K;
+    RbAcc<kN, kScores> acc;
 
-    // the work-stealing loop of walk_kernel (general form, no replays)
     const uint32_t queue = blockIdx.x % a.queues;
     const uint32_t q_begin = uint32_t(uint64_t(a.nunits) * queue / a.queues);
     const uint32_t q_end = uint32_t(uint64_t(a.nunits) * (queue + 1u) / a.queues);
     uint32_t pool_next = 0, pool_end = 0;
     bool busy = false;
     uint32_t u = 0;
-    WalkState<false> st{};
+    WalkState<true> st{};
     UnitKey key{a.seed, a.set, 0u, 0u};
     BurstDesc d{0u, 0u, 0u, 0.0f};
     bool drained = false;
-    auto store = [&](uint32_t unit) {
-        uint2 packed;  // rb_pack of the four counts (states >= n hold 0)
-        packed.x = (s_cnt[0][lane] & 0xFFFFu) | (s_cnt[1][lane] << 16);
-        packed.y = (s_cnt[2][lane] & 0xFFFFu) | (s_cnt[3][lane] << 16);
-        cnt_out[unit] = packed;
-        F_out[unit] = st.F;
-        if (kScores)
-            for (int i = 0; i < K; ++i) S_out[size_t(unit) * size_t(K) + i] = s_S[i][lane];
-        // units too long for the memo join the list, one atomic per wave
-        // and call site: the lanes storing now, ranked among themselves
-        const uint32_t mt = s_cnt[0][lane] + s_cnt[1][lane] + s_cnt[2][lane] + s_cnt[3][lane];
-        const bool longer = mt >= uint32_t(kRbMemoCap);
-        const uint64_t m = __ballot(longer);
-        if (m) {
-            const int leader = __ffsll(static_cast<long long>(m)) - 1;
-            uint32_t base = 0;
-            if (int(lane) == leader) base = atomicAdd(big_count, uint32_t(__popcll(m)));
-            base = __shfl(base, leader);
-            if (longer)
-                big[base + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u))] = unit;
-        }
-    };
     while (true) {
         bool need = !busy;
         while (!drained) {
@@ -146,14 +206,12 @@ __global__ __launch_bounds__(kWave) void rb_walk_kernel(
                 d = bursts[b];
                 key.r = u - bl * a.R;
                 key.b = b;
-                for (int s = 0; s < kRbMaxStates; ++s) s_cnt[s][lane] = 0u;
-                if (kScores)
-                    for (int i = 0; i < K; ++i) s_S[i][lane] = 0.0f;
-                if (begin_unit<false>(sM, key, d.P, d.T, st) == kStepMore) {
+                acc.clear();
+                if (begin_unit<true>(sM, key, d.P, d.T, st) == kStepMore) {
                     busy = true;
                     need = false;
                 } else {
-                    store(u);
+                    acc.store(u, st.F, lane, cnt_out, F_out, S_out, big, big_count);
                 }
             }
             const uint32_t taken = uint32_t(__popcll(want));
@@ -162,20 +220,68 @@ __global__ __launch_bounds__(kWave) void rb_walk_kernel(
         if (!__any(busy)) break;
         if (busy) {
             WindowInfo wi;
-            const int r = step_window<false, true>(sM, nullptr, key, d.P, d.stop_rel, d.T,
-                                                   ticks + d.off, links + d.off, a, st, &wi);
-            if (wi.m > 0) s_cnt[wi.s][lane] += uint32_t(wi.m);
-            if (kScores) {
-                // d/dK(j', s) log[P(j | s) K(j, s) exp(-K(j, s) tau)]
-                //   = [j' = j] (2 / K(j, s) - tau) - 1 / sum_{j''} K(j'', s)
-                s_S[sR.pair[wi.s][wi.j]][lane] += sR.two_over_k[wi.s][wi.j] - wi.tau;
-                for (int j2 = 0; j2 < n; ++j2)
-                    if (j2 != wi.s) s_S[sR.pair[wi.s][j2]][lane] -= sR.inv_sum[wi.s];
-            }
-            if (r != kStepMore) {
-                store(u);
+            const int r = step_window<true, true>(sM, nullptr, key, d.P, d.stop_rel, d.T,
+                                                  ticks + d.off, links + d.off, a, st, &wi);
+            if (r == kStepReplay) {
+                replay[atomicAdd(replay_count, 1u)] = u;
                 busy = false;
+            } else {
+                acc.absorb(sR, wi);
+                if (r != kStepMore) {
+                    acc.store(u, st.F, lane, cnt_out, F_out, S_out, big, big_count);
+                    busy = false;
+                }
             }
+        }
+    }
+}
+
+// The units rb_walk_kernel listed, in the general form, one unit per lane
+// over the list; workgroups beyond the list's need exit at once.  Launched
+// after rb_walk_kernel on the same stream.
+template <bool kScores, int kN>
+__global__ __launch_bounds__(kWave) void rb_replay_kernel(
+    const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
+    const uint2* __restrict__ links, const ModelBlob* __restrict__ blob,
+    const RbModel* __restrict__ rbm, const RunArgs a, uint2* __restrict__ cnt_out,
+    float* __restrict__ F_out, float* __restrict__ S_out, uint32_t* __restrict__ big,
+    uint32_t* __restrict__ big_count, const uint32_t* __restrict__ replay,
+    const uint32_t* __restrict__ replay_count) {
+    const uint32_t count = *replay_count;
+    if (blockIdx.x * kWave >= count) return;  // uniform over the workgroup
+    __shared__ Model sM;
+    __shared__ RbModel sR;
+    const uint32_t lane = threadIdx.x;
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&blob->m);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&sM);
+        for (uint32_t i = lane; i < sizeof(Model) / 4; i += kWave) dst[i] = src[i];
+        const uint32_t* src2 = reinterpret_cast<const uint32_t*>(rbm);
+        uint32_t* dst2 = reinterpret_cast<uint32_t*>(&sR);
+        for (uint32_t i = lane; i < sizeof(RbModel) / 4; i += kWave) dst2[i] = src2[i];
+    }
+    __syncthreads();
+    const uint32_t first = blockIdx.x * kWave;
+    const uint32_t stride = gridDim.x * kWave;
+    for (uint32_t base = first; base < count; base += stride) {
+        const uint32_t i = base + lane;
+        if (i < count) {
+            const uint32_t u = replay[i];
+            const uint32_t bl = u / a.R;
+            const uint32_t b = a.begin + bl;
+            const BurstDesc d = bursts[b];
+            const UnitKey key{a.seed, a.set, u - bl * a.R, b};
+            WalkState<false> st{};
+            RbAcc<kN, kScores> acc;
+            acc.clear();
+            int r = begin_unit<false>(sM, key, d.P, d.T, st);
+            while (r == kStepMore) {
+                WindowInfo wi;
+                r = step_window<false, true>(sM, nullptr, key, d.P, d.stop_rel, d.T,
+                                             ticks + d.off, links + d.off, a, st, &wi);
+                acc.absorb(sR, wi);
+            }
+            acc.store(u, st.F, lane, cnt_out, F_out, S_out, big, big_count);
         }
     }
 }
@@ -938,12 +1044,18 @@ __global__ __launch_bounds__(kWave) void rb_finish_kernel(
         atomicExch(&work[q * kCounterStride], 0u);
 }
 
+hipError_t rb_walk_occupancy(bool scores, int n, int* per_cu) {
+#define PDA_RB_OCC(SC, N) \
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_walk_kernel<SC, N>, kWave, 0)
+    if (scores) return n == 2 ? PDA_RB_OCC(true, 2) : n == 3 ? PDA_RB_OCC(true, 3) : PDA_RB_OCC(true, 4);
+    return n == 2 ? PDA_RB_OCC(false, 2) : n == 3 ? PDA_RB_OCC(false, 3) : PDA_RB_OCC(false, 4);
+#undef PDA_RB_OCC
+}
+
 hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu) {
     const size_t l = size_t(dyn_lds);
     const int tu = kWave * kRbUnitWaves;
     switch (which) {
-        case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_walk_kernel<false>, kWave, 0);
-        case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_walk_kernel<true>, kWave, 0);
         case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kRbFwd>, tu, l);
         case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kRbGrad>, tu, l);
         case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kRbJac>, tu, l);
@@ -953,16 +1065,39 @@ hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu) {
     }
 }
 
-hipError_t launch_rb_walk(bool scores, uint32_t grid, const BurstDesc* bursts, const uint32_t* ticks,
-                          const uint2* links, const ModelBlob* blob, const RbModel* rbm,
-                          const RunArgs& a, uint2* cnt, float* F, float* S, uint32_t* work,
-                          uint32_t* big, uint32_t* big_count, hipStream_t stream) {
-    if (scores)
-        hipLaunchKernelGGL(rb_walk_kernel<true>, dim3(grid), dim3(kWave), 0, stream, bursts, ticks,
-                           links, blob, rbm, a, cnt, F, S, work, big, big_count);
-    else
-        hipLaunchKernelGGL(rb_walk_kernel<false>, dim3(grid), dim3(kWave), 0, stream, bursts,
-                           ticks, links, blob, rbm, a, cnt, F, S, work, big, big_count);
+template <bool kScores, int kN>
+static void launch_rb_walk_n(uint32_t grid, uint32_t replay_grid, const BurstDesc* bursts,
+                             const uint32_t* ticks, const uint2* links, const ModelBlob* blob,
+                             const RbModel* rbm, const RunArgs& a, uint2* cnt, float* F, float* S,
+                             uint32_t* work, uint32_t* big, uint32_t* big_count, uint32_t* replay,
+                             uint32_t* replay_count, hipStream_t stream) {
+    hipLaunchKernelGGL((rb_walk_kernel<kScores, kN>), dim3(grid), dim3(kWave), 0, stream, bursts,
+                       ticks, links, blob, rbm, a, cnt, F, S, work, big, big_count, replay,
+                       replay_count);
+    hipLaunchKernelGGL((rb_replay_kernel<kScores, kN>), dim3(replay_grid), dim3(kWave), 0, stream,
+                       bursts, ticks, links, blob, rbm, a, cnt, F, S, big, big_count, replay,
+                       replay_count);
+}
+
+hipError_t launch_rb_walk(bool scores, int n, uint32_t grid, uint32_t replay_grid,
+                          const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
+                          const ModelBlob* blob, const RbModel* rbm, const RunArgs& a, uint2* cnt,
+                          float* F, float* S, uint32_t* work, uint32_t* big, uint32_t* big_count,
+                          uint32_t* replay, uint32_t* replay_count, hipStream_t stream) {
+#define PDA_RB_WALK(SC, N)                                                                    \
+    launch_rb_walk_n<SC, N>(grid, replay_grid, bursts, ticks, links, blob, rbm, a, cnt, F, S,  \
+                            work, big, big_count, replay, replay_count, stream)
+    if (n < 2 || n > kRbMaxStates) return hipErrorInvalidValue;
+    if (scores) {
+        if (n == 2) PDA_RB_WALK(true, 2);
+        else if (n == 3) PDA_RB_WALK(true, 3);
+        else PDA_RB_WALK(true, 4);
+    } else {
+        if (n == 2) PDA_RB_WALK(false, 2);
+        else if (n == 3) PDA_RB_WALK(false, 3);
+        else PDA_RB_WALK(false, 4);
+    }
+#undef PDA_RB_WALK
     return hipGetLastError();
 }
 

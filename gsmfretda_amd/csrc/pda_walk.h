@@ -293,7 +293,6 @@ PDA_HD int step_window(const Model& M, const typename NoDeduce<Tab>::type* tab, 
                        uint32_t P,
                        uint32_t stop_rel, float T, const uint32_t* tick, const uint2* link,
                        const RunArgs& a, WalkState<kFast>& st, WindowInfo* wi = nullptr) {
-    static_assert(!(kFast && kCounts), "the counts form walks in the general form");
     using Tick = typename WalkState<kFast>::Tick;
     const int n = M.n;
     const int s = st.s;

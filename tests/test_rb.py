@@ -166,6 +166,39 @@ def test_rb_adjoint_matches_fp64_oracle(gpu_mc, oracle):
     assert np.all(grad[9:] == 0.0)
 
 
+def test_rb_units_beyond_the_fast_walk(gpu_mc, oracle):
+    """A 30 kHz donor / 15 kHz acceptor background: windows whose background
+    means exceed 64 leave the RB walk's fast form, and the general-form replay
+    pass walks those units again.  The forward equals the oracle's bit for
+    bit, the adjoint its FP64 gradient, and the plain walk on the same units
+    abandons at least 1 % of them (the same windows)."""
+    d = _data(400, 57)
+    d.bg_dd, d.bg_ad = 3e4, 1.5e4
+    obs = _load(gpu_mc, d)
+    R, seed = 16, 12
+    M, fixed, _, _ = _rb(gpu_mc, 3, ARGS3, R, seed, set_id=2)
+    od = oracle.Data.from_bursts(d)
+    p = oracle.make_params(3, np.asarray(ARGS3, np.float32), *PH)
+    want = oracle.rb_run(od, p, R, seed, 2)
+    assert np.array_equal(fixed, want), np.nonzero(fixed != want)
+    oh = _obs_hist(obs)
+    g = _chi2_cotangent(M, oh, R, 3)
+    grad = gpu_mc.rb_adjoint(g, 3)
+    Mo, go = oracle.rb_grad(od, p, R, seed, g, set_id=2)
+    # the rate components carry the replayed units' scores: per component;
+    # the efficiency components come from float leave-one-out pmf
+    # differences, and under this chi2's cotangent (~1e9) the middle state's
+    # cancels to 5 % of the largest: against the largest efficiency component
+    rate_scale = np.abs(go[3:9]).max()
+    assert np.all(np.abs(grad[3:9] - go[3:9]) <= 1e-3 * np.maximum(np.abs(go[3:9]), 1e-3 * rate_scale)), (grad, go)
+    assert np.all(np.abs(grad[:3] - go[:3]) <= 1e-3 * np.abs(go[:3]).max()), (grad, go)
+    gpu_mc.reSampleTimes = R
+    gpu_mc.run_kernel(sid=0, seed=seed, set_id=2)
+    gpu_mc.synchronize(0)
+    units, replays = gpu_mc.run_stats(0)
+    assert units == d.nbursts * R and replays >= units // 100, (units, replays)
+
+
 def test_rb_at_200_bins(gpu_mc, oracle):
     """The reference's default of 200 E bins: forward bit-exact, adjoint
     within 1e-3 of the FP64 oracle, Jacobian equal to the adjoint."""
