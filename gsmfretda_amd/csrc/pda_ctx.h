@@ -135,8 +135,7 @@ struct RbState {
     // states): every count vector's pmf, and (with scores) its pmfs with
     // one trial of each counted state removed
     float* d_poolP = nullptr;
-    float* d_poolD = nullptr;
-    size_t cap_poolP = 0, cap_poolD = 0;
+    size_t cap_poolP = 0;
     bool sbar_valid = false;           // d_Sbar holds the last forward's means
     double* h_grad = nullptr;          // pinned
     hipEvent_t ev0 = nullptr, ev_walk = nullptr, ev1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;
@@ -197,14 +196,15 @@ struct Slot {
 namespace pda {
 // Parameter-independent layout of the RB pmf memo for n states: the count
 // vectors with sum < kRbMemoCap, listed by total (so the lanes of a wave
-// build pmfs of one length), and per vector its offsets in the pools.
+// build pmfs of one length), and per vector its offset in the pool and the
+// offsets of its leave-one-out pmfs (the pmfs of the vectors with one trial
+// of the 1st..4th counted state removed, which the pool holds too).
 struct RbMemoLayout {
     uint32_t nvec = 0;
-    uint64_t poolP = 0, poolD = 0;  // floats
-    uint4* d_list = nullptr;        // list order: counts (x, y), offP, 0
-    uint4* d_listD = nullptr;       // list order: offD of the 1st..4th counted state
+    uint64_t poolP = 0;               // floats
+    uint4* d_list = nullptr;          // list order: counts (x, y), offP, 0
     uint32_t* d_rank_offP = nullptr;  // by rb_rank
-    uint4* d_rank_offD = nullptr;     // by rb_rank
+    uint4* d_rank_offD = nullptr;     // by rb_rank: leave-one-out offsets in the pool
 };
 }  // namespace pda
 
@@ -299,16 +299,15 @@ hipError_t launch_rb_walk(bool scores, int n, uint32_t grid, uint32_t replay_gri
 int rb_unit_threads();
 size_t rb_unit_lds_bytes(int B, int mode);
 size_t rb_big_lds_bytes(int B, int cap, int mode);
-size_t rb_memo_lds_bytes(int n, bool deriv);
+size_t rb_memo_lds_bytes(int n);
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
                            hipStream_t stream);
-hipError_t launch_rb_memo(bool deriv, int n, const RbModel* rbm, const float* pmf_tab,
-                          const uint4* list, const uint4* listD, uint32_t nvec, float* poolP,
-                          float* poolD, hipStream_t stream);
+hipError_t launch_rb_memo(int n, const RbModel* rbm, const float* pmf_tab, const uint4* list,
+                          uint32_t nvec, float* poolP, hipStream_t stream);
 hipError_t launch_rb_unit(int mode, uint32_t grid, const RbModel* rbm, const RunArgs& a,
                           const uint2* cnt, const float* F, const float* S, const double* Sbar,
                           const double* g, const uint32_t* rank_offP, const uint4* rank_offD,
-                          const float* poolP, const float* poolD, unsigned long long* hist_acc,
+                          const float* poolP, unsigned long long* hist_acc,
                           double* grad, uint32_t* work, hipStream_t stream);
 hipError_t launch_rb_big(int mode, uint32_t grid, int cap, const RbModel* rbm, const double* lf,
                          const float* pmf_tab, const RunArgs& a, const uint2* cnt, const float* F,

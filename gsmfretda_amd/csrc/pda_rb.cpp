@@ -50,7 +50,6 @@ void free_rb(RbState& r) {
     (void)hipFree(r.d_replay_count);
     (void)hipFree(r.d_Sbar);
     (void)hipFree(r.d_poolP);
-    (void)hipFree(r.d_poolD);
     (void)hipHostFree(r.h_grad);
     for (hipEvent_t e : {r.ev0, r.ev_walk, r.ev1, r.ev_a0, r.ev_a1})
         if (e) (void)hipEventDestroy(e);
@@ -60,7 +59,6 @@ void free_rb(RbState& r) {
 void free_rb_memo(pda_ctx* ctx) {
     for (RbMemoLayout& L : ctx->rb_memo) {
         (void)hipFree(L.d_list);
-        (void)hipFree(L.d_listD);
         (void)hipFree(L.d_rank_offP);
         (void)hipFree(L.d_rank_offD);
         L = RbMemoLayout();
@@ -160,8 +158,9 @@ void build_rb_model(const Model& M, RbModel& R, bool* rates_positive, bool* p_in
 // 0, 1, ... (rb_rank); the build list holds them by total, and the pools
 // follow the list, so that the blocks written by one wave of rb_memo_kernel
 // (64 consecutive list entries) are adjacent: P blocks of Mt + 1 floats in
-// list order; per wave, for each state s in order, the leave-one-out blocks
-// (Mt floats) of the wave's vectors that count state s, in list order.
+// list order.  A vector's leave-one-out pmfs are the blocks of the vectors
+// with one trial fewer of each counted state (Mt - 1 < kRbMemoCap: in the
+// pool too); their offsets are kept per vector.
 int ensure_memo_layout(pda_ctx* ctx, int n) {
     RbMemoLayout& L = ctx->rb_memo[n];
     if (L.d_list) return PDA_OK;
@@ -199,10 +198,10 @@ int ensure_memo_layout(pda_ctx* ctx, int n) {
             return fail(PDA_E_STATE, "RB memo: rank of a count vector disagrees with the enumeration");
     std::stable_sort(v.begin(), v.end(), [](const Vec& x, const Vec& y) { return x.Mt < y.Mt; });
     const size_t nv = v.size();
-    std::vector<uint4> list(nv), listD(nv, make_uint4(0u, 0u, 0u, 0u));
+    std::vector<uint4> list(nv);
     std::vector<uint32_t> rank_offP(nv);
     std::vector<uint4> rank_offD(nv, make_uint4(0u, 0u, 0u, 0u));
-    uint64_t offP = 0, offD = 0;
+    uint64_t offP = 0;
     for (size_t i = 0; i < nv; ++i) {
         uint32_t cnt[kRbMaxStates];
         for (int s = 0; s < kRbMaxStates; ++s) cnt[s] = uint32_t(v[i].m[s]);
@@ -211,22 +210,21 @@ int ensure_memo_layout(pda_ctx* ctx, int n) {
         rank_offP[v[i].rank] = uint32_t(offP);
         offP += uint64_t(v[i].Mt) + 1u;
     }
-    constexpr size_t kWaveVecs = kWave;  // list entries per wave of rb_memo_kernel
-    for (size_t w0 = 0; w0 < nv; w0 += kWaveVecs) {
-        const size_t w1 = std::min(nv, w0 + kWaveVecs);
-        for (int s = 0; s < n; ++s)
-            for (size_t i = w0; i < w1; ++i) {
-                if (v[i].m[s] == 0) continue;
-                int j = 0;  // slot: counted states before s
-                for (int t = 0; t < s; ++t) j += v[i].m[t] > 0 ? 1 : 0;
-                uint32_t* d = &listD[i].x;
-                d[j] = uint32_t(offD);
-                offD += uint64_t(v[i].Mt);
-            }
-        for (size_t i = w0; i < w1; ++i) rank_offD[v[i].rank] = listD[i];
+    // the pmf with one trial of counted state s removed is the pmf of the
+    // count vector with m_s - 1 (the same convolutions, one trial fewer):
+    // the derivative passes read it from the pool at that vector's offset
+    for (const Vec& e : v) {
+        uint32_t* d = &rank_offD[e.rank].x;
+        int j = 0;  // slot: counted states before s
+        for (int s = 0; s < n; ++s) {
+            if (e.m[s] == 0) continue;
+            int mm[kRbMaxStates];
+            for (int t = 0; t < kRbMaxStates; ++t) mm[t] = e.m[t];
+            mm[s] -= 1;
+            d[j++] = rank_offP[rb_rank(mm, n, T)];
+        }
     }
-    if (offP >= (uint64_t(1) << 32) || offD >= (uint64_t(1) << 32))
-        return fail(PDA_E_STATE, "RB memo pools exceed 2^32 entries");
+    if (offP >= (uint64_t(1) << 32)) return fail(PDA_E_STATE, "RB memo pool exceeds 2^32 entries");
     auto upload = [&](auto** dst, const auto& src) -> int {
         using T = std::remove_reference_t<decltype(src[0])>;
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), sizeof(T) * src.size()));
@@ -235,19 +233,16 @@ int ensure_memo_layout(pda_ctx* ctx, int n) {
     };
     RbMemoLayout out;
     int rc = upload(&out.d_list, list);
-    if (!rc) rc = upload(&out.d_listD, listD);
     if (!rc) rc = upload(&out.d_rank_offP, rank_offP);
     if (!rc) rc = upload(&out.d_rank_offD, rank_offD);
     if (rc) {
         (void)hipFree(out.d_list);
-        (void)hipFree(out.d_listD);
         (void)hipFree(out.d_rank_offP);
         (void)hipFree(out.d_rank_offD);
         return rc;
     }
     out.nvec = uint32_t(nv);
     out.poolP = offP;
-    out.poolD = offD;
     L = out;
     return PDA_OK;
 }
@@ -389,10 +384,10 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     if (int rc = pass_shape(ctx, 0, a.B, cap, a.nunits, ps)) return rc;
     RunArgs la = a;
     la.batch = ps.batch;
-    // the memo of this set: pmfs, and with scores the leave-one-out pmfs
+    // the memo of this set: every count vector's pmf
     if (int rc = ensure_memo_layout(ctx, n)) return rc;
     const RbMemoLayout& ML = ctx->rb_memo[n];
-    if (rb_memo_lds_bytes(n, scores) + 512 > ctx->lds_per_block)
+    if (rb_memo_lds_bytes(n) + 512 > ctx->lds_per_block)
         return fail(PDA_E_STATE, "the RB memo kernel exceeds the LDS of a workgroup");
     if (ML.poolP > r.cap_poolP) {
         HIP_TRY(hipStreamSynchronize(s.stream));
@@ -401,14 +396,6 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
         r.cap_poolP = 0;
         HIP_TRY(hipMalloc(&r.d_poolP, sizeof(float) * ML.poolP));
         r.cap_poolP = ML.poolP;
-    }
-    if (scores && ML.poolD > r.cap_poolD) {
-        HIP_TRY(hipStreamSynchronize(s.stream));
-        (void)hipFree(r.d_poolD);
-        r.d_poolD = nullptr;
-        r.cap_poolD = 0;
-        HIP_TRY(hipMalloc(&r.d_poolD, sizeof(float) * ML.poolD));
-        r.cap_poolD = ML.poolD;
     }
 
     if (!s.rb_aux) HIP_TRY(hipStreamCreateWithFlags(&s.rb_aux, hipStreamNonBlocking));
@@ -425,8 +412,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     HIP_TRY(hipEventRecord(s.rb_fork, s.stream));
     HIP_TRY(hipStreamWaitEvent(s.rb_aux, s.rb_fork, 0));
     HIP_TRY(launch_rb_table(d_rbm, ctx->d_lf, r.d_pmf_tab, n, s.rb_aux));
-    HIP_TRY(launch_rb_memo(scores, n, d_rbm, r.d_pmf_tab, ML.d_list, ML.d_listD, ML.nvec,
-                           r.d_poolP, r.d_poolD, s.rb_aux));
+    HIP_TRY(launch_rb_memo(n, d_rbm, r.d_pmf_tab, ML.d_list, ML.nvec, r.d_poolP, s.rb_aux));
     HIP_TRY(hipEventRecord(s.rb_join, s.rb_aux));
     HIP_TRY(launch_rb_walk(scores, n, grid, uint32_t(ctx->num_cu) * 4u, ctx->d_bursts, ctx->d_ticks,
                            ctx->d_links, d_blob, d_rbm, a, r.d_cnt, r.d_F, r.d_S, r.d_work,
@@ -434,7 +420,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     HIP_TRY(hipEventRecord(r.ev_walk, s.stream));
     HIP_TRY(hipStreamWaitEvent(s.stream, s.rb_join, 0));
     HIP_TRY(launch_rb_unit(0, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, nullptr, nullptr, nullptr,
-                           ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, r.d_poolD, r.d_hist,
+                           ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, r.d_hist,
                            nullptr, r.d_work, s.stream));
     HIP_TRY(launch_rb_big(0, ps.big_grid, cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt, r.d_F,
                           nullptr, nullptr, nullptr, r.d_big, r.d_big_count, r.d_hist, nullptr,
@@ -550,7 +536,7 @@ int enqueue_derivative(pda_ctx* ctx, Slot& s, RbState& r, int mode, double* out)
         r.sbar_valid = true;
     }
     HIP_TRY(launch_rb_unit(mode, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, r.d_S, r.d_Sbar, g,
-                           ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, r.d_poolD, nullptr, out,
+                           ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, nullptr, out,
                            r.d_work, s.stream));
     HIP_TRY(launch_rb_big(mode, ps.big_grid, r.cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt,
                           r.d_F, r.d_S, r.d_Sbar, g, r.d_big, r.d_big_count, nullptr, out,

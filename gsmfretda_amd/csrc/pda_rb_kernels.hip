@@ -12,7 +12,8 @@
 //   rb_memo_kernel       the pmf (a convolution of one binomial per state)
 //                        of every count vector with a small acceptor total,
 //                        one lane per vector, in a strip of LDS of its own,
-//                        and (for derivatives) its leave-one-trial-out pmfs
+//                        (the derivative passes' leave-one-trial-out pmfs
+//                        are the pmfs of the vectors with one trial fewer)
 //   rb_unit_kernel       every unit with a small total: its pmf from the
 //                        memo spread into the E histogram as 2^-32 fixed
 //                        point (forward), or the cotangent-weighted pmf and
@@ -362,9 +363,11 @@ __device__ __forceinline__ void wave_windows(int len, int pos, int total, int la
 // per state s with m_s photons,
 //   d h / d p_s = m_s sum_c P_s(c) (G(bin(c + 1)) - G(bin(c)))
 // where P_s is the pmf with one trial of state s removed,
-// P(a) = q_s P_s(a) + p_s P_s(a - 1).  P_s follows from P by a first-order
-// recurrence, upwards when p_s <= 1/2 and downwards otherwise (the direction
-// in which it damps rounding errors).  The rate gradient is the likelihood
+// P(a) = q_s P_s(a) + p_s P_s(a - 1): for a memo unit the memo's pmf of its
+// count vector with m_s - 1 (built by the same convolutions); rb_big_kernel
+// gets it from P by a first-order recurrence, upwards when p_s <= 1/2 and
+// downwards otherwise (the direction in which it damps rounding errors).
+// The rate gradient is the likelihood
 // ratio R / (R - 1) sum_units h (S - Sbar), S the unit's score and Sbar the
 // mean score of its burst's R resamples: the per-burst leave-one-out
 // baseline, sum_r h_r S_r - (sum_r h_r)(sum_r S_r) / R, summed per unit.
@@ -375,25 +378,21 @@ __device__ __forceinline__ void wave_windows(int len, int pos, int total, int la
 // boundaries; the rate columns add P(a) (S - Sbar) to the row of bin(a).
 
 // Bytes of rb_memo_kernel's dynamic LDS: the n states' pmf tables, then per
-// lane one strip (the pmf) or, building the leave-one-out pmfs, two.
-__host__ __device__ inline size_t rb_memo_lds(int n, bool deriv) {
-    return sizeof(float) * (size_t(n) * kRbMemoTab +
-                            size_t(deriv ? 2 : 1) * kRbMemoWaves * kWave * kRbStride);
+// lane one strip (the pmf).
+__host__ __device__ inline size_t rb_memo_lds(int n) {
+    return sizeof(float) * (size_t(n) * kRbMemoTab + size_t(kRbMemoWaves) * kWave * kRbStride);
 }
 
 // The memo: for each count vector of the list (one lane each) its pmf P
-// into poolP at its offP, and (kDeriv) for its j-th state with a nonzero
-// count s, P_s(c), c < Mt, rounded to float into poolD at its j-th offD.
-// The list runs by total, the offsets follow it, and each wave copies its
-// lanes' blocks out of LDS together: adjacent blocks, coalesced writes.
-template <bool kDeriv>
+// into poolP at its offP.  (The derivative passes' leave-one-out pmfs are
+// the pmfs of the vectors with one trial fewer, in the same pool.)  The list
+// runs by total, the offsets follow it, and each wave copies its lanes'
+// blocks out of LDS together: adjacent blocks, coalesced writes.
 __global__ __launch_bounds__(kWave * kRbMemoWaves) void rb_memo_kernel(
     const RbModel* __restrict__ rbm, const float* __restrict__ pmf_tab,
-    const uint4* __restrict__ list, const uint4* __restrict__ listD, uint32_t nvec,
-    float* __restrict__ poolP, float* __restrict__ poolD) {
+    const uint4* __restrict__ list, uint32_t nvec, float* __restrict__ poolP) {
     extern __shared__ float s_f[];
     __shared__ RbModel sR;
-    __shared__ double s_alpha[kRbMaxStates], s_inv[kRbMaxStates];
     __shared__ int s_slot[kRbMemoWaves][kWave];
     __shared__ int s_pos[kRbMemoWaves][kWave];
     __shared__ uint32_t s_dst[kRbMemoWaves][kWave];
@@ -410,17 +409,9 @@ __global__ __launch_bounds__(kWave * kRbMemoWaves) void rb_memo_kernel(
         const int s = i / kRbMemoTab;
         tab[i] = pmf_tab[size_t(s) * kRbPmfStride + size_t(i - s * kRbMemoTab)];
     }
-    if (kDeriv && tid < n) {
-        // the recurrence P_s(c) = alpha P_s(c -/+ 1) + P(.) inv
-        const double p = double(sR.p[tid]), q = 1.0 - p;
-        const bool up = p <= 0.5;
-        s_alpha[tid] = up ? -p / q : -q / p;
-        s_inv[tid] = up ? 1.0 / q : 1.0 / p;
-    }
     __syncthreads();
     float* strips = tab + n * kRbMemoTab + size_t(wave) * kWave * kRbStride;
     float* X = strips + lane * kRbStride;
-    float* W = X + size_t(kRbMemoWaves) * kWave * kRbStride;  // kDeriv
     const uint32_t v = blockIdx.x * blockDim.x + uint32_t(tid);
     const bool act = v < nvec;
     const uint4 e = act ? list[v] : make_uint4(0u, 0u, 0u, 0u);
@@ -452,44 +443,18 @@ __global__ __launch_bounds__(kWave * kRbMemoWaves) void rb_memo_kernel(
     int* spos = s_pos[wave];
     uint32_t* sdst = s_dst[wave];
     // copy the wave's pmfs out: lane l's block of xl floats at offP
-    auto copy_out = [&](int len, uint32_t dst, int which, float* pool) {
-        const int incl = wave_scan_sum(len, lane);
-        const int pos = incl - len;
-        const int total = __shfl(incl, kWave - 1);
-        spos[lane] = pos;
-        sdst[lane] = dst;
-        wave_windows(len, pos, total, lane, slot, [&](int j, int own) {
-            if (j < total) {
-                const int ai = j - spos[own];
-                const float* src = strips + own * kRbStride +
-                                   (which ? size_t(kRbMemoWaves) * kWave * kRbStride : 0u);
-                pool[sdst[own] + uint32_t(ai)] = src[ai];
-            }
-        });
-    };
-    copy_out(act ? xl : 0, e.z, 0, poolP);
-    if (kDeriv) {
-        const uint4 od = act ? listD[v] : make_uint4(0u, 0u, 0u, 0u);
-        const int Mt = xl - 1;
-        int j = 0;  // the lane's counted states so far
-#pragma unroll
-        for (int s = 0; s < kRbMaxStates; ++s) {
-            // every lane takes part in each copy (wave-uniform loop)
-            const int m = (act && s < n) ? int(rb_count(c, s)) : 0;
-            if (m > 0) {
-                const bool up = sR.p[s] <= 0.5f;
-                const double al = s_alpha[s], inv = s_inv[s];
-                double x = 0.0;
-                for (int i = 0; i < Mt; ++i) {
-                    x = al * x + double(X[up ? i : Mt - i]) * inv;  // P_s(up ? i : Mt - 1 - i)
-                    W[up ? i : Mt - 1 - i] = float(x);
-                }
-            }
-            const uint32_t dst = j == 0 ? od.x : j == 1 ? od.y : j == 2 ? od.z : od.w;
-            copy_out(m > 0 ? Mt : 0, dst, 1, poolD);
-            j += m > 0 ? 1 : 0;
+    const int len = act ? xl : 0;
+    const int incl = wave_scan_sum(len, lane);
+    const int pos = incl - len;
+    const int total = __shfl(incl, kWave - 1);
+    spos[lane] = pos;
+    sdst[lane] = e.z;
+    wave_windows(len, pos, total, lane, slot, [&](int j, int own) {
+        if (j < total) {
+            const int ai = j - spos[own];
+            poolP[sdst[own] + uint32_t(ai)] = strips[own * kRbStride + ai];
         }
-    }
+    });
 }
 
 // Owner of position j of the wave's concatenation: the last lane l with
@@ -554,8 +519,8 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
     const float* __restrict__ F_in, const float* __restrict__ S_in,
     const double* __restrict__ Sbar, const double* __restrict__ g,
     const uint32_t* __restrict__ rank_offP, const uint4* __restrict__ rank_offD,
-    const float* __restrict__ poolP, const float* __restrict__ poolD,
-    unsigned long long* __restrict__ hist_acc, double* __restrict__ grad,
+    const float* __restrict__ poolP, unsigned long long* __restrict__ hist_acc,
+    double* __restrict__ grad,
     uint32_t* __restrict__ work) {
     constexpr bool kDeriv = kMode != kRbFwd;
     extern __shared__ __align__(16) double s_unit[];
@@ -657,7 +622,7 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
                     for (int s = 0; s < kRbMaxStates; ++s) {
                         const bool counted = rb_count(ci, s) > 0u;
                         const uint32_t base = js == 0 ? d0 : js == 1 ? d1 : js == 2 ? d2 : d3;
-                        e.pd[s] = inner && counted ? poolD[base + uint32_t(e.ai)] : 0.0f;
+                        e.pd[s] = inner && counted ? poolP[base + uint32_t(e.ai)] : 0.0f;
                         js += counted ? 1 : 0;
                     }
                 }
@@ -1104,7 +1069,7 @@ hipError_t launch_rb_walk(bool scores, int n, uint32_t grid, uint32_t replay_gri
 int rb_unit_threads() { return kWave * kRbUnitWaves; }
 size_t rb_unit_lds_bytes(int B, int mode) { return rb_unit_lds(B, mode); }
 size_t rb_big_lds_bytes(int B, int cap, int mode) { return rb_big_lds(B, cap, mode); }
-size_t rb_memo_lds_bytes(int n, bool deriv) { return rb_memo_lds(n, deriv); }
+size_t rb_memo_lds_bytes(int n) { return rb_memo_lds(n); }
 
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
                            hipStream_t stream) {
@@ -1114,37 +1079,31 @@ hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab,
     return hipGetLastError();
 }
 
-hipError_t launch_rb_memo(bool deriv, int n, const RbModel* rbm, const float* pmf_tab,
-                          const uint4* list, const uint4* listD, uint32_t nvec, float* poolP,
-                          float* poolD, hipStream_t stream) {
+hipError_t launch_rb_memo(int n, const RbModel* rbm, const float* pmf_tab, const uint4* list,
+                          uint32_t nvec, float* poolP, hipStream_t stream) {
     const uint32_t threads = kWave * kRbMemoWaves;
     const dim3 grid((nvec + threads - 1) / threads);
-    const size_t lds = rb_memo_lds(n, deriv);
-    if (deriv)
-        hipLaunchKernelGGL(rb_memo_kernel<true>, grid, dim3(threads), lds, stream, rbm, pmf_tab,
-                           list, listD, nvec, poolP, poolD);
-    else
-        hipLaunchKernelGGL(rb_memo_kernel<false>, grid, dim3(threads), lds, stream, rbm, pmf_tab,
-                           list, listD, nvec, poolP, poolD);
+    hipLaunchKernelGGL(rb_memo_kernel, grid, dim3(threads), rb_memo_lds(n), stream, rbm, pmf_tab,
+                       list, nvec, poolP);
     return hipGetLastError();
 }
 
 hipError_t launch_rb_unit(int mode, uint32_t grid, const RbModel* rbm, const RunArgs& a,
                           const uint2* cnt, const float* F, const float* S, const double* Sbar,
                           const double* g, const uint32_t* rank_offP, const uint4* rank_offD,
-                          const float* poolP, const float* poolD, unsigned long long* hist_acc,
-                          double* grad, uint32_t* work, hipStream_t stream) {
+                          const float* poolP, unsigned long long* hist_acc, double* grad,
+                          uint32_t* work, hipStream_t stream) {
     const size_t lds = rb_unit_lds(a.B, mode);
     const dim3 block(uint32_t(kWave * kRbUnitWaves));
     if (mode == kRbGrad)
         hipLaunchKernelGGL(rb_unit_kernel<kRbGrad>, dim3(grid), block, lds, stream, rbm, a, cnt, F,
-                           S, Sbar, g, rank_offP, rank_offD, poolP, poolD, hist_acc, grad, work);
+                           S, Sbar, g, rank_offP, rank_offD, poolP, hist_acc, grad, work);
     else if (mode == kRbJac)
         hipLaunchKernelGGL(rb_unit_kernel<kRbJac>, dim3(grid), block, lds, stream, rbm, a, cnt, F,
-                           S, Sbar, g, rank_offP, rank_offD, poolP, poolD, hist_acc, grad, work);
+                           S, Sbar, g, rank_offP, rank_offD, poolP, hist_acc, grad, work);
     else
         hipLaunchKernelGGL(rb_unit_kernel<kRbFwd>, dim3(grid), block, lds, stream, rbm, a, cnt, F,
-                           S, Sbar, g, rank_offP, rank_offD, poolP, poolD, hist_acc, grad, work);
+                           S, Sbar, g, rank_offP, rank_offD, poolP, hist_acc, grad, work);
     return hipGetLastError();
 }
 
