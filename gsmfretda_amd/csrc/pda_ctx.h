@@ -284,8 +284,10 @@ hipError_t microbench_shape(int op, uint32_t* grid, uint32_t* threads);
 hipError_t launch_microbench(int op, uint32_t grid, int iters, float param, const double* qn,
                              const int32_t* bound, uint64_t* out, hipStream_t stream);
 struct RbModel;
-// which: 2/3/4 rb_unit_kernel forward, adjoint, Jacobian; 5/6/7 rb_big_kernel
+// which: 5/6/7 rb_big_kernel forward, adjoint, Jacobian
 hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu);
+// rb_unit_kernel of n states (2..4), mode 0 forward, 1 adjoint, 2 Jacobian
+hipError_t rb_unit_occupancy(int mode, int n, int dyn_lds, int* per_cu);
 // the RB walk of n states (2..4), with or without scores
 hipError_t rb_walk_occupancy(bool scores, int n, int* per_cu);
 // the fast-form walk and, behind it, the general-form replay of the units it
@@ -304,7 +306,7 @@ hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab,
                            hipStream_t stream);
 hipError_t launch_rb_memo(int n, const RbModel* rbm, const float* pmf_tab, const uint4* list,
                           uint32_t nvec, float* poolP, hipStream_t stream);
-hipError_t launch_rb_unit(int mode, uint32_t grid, const RbModel* rbm, const RunArgs& a,
+hipError_t launch_rb_unit(int mode, int n, uint32_t grid, const RbModel* rbm, const RunArgs& a,
                           const uint2* cnt, const float* F, const float* S, const double* Sbar,
                           const double* g, const uint32_t* rank_offP, const uint4* rank_offD,
                           const float* poolP, unsigned long long* hist_acc,

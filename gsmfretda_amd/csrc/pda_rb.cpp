@@ -254,14 +254,14 @@ int ensure_memo_layout(pda_ctx* ctx, int n) {
 struct PassShape {
     uint32_t unit_grid = 1, batch = kWave, big_grid = 1;
 };
-int pass_shape(pda_ctx* ctx, int mode, int B, int cap, uint32_t nunits, PassShape& ps) {
+int pass_shape(pda_ctx* ctx, int mode, int n, int B, int cap, uint32_t nunits, PassShape& ps) {
     static const char* names[] = {"forward", "adjoint", "Jacobian"};
     constexpr size_t kStatic = sizeof(RbModel) + 64;  // the kernels' static LDS
     const size_t lds = rb_unit_lds_bytes(B, mode);
     if (lds + kStatic > ctx->lds_per_block)
         return fail(PDA_E_DATA, "%d bins exceed the LDS of the RB %s pass", B, names[mode]);
     int per = 0;
-    HIP_TRY(rb_occupancy(2 + mode, int(lds), &per));
+    HIP_TRY(rb_unit_occupancy(mode, n, int(lds), &per));
     const uint64_t threads = uint64_t(rb_unit_threads());
     const uint64_t g = std::min<uint64_t>(uint64_t(ctx->num_cu) * uint64_t(std::max(per, 1)),
                                           (uint64_t(nunits) + threads - 1) / threads);
@@ -381,7 +381,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     a.batch = uint32_t(batch);
     a.queues = std::min<uint32_t>(ctx->shape.queues ? ctx->shape.queues : kDefaultQueues, grid);
     PassShape ps;
-    if (int rc = pass_shape(ctx, 0, a.B, cap, a.nunits, ps)) return rc;
+    if (int rc = pass_shape(ctx, 0, n, a.B, cap, a.nunits, ps)) return rc;
     RunArgs la = a;
     la.batch = ps.batch;
     // the memo of this set: every count vector's pmf
@@ -419,7 +419,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
                            r.d_big, r.d_big_count, r.d_replay, r.d_replay_count, s.stream));
     HIP_TRY(hipEventRecord(r.ev_walk, s.stream));
     HIP_TRY(hipStreamWaitEvent(s.stream, s.rb_join, 0));
-    HIP_TRY(launch_rb_unit(0, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, nullptr, nullptr, nullptr,
+    HIP_TRY(launch_rb_unit(0, n, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, nullptr, nullptr, nullptr,
                            ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, r.d_hist,
                            nullptr, r.d_work, s.stream));
     HIP_TRY(launch_rb_big(0, ps.big_grid, cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt, r.d_F,
@@ -523,7 +523,7 @@ int enqueue_derivative(pda_ctx* ctx, Slot& s, RbState& r, int mode, double* out)
     a.left_pad = r.left_pad;
     a.nstates = r.n;
     PassShape ps;
-    if (int rc = pass_shape(ctx, mode, r.B, r.cap, a.nunits, ps)) return rc;
+    if (int rc = pass_shape(ctx, mode, r.n, r.B, r.cap, a.nunits, ps)) return rc;
     RunArgs la = a;
     la.batch = ps.batch;
     const RbModel* d_rbm = static_cast<const RbModel*>(r.d_rbm);
@@ -535,7 +535,7 @@ int enqueue_derivative(pda_ctx* ctx, Slot& s, RbState& r, int mode, double* out)
         HIP_TRY(launch_rb_score_mean(a, r.K, r.d_S, r.d_Sbar, ctx->num_cu, s.stream));
         r.sbar_valid = true;
     }
-    HIP_TRY(launch_rb_unit(mode, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, r.d_S, r.d_Sbar, g,
+    HIP_TRY(launch_rb_unit(mode, r.n, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, r.d_S, r.d_Sbar, g,
                            ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, nullptr, out,
                            r.d_work, s.stream));
     HIP_TRY(launch_rb_big(mode, ps.big_grid, r.cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt,

@@ -512,7 +512,7 @@ struct RbElem {
 // shuffles; the pmf entries come from the memo, each window's loads issued
 // while the previous window is processed.  No barrier after the prologue:
 // each wave adds to the workgroup's LDS accumulators with atomics.
-template <int kMode>
+template <int kMode, int kN>
 __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_per_eu(
     kMode == kRbFwd ? 1 : 4))) void rb_unit_kernel(
     const RbModel* __restrict__ rbm, const RunArgs a, const uint2* __restrict__ cnt_in,
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
         else s_h[i] = 0ull;
     }
     __syncthreads();
-    const int n = sR.n, K = sR.K;
+    constexpr int n = kN, K = kN * (kN - 1);  // states and rate pairs of the set
     double* w_x = s_unit + head + wave * rb_unit_wave_doubles(kMode);
 
     const uint32_t nq = gridDim.x < kAuxQueues ? gridDim.x : kAuxQueues;
@@ -549,8 +549,8 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
     const uint32_t q_begin = uint32_t(uint64_t(a.nunits) * queue / nq);
     const uint32_t q_len = uint32_t(uint64_t(a.nunits) * (queue + 1u) / nq) - q_begin;
     uint32_t* counter = &work[kRbUnitCounter + queue * kCounterStride];
-    double gp[kRbMaxStates] = {};  // adjoint: d h / d p_s over the lane's elements
-    double gk[kRbMaxPairs] = {};   // adjoint: G P (S - Sbar) over the lane's elements
+    double gp[kN] = {};  // adjoint: d h / d p_s over the lane's elements
+    double gk[K] = {};   // adjoint: G P (S - Sbar) over the lane's elements
     while (true) {
         uint32_t off = 0;
         if (lane == 0) off = atomicAdd(counter, a.batch);
@@ -563,10 +563,10 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
             const bool valid = o < end;
             const uint32_t u = q_begin + (valid ? o : r0);
             const uint2 c = valid ? cnt_in[u] : make_uint2(0u, 0u);
-            int mm[kRbMaxStates];
+            int mm[kRbMaxStates] = {0, 0, 0, 0};
             int Mt = 0;
 #pragma unroll
-            for (int s = 0; s < kRbMaxStates; ++s) {
+            for (int s = 0; s < kN; ++s) {
                 mm[s] = int(rb_count(c, s));
                 Mt += mm[s];
             }
@@ -588,8 +588,8 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
             if (kDeriv) {
                 const uint32_t bl = u / a.R;
 #pragma unroll
-                for (int k = 0; k < kRbMaxPairs; ++k)
-                    w_sc[k] = memo && k < K ? double(S_in[size_t(u) * size_t(K) + size_t(k)]) -
+                for (int k = 0; k < K; ++k)
+                    w_sc[k] = memo ? double(S_in[size_t(u) * size_t(K) + size_t(k)]) -
                                                   Sbar[size_t(bl) * size_t(K) + size_t(k)]
                                             : 0.0;
             }
@@ -614,12 +614,12 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
                     const uint32_t d2 = __shfl(offD.z, e.own), d3 = __shfl(offD.w, e.own);
                     int Mi = 0;
 #pragma unroll
-                    for (int s = 0; s < kRbMaxStates; ++s) Mi += int(rb_count(ci, s));
+                    for (int s = 0; s < kN; ++s) Mi += int(rb_count(ci, s));
                     e.Mi = Mi;
                     const bool inner = e.act && e.ai < Mi;
                     int js = 0;
 #pragma unroll
-                    for (int s = 0; s < kRbMaxStates; ++s) {
+                    for (int s = 0; s < kN; ++s) {
                         const bool counted = rb_count(ci, s) > 0u;
                         const uint32_t base = js == 0 ? d0 : js == 1 ? d1 : js == 2 ? d2 : d3;
                         e.pd[s] = inner && counted ? poolP[base + uint32_t(e.ai)] : 0.0f;
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
                         const double G1 = b1 >= 0 ? s_d[b1] : 0.0;
                         const double dG = double(float(G1 - G0));
 #pragma unroll
-                        for (int s = 0; s < kRbMaxStates; ++s)
+                        for (int s = 0; s < kN; ++s)
                             gp[s] += double(rb_count(ci, s)) * double(e.pd[s]) * dG;
                     }
                     // h (S - Sbar) of the unit, one element at a time (h is
@@ -654,20 +654,20 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
                         const double hv = G0 * double(e.P);
                         const double* scu = w_x + e.own * kRbMaxPairs;
 #pragma unroll
-                        for (int k = 0; k < kRbMaxPairs; ++k)
-                            if (k < K) gk[k] += hv * scu[k];
+                        for (int k = 0; k < K; ++k)
+                            gk[k] += hv * scu[k];
                     }
                 } else {
                     if (b0 >= 0) {
                         double* row = s_d + b0 * kRbJacRow + kRbMaxStates;
                         const double* scu = w_x + e.own * kRbMaxPairs;
 #pragma unroll
-                        for (int k = 0; k < kRbMaxPairs; ++k)
-                            if (k < K) atomicAdd(&row[k], double(e.P) * scu[k]);
+                        for (int k = 0; k < K; ++k)
+                            atomicAdd(&row[k], double(e.P) * scu[k]);
                     }
                     if (inner && b0 != b1) {
 #pragma unroll
-                        for (int s = 0; s < kRbMaxStates; ++s) {
+                        for (int s = 0; s < kN; ++s) {
                             const uint32_t m = rb_count(ci, s);
                             if (m > 0u) {
                                 const double w = double(m) * double(e.pd[s]);
@@ -698,14 +698,14 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
     } else if (kMode == kRbGrad) {
         const double f = a.R > 1 ? double(a.R) / double(a.R - 1) : 0.0;
 #pragma unroll
-        for (int s = 0; s < kRbMaxStates; ++s) {
+        for (int s = 0; s < kN; ++s) {
             const double v = wave_sum(gp[s]);
             if (lane == 0 && s < n) atomicAdd(&grad[s], v);
         }
 #pragma unroll
-        for (int k = 0; k < kRbMaxPairs; ++k) {
+        for (int k = 0; k < K; ++k) {
             const double v = wave_sum(gk[k]);
-            if (lane == 0 && k < K) atomicAdd(&grad[n + k], f * v);
+            if (lane == 0) atomicAdd(&grad[n + k], f * v);
         }
     } else {
         // grad holds J[b][i], i < n + K (efficiencies, then rates in args
@@ -1017,13 +1017,24 @@ hipError_t rb_walk_occupancy(bool scores, int n, int* per_cu) {
 #undef PDA_RB_OCC
 }
 
+template <int kMode>
+static hipError_t rb_unit_occupancy_m(int n, size_t l, int* per_cu) {
+    const int tu = kWave * kRbUnitWaves;
+    if (n == 2) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kMode, 2>, tu, l);
+    if (n == 3) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kMode, 3>, tu, l);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kMode, 4>, tu, l);
+}
+
+hipError_t rb_unit_occupancy(int mode, int n, int dyn_lds, int* per_cu) {
+    const size_t l = size_t(dyn_lds);
+    if (mode == kRbGrad) return rb_unit_occupancy_m<kRbGrad>(n, l, per_cu);
+    if (mode == kRbJac) return rb_unit_occupancy_m<kRbJac>(n, l, per_cu);
+    return rb_unit_occupancy_m<kRbFwd>(n, l, per_cu);
+}
+
 hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu) {
     const size_t l = size_t(dyn_lds);
-    const int tu = kWave * kRbUnitWaves;
     switch (which) {
-        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kRbFwd>, tu, l);
-        case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kRbGrad>, tu, l);
-        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_unit_kernel<kRbJac>, tu, l);
         case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbFwd>, kWave, l);
         case 6: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbGrad>, kWave, l);
         default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbJac>, kWave, l);
@@ -1088,22 +1099,40 @@ hipError_t launch_rb_memo(int n, const RbModel* rbm, const float* pmf_tab, const
     return hipGetLastError();
 }
 
-hipError_t launch_rb_unit(int mode, uint32_t grid, const RbModel* rbm, const RunArgs& a,
+template <int kMode, int kN>
+static void launch_rb_unit_n(uint32_t grid, size_t lds, const RbModel* rbm, const RunArgs& a,
+                             const uint2* cnt, const float* F, const float* S, const double* Sbar,
+                             const double* g, const uint32_t* rank_offP, const uint4* rank_offD,
+                             const float* poolP, unsigned long long* hist_acc, double* grad,
+                             uint32_t* work, hipStream_t stream) {
+    hipLaunchKernelGGL((rb_unit_kernel<kMode, kN>), dim3(grid), dim3(uint32_t(kWave * kRbUnitWaves)),
+                       lds, stream, rbm, a, cnt, F, S, Sbar, g, rank_offP, rank_offD, poolP,
+                       hist_acc, grad, work);
+}
+
+hipError_t launch_rb_unit(int mode, int n, uint32_t grid, const RbModel* rbm, const RunArgs& a,
                           const uint2* cnt, const float* F, const float* S, const double* Sbar,
                           const double* g, const uint32_t* rank_offP, const uint4* rank_offD,
                           const float* poolP, unsigned long long* hist_acc, double* grad,
                           uint32_t* work, hipStream_t stream) {
+    if (n < 2 || n > kRbMaxStates) return hipErrorInvalidValue;
     const size_t lds = rb_unit_lds(a.B, mode);
-    const dim3 block(uint32_t(kWave * kRbUnitWaves));
-    if (mode == kRbGrad)
-        hipLaunchKernelGGL(rb_unit_kernel<kRbGrad>, dim3(grid), block, lds, stream, rbm, a, cnt, F,
-                           S, Sbar, g, rank_offP, rank_offD, poolP, hist_acc, grad, work);
-    else if (mode == kRbJac)
-        hipLaunchKernelGGL(rb_unit_kernel<kRbJac>, dim3(grid), block, lds, stream, rbm, a, cnt, F,
-                           S, Sbar, g, rank_offP, rank_offD, poolP, hist_acc, grad, work);
-    else
-        hipLaunchKernelGGL(rb_unit_kernel<kRbFwd>, dim3(grid), block, lds, stream, rbm, a, cnt, F,
-                           S, Sbar, g, rank_offP, rank_offD, poolP, hist_acc, grad, work);
+#define PDA_RB_UNIT(M, N)                                                                      \
+    launch_rb_unit_n<M, N>(grid, lds, rbm, a, cnt, F, S, Sbar, g, rank_offP, rank_offD, poolP, \
+                           hist_acc, grad, work, stream)
+#define PDA_RB_UNIT_M(M)            \
+    if (n == 2) PDA_RB_UNIT(M, 2);  \
+    else if (n == 3) PDA_RB_UNIT(M, 3); \
+    else PDA_RB_UNIT(M, 4)
+    if (mode == kRbGrad) {
+        PDA_RB_UNIT_M(kRbGrad);
+    } else if (mode == kRbJac) {
+        PDA_RB_UNIT_M(kRbJac);
+    } else {
+        PDA_RB_UNIT_M(kRbFwd);
+    }
+#undef PDA_RB_UNIT_M
+#undef PDA_RB_UNIT
     return hipGetLastError();
 }
 
