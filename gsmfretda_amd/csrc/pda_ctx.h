@@ -121,9 +121,11 @@ struct RbState {
     float* d_pmf_tab = nullptr;        // binomial pmfs of the states, counts < 256
     double* d_jac = nullptr;           // [kMaxBins][n + K] rows of 16
     double* h_jac = nullptr;           // pinned
-    // units whose acceptor total is too large for the per-lane pass, listed
-    // by the walk (capacity cap_units), and their count
+    // units whose acceptor total is too large for the memo, listed by the
+    // walk: totals below kRbBigSplit (d_big) and the rest (d_huge), each of
+    // capacity cap_units, with their counts d_big_count[0] and [1]
     uint32_t* d_big = nullptr;
+    uint32_t* d_huge = nullptr;
     uint32_t* d_big_count = nullptr;
     // units the fast-form walk abandoned (capacity cap_units), and their
     // count, for the general-form replay
@@ -146,6 +148,7 @@ struct RbState {
     uint32_t seed = 0, set = 0;
     int init_state = 0;
     bool rates_positive = false, p_interior = false;
+    bool wide = false;  // a state with 0.4 < p < 0.6: long units' derivatives in double
 };
 
 struct Slot {
@@ -284,8 +287,9 @@ hipError_t microbench_shape(int op, uint32_t* grid, uint32_t* threads);
 hipError_t launch_microbench(int op, uint32_t grid, int iters, float param, const double* qn,
                              const int32_t* bound, uint64_t* out, hipStream_t stream);
 struct RbModel;
-// which: 5/6/7 rb_big_kernel forward, adjoint, Jacobian
-hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu);
+// rb_big_kernel, mode 0 forward, 1 adjoint, 2 Jacobian; wide: derivative
+// passes in double
+hipError_t rb_big_occupancy(int mode, bool wide, int dyn_lds, int* per_cu);
 // rb_unit_kernel of n states (2..4), mode 0 forward, 1 adjoint, 2 Jacobian
 hipError_t rb_unit_occupancy(int mode, int n, int dyn_lds, int* per_cu);
 // the RB walk of n states (2..4), with or without scores
@@ -295,12 +299,13 @@ hipError_t rb_walk_occupancy(bool scores, int n, int* per_cu);
 hipError_t launch_rb_walk(bool scores, int n, uint32_t grid, uint32_t replay_grid,
                           const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
                           const ModelBlob* blob, const RbModel* rbm, const RunArgs& a, uint2* cnt,
-                          float* F, float* S, uint32_t* work, uint32_t* big, uint32_t* big_count,
-                          uint32_t* replay, uint32_t* replay_count, hipStream_t stream);
+                          float* F, float* S, uint32_t* work, uint32_t* big, uint32_t* huge,
+                          uint32_t* big_count, uint32_t* replay, uint32_t* replay_count,
+                          hipStream_t stream);
 // dynamic LDS bytes (mode: 0 forward, 1 adjoint, 2 Jacobian)
 int rb_unit_threads();
 size_t rb_unit_lds_bytes(int B, int mode);
-size_t rb_big_lds_bytes(int B, int cap, int mode);
+size_t rb_big_lds_bytes(int B, int cap, int mode, bool wide);
 size_t rb_memo_lds_bytes(int n);
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
                            hipStream_t stream);
@@ -311,7 +316,8 @@ hipError_t launch_rb_unit(int mode, int n, uint32_t grid, const RbModel* rbm, co
                           const double* g, const uint32_t* rank_offP, const uint4* rank_offD,
                           const float* poolP, unsigned long long* hist_acc,
                           double* grad, uint32_t* work, hipStream_t stream);
-hipError_t launch_rb_big(int mode, uint32_t grid, int cap, const RbModel* rbm, const double* lf,
+hipError_t launch_rb_big(int mode, bool wide, uint32_t grid, int cap, const RbModel* rbm,
+                         const double* lf,
                          const float* pmf_tab, const RunArgs& a, const uint2* cnt, const float* F,
                          const float* S, const double* Sbar, const double* g,
                          const uint32_t* big, const uint32_t* big_count,

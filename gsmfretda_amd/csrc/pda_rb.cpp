@@ -45,6 +45,7 @@ void free_rb(RbState& r) {
     (void)hipFree(r.d_jac);
     (void)hipHostFree(r.h_jac);
     (void)hipFree(r.d_big);
+    (void)hipFree(r.d_huge);
     (void)hipFree(r.d_big_count);
     (void)hipFree(r.d_replay);
     (void)hipFree(r.d_replay_count);
@@ -89,7 +90,7 @@ int ensure_rb(pda_ctx* ctx, RbState& r) {
         if (!r.d_hist) HIP_TRY(hipMalloc(&r.d_hist, sizeof(unsigned long long) * 8u * kMaxBins));
         HIP_TRY(hipMemset(r.d_hist, 0, sizeof(unsigned long long) * 8u * kMaxBins));
         if (!r.d_work) HIP_TRY(hipMalloc(&r.d_work, sizeof(uint32_t) * kRbCounters));
-        if (!r.d_big_count) HIP_TRY(hipMalloc(&r.d_big_count, sizeof(uint32_t)));
+        if (!r.d_big_count) HIP_TRY(hipMalloc(&r.d_big_count, 2 * sizeof(uint32_t)));
         if (!r.d_replay_count) HIP_TRY(hipMalloc(&r.d_replay_count, sizeof(uint32_t)));
         HIP_TRY(hipMemset(r.d_work, 0, sizeof(uint32_t) * kRbCounters));
         if (!r.d_res) HIP_TRY(hipMalloc(&r.d_res, kRbResBytes));
@@ -252,9 +253,12 @@ int ensure_memo_layout(pda_ctx* ctx, int n) {
 // the units), about eight grabs per wave; rb_big_kernel on every slot its
 // LDS allows (its workgroups beyond the listed units exit at once).
 struct PassShape {
-    uint32_t unit_grid = 1, batch = kWave, big_grid = 1;
+    uint32_t unit_grid = 1, batch = kWave, big_grid = 1, huge_grid = 1;
+    int big_cap = 1;    // rb_big_kernel's buffers for the big list (the huge list: the run's cap)
+    bool wide = false;  // rb_big_kernel's derivative buffers in double
 };
-int pass_shape(pda_ctx* ctx, int mode, int n, int B, int cap, uint32_t nunits, PassShape& ps) {
+int pass_shape(pda_ctx* ctx, int mode, int n, bool wide, int B, int cap, uint32_t nunits,
+               PassShape& ps) {
     static const char* names[] = {"forward", "adjoint", "Jacobian"};
     constexpr size_t kStatic = sizeof(RbModel) + 64;  // the kernels' static LDS
     const size_t lds = rb_unit_lds_bytes(B, mode);
@@ -269,12 +273,19 @@ int pass_shape(pda_ctx* ctx, int mode, int n, int B, int cap, uint32_t nunits, P
     uint64_t bt = uint64_t(nunits) / (uint64_t(ps.unit_grid) * (threads / kWave) * 8u);
     bt = (bt + kWave - 1) / kWave * kWave;
     ps.batch = uint32_t(std::min<uint64_t>(std::max<uint64_t>(bt, kWave), 4096));
-    const size_t blds = rb_big_lds_bytes(B, cap, mode);
+    // double buffers where asked for and they fit, else float
+    ps.wide = wide && rb_big_lds_bytes(B, cap, mode, true) + kStatic <= ctx->lds_per_block;
+    const size_t blds = rb_big_lds_bytes(B, cap, mode, ps.wide);
     if (blds + kStatic > ctx->lds_per_block)
         return fail(PDA_E_DATA, "%d bins and bursts needing %d acceptor outcomes exceed the LDS "
                     "of the RB %s pass", B, cap, names[mode]);
     per = 0;
-    HIP_TRY(rb_occupancy(5 + mode, int(blds), &per));
+    HIP_TRY(rb_big_occupancy(mode, ps.wide, int(blds), &per));
+    ps.huge_grid = uint32_t(std::max(1, ctx->num_cu * std::max(per, 1)));
+    ps.big_cap = std::min(cap, kRbBigSplit);
+    per = 0;
+    HIP_TRY(rb_big_occupancy(mode, ps.wide, int(rb_big_lds_bytes(B, ps.big_cap, mode, ps.wide)),
+                             &per));
     ps.big_grid = uint32_t(std::max(1, ctx->num_cu * std::max(per, 1)));
     return PDA_OK;
 }
@@ -306,7 +317,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
                     "%u photons need %d)", kRbMaxCount, maxP, bound);
     const int cap = bound + 1;  // rb_big_kernel's pmf buffers
     const bool scores = (flags & PDA_RB_SCORES) != 0;
-    if (rb_big_lds_bytes(ctx->B, cap, 0) + sizeof(RbModel) > ctx->lds_per_block)
+    if (rb_big_lds_bytes(ctx->B, cap, 0, false) + sizeof(RbModel) > ctx->lds_per_block)
         return fail(PDA_E_DATA, "bursts of %u photons exceed the LDS of the RB kernels", maxP);
 
     // buffers of the per-unit summaries and the walk's list of long units
@@ -315,16 +326,19 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
         (void)hipFree(r.d_cnt);
         (void)hipFree(r.d_F);
         (void)hipFree(r.d_big);
+        (void)hipFree(r.d_huge);
         (void)hipFree(r.d_replay);
         r.d_cnt = nullptr;
         r.d_F = nullptr;
         r.d_big = nullptr;
+        r.d_huge = nullptr;
         r.d_replay = nullptr;
         r.cap_units = 0;
         const size_t nu = size_t(std::max<int64_t>(units, 1));
         HIP_TRY(hipMalloc(&r.d_cnt, sizeof(uint2) * nu));
         HIP_TRY(hipMalloc(&r.d_F, sizeof(float) * nu));
         HIP_TRY(hipMalloc(&r.d_big, sizeof(uint32_t) * nu));
+        HIP_TRY(hipMalloc(&r.d_huge, sizeof(uint32_t) * nu));
         HIP_TRY(hipMalloc(&r.d_replay, sizeof(uint32_t) * nu));
         r.cap_units = size_t(units);
     }
@@ -351,6 +365,8 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     RbModel* hm = static_cast<RbModel*>(r.h_rbm);
     bool rates_positive = false, p_interior = false;
     build_rb_model(M, *hm, &rates_positive, &p_interior);
+    bool wide = false;  // a state whose leave-one-out recurrence barely damps errors
+    for (int st = 0; st < n; ++st) wide = wide || (hm->p[st] > 0.4f && hm->p[st] < 0.6f);
 
     RunArgs a{};
     a.begin = uint32_t(b0);
@@ -381,7 +397,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     a.batch = uint32_t(batch);
     a.queues = std::min<uint32_t>(ctx->shape.queues ? ctx->shape.queues : kDefaultQueues, grid);
     PassShape ps;
-    if (int rc = pass_shape(ctx, 0, n, a.B, cap, a.nunits, ps)) return rc;
+    if (int rc = pass_shape(ctx, 0, n, false, a.B, cap, a.nunits, ps)) return rc;
     RunArgs la = a;
     la.batch = ps.batch;
     // the memo of this set: every count vector's pmf
@@ -402,7 +418,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     if (!s.rb_fork) HIP_TRY(hipEventCreateWithFlags(&s.rb_fork, hipEventDisableTiming));
     if (!s.rb_join) HIP_TRY(hipEventCreateWithFlags(&s.rb_join, hipEventDisableTiming));
     HIP_TRY(hipMemsetAsync(r.d_work, 0, sizeof(uint32_t) * kRbCounters, s.stream));
-    HIP_TRY(hipMemsetAsync(r.d_big_count, 0, sizeof(uint32_t), s.stream));
+    HIP_TRY(hipMemsetAsync(r.d_big_count, 0, 2 * sizeof(uint32_t), s.stream));
     HIP_TRY(hipMemsetAsync(r.d_replay_count, 0, sizeof(uint32_t), s.stream));
     HIP_TRY(hipMemcpyAsync(r.d_rbm, r.h_rbm, sizeof(RbModel), hipMemcpyHostToDevice, s.stream));
     const RbModel* d_rbm = static_cast<const RbModel*>(r.d_rbm);
@@ -416,15 +432,20 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     HIP_TRY(hipEventRecord(s.rb_join, s.rb_aux));
     HIP_TRY(launch_rb_walk(scores, n, grid, uint32_t(ctx->num_cu) * 4u, ctx->d_bursts, ctx->d_ticks,
                            ctx->d_links, d_blob, d_rbm, a, r.d_cnt, r.d_F, r.d_S, r.d_work,
-                           r.d_big, r.d_big_count, r.d_replay, r.d_replay_count, s.stream));
+                           r.d_big, r.d_huge, r.d_big_count, r.d_replay, r.d_replay_count,
+                           s.stream));
     HIP_TRY(hipEventRecord(r.ev_walk, s.stream));
     HIP_TRY(hipStreamWaitEvent(s.stream, s.rb_join, 0));
     HIP_TRY(launch_rb_unit(0, n, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, nullptr, nullptr, nullptr,
                            ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, r.d_hist,
                            nullptr, r.d_work, s.stream));
-    HIP_TRY(launch_rb_big(0, ps.big_grid, cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt, r.d_F,
-                          nullptr, nullptr, nullptr, r.d_big, r.d_big_count, r.d_hist, nullptr,
-                          s.stream));
+    HIP_TRY(launch_rb_big(0, false, ps.big_grid, ps.big_cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt,
+                          r.d_F, nullptr, nullptr, nullptr, r.d_big, r.d_big_count, r.d_hist,
+                          nullptr, s.stream));
+    if (cap > kRbBigSplit)
+        HIP_TRY(launch_rb_big(0, false, ps.huge_grid, cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt,
+                              r.d_F, nullptr, nullptr, nullptr, r.d_huge, r.d_big_count + 1,
+                              r.d_hist, nullptr, s.stream));
     HIP_TRY(launch_rb_finish(a, ctx->d_obs, r.d_hist, r.d_work, r.d_res, s.stream));
     HIP_TRY(hipEventRecord(r.ev1, s.stream));
     HIP_TRY(hipMemcpyAsync(r.h_res, r.d_res, sizeof(RbResultHeaderH) + sizeof(uint64_t) * size_t(a.B),
@@ -446,6 +467,7 @@ int rb_enqueue_forward(pda_ctx* ctx, Slot& s, RbState& r, const Model& M,
     r.init_state = M.init_state;
     r.rates_positive = rates_positive;
     r.p_interior = p_interior;
+    r.wide = wide;
     return PDA_OK;
 }
 
@@ -523,7 +545,7 @@ int enqueue_derivative(pda_ctx* ctx, Slot& s, RbState& r, int mode, double* out)
     a.left_pad = r.left_pad;
     a.nstates = r.n;
     PassShape ps;
-    if (int rc = pass_shape(ctx, mode, r.n, r.B, r.cap, a.nunits, ps)) return rc;
+    if (int rc = pass_shape(ctx, mode, r.n, r.wide, r.B, r.cap, a.nunits, ps)) return rc;
     RunArgs la = a;
     la.batch = ps.batch;
     const RbModel* d_rbm = static_cast<const RbModel*>(r.d_rbm);
@@ -538,9 +560,13 @@ int enqueue_derivative(pda_ctx* ctx, Slot& s, RbState& r, int mode, double* out)
     HIP_TRY(launch_rb_unit(mode, r.n, ps.unit_grid, d_rbm, la, r.d_cnt, r.d_F, r.d_S, r.d_Sbar, g,
                            ML.d_rank_offP, ML.d_rank_offD, r.d_poolP, nullptr, out,
                            r.d_work, s.stream));
-    HIP_TRY(launch_rb_big(mode, ps.big_grid, r.cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a, r.d_cnt,
-                          r.d_F, r.d_S, r.d_Sbar, g, r.d_big, r.d_big_count, nullptr, out,
+    HIP_TRY(launch_rb_big(mode, ps.wide, ps.big_grid, ps.big_cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a,
+                          r.d_cnt, r.d_F, r.d_S, r.d_Sbar, g, r.d_big, r.d_big_count, nullptr, out,
                           s.stream));
+    if (r.cap > kRbBigSplit)
+        HIP_TRY(launch_rb_big(mode, ps.wide, ps.huge_grid, r.cap, d_rbm, ctx->d_lf, r.d_pmf_tab, a,
+                              r.d_cnt, r.d_F, r.d_S, r.d_Sbar, g, r.d_huge, r.d_big_count + 1,
+                              nullptr, out, s.stream));
     HIP_TRY(hipEventRecord(r.ev_a1, s.stream));
     return PDA_OK;
 }

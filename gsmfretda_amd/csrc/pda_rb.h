@@ -77,11 +77,12 @@ PDA_HD double rb_binom_pmf(int m, int k, const double* lf, float p, double lp, d
 
 // One output of Z = Y (*) X: sum_{k} Y[k] X[a - k], k ascending over the
 // overlap, float accumulation from 0.
-PDA_HD float rb_conv_at(const float* Y, int ylen, const float* X, int xlen, int a) {
+template <class T>
+PDA_HD T rb_conv_at(const T* Y, int ylen, const T* X, int xlen, int a) {
     int k0 = a - (xlen - 1);
     if (k0 < 0) k0 = 0;
     const int k1 = a < ylen - 1 ? a : ylen - 1;
-    float z = 0.0f;
+    T z = T(0);
     for (int k = k0; k <= k1; ++k) z = z + Y[k] * X[a - k];
     return z;
 }
@@ -107,6 +108,11 @@ PDA_HD int rb_pmf_index(int s, int m, int k) { return s * kRbPmfStride + m * (m 
 // total Mt < kRbMemoCap are computed once per forward and looked up by the
 // units; units with larger totals compute their own.
 constexpr int kRbMemoCap = 48;
+// Units that compute their own pmf are listed in two lists: totals below
+// kRbBigSplit (pmfs of at most kRbBigSplit entries: small LDS buffers, many
+// workgroups per CU) and the rest (buffers for the largest total any burst
+// of the run allows).
+constexpr int kRbBigSplit = 256;
 
 // C(x, y) for x >= 0 (0 when x < y); y <= kRbMaxStates in closed form with
 // constant divisors (each product of y consecutive integers is divisible

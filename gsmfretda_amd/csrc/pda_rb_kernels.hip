@@ -107,11 +107,13 @@ struct RbAcc {
     }
 
     // Stores the unit: its packed counts, F and scores; a unit too long for
-    // the memo joins the big list (one atomic per wave and call site: the
-    // lanes storing now, ranked among themselves).
+    // the memo joins the big list (total < kRbBigSplit, count big_count[0])
+    // or the huge one (count big_count[1]): one atomic per wave, list and
+    // call site, the lanes storing now ranked among themselves.
     __device__ __forceinline__ void store(uint32_t unit, float F, uint32_t lane,
                                           uint2* __restrict__ cnt_out, float* __restrict__ F_out,
                                           float* __restrict__ S_out, uint32_t* __restrict__ big,
+                                          uint32_t* __restrict__ huge,
                                           uint32_t* __restrict__ big_count) const {
         uint32_t c[kRbMaxStates] = {0u, 0u, 0u, 0u};
 #pragma unroll
@@ -129,16 +131,23 @@ struct RbAcc {
             for (int i = 0; i < kK / 2; ++i) row[i] = make_float2(S[2 * i], S[2 * i + 1]);
         }
         const uint32_t mt = c[0] + c[1] + c[2] + c[3];
-        const bool longer = mt >= uint32_t(kRbMemoCap);
-        const uint64_t mk = __ballot(longer);
+        append(mt >= uint32_t(kRbMemoCap) && mt < uint32_t(kRbBigSplit), unit, lane, big,
+               &big_count[0]);
+        append(mt >= uint32_t(kRbBigSplit), unit, lane, huge, &big_count[1]);
+    }
+
+    __device__ __forceinline__ static void append(bool take, uint32_t unit, uint32_t lane,
+                                                  uint32_t* __restrict__ list,
+                                                  uint32_t* __restrict__ count) {
+        const uint64_t mk = __ballot(take);
         if (mk) {
             const int leader = __ffsll(static_cast<long long>(mk)) - 1;
             uint32_t base = 0;
-            if (int(lane) == leader) base = atomicAdd(big_count, uint32_t(__popcll(mk)));
+            if (int(lane) == leader) base = atomicAdd(count, uint32_t(__popcll(mk)));
             base = __shfl(base, leader);
-            if (longer)
-                big[base + __builtin_amdgcn_mbcnt_hi(uint32_t(mk >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo(uint32_t(mk), 0u))] = unit;
+            if (take)
+                list[base + __builtin_amdgcn_mbcnt_hi(uint32_t(mk >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(uint32_t(mk), 0u))] = unit;
         }
     }
 };
@@ -155,8 +164,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(7))) void
     const uint2* __restrict__ links, const ModelBlob* __restrict__ blob,
     const RbModel* __restrict__ rbm, const RunArgs a, uint2* __restrict__ cnt_out,
     float* __restrict__ F_out, float* __restrict__ S_out, uint32_t* __restrict__ work,
-    uint32_t* __restrict__ big, uint32_t* __restrict__ big_count, uint32_t* __restrict__ replay,
-    uint32_t* __restrict__ replay_count) {
+    uint32_t* __restrict__ big, uint32_t* __restrict__ huge, uint32_t* __restrict__ big_count,
+    uint32_t* __restrict__ replay, uint32_t* __restrict__ replay_count) {
     __shared__ Model sM;
     __shared__ RbModel sR;
     const uint32_t lane = threadIdx.x;
@@ -212,7 +221,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(7))) void
                     busy = true;
                     need = false;
                 } else {
-                    acc.store(u, st.F, lane, cnt_out, F_out, S_out, big, big_count);
+                    acc.store(u, st.F, lane, cnt_out, F_out, S_out, big, huge, big_count);
                 }
             }
             const uint32_t taken = uint32_t(__popcll(want));
@@ -229,7 +238,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(7))) void
             } else {
                 acc.absorb(sR, wi);
                 if (r != kStepMore) {
-                    acc.store(u, st.F, lane, cnt_out, F_out, S_out, big, big_count);
+                    acc.store(u, st.F, lane, cnt_out, F_out, S_out, big, huge, big_count);
                     busy = false;
                 }
             }
@@ -246,8 +255,8 @@ __global__ __launch_bounds__(kWave) void rb_replay_kernel(
     const uint2* __restrict__ links, const ModelBlob* __restrict__ blob,
     const RbModel* __restrict__ rbm, const RunArgs a, uint2* __restrict__ cnt_out,
     float* __restrict__ F_out, float* __restrict__ S_out, uint32_t* __restrict__ big,
-    uint32_t* __restrict__ big_count, const uint32_t* __restrict__ replay,
-    const uint32_t* __restrict__ replay_count) {
+    uint32_t* __restrict__ huge, uint32_t* __restrict__ big_count,
+    const uint32_t* __restrict__ replay, const uint32_t* __restrict__ replay_count) {
     const uint32_t count = *replay_count;
     if (blockIdx.x * kWave >= count) return;  // uniform over the workgroup
     __shared__ Model sM;
@@ -282,7 +291,7 @@ __global__ __launch_bounds__(kWave) void rb_replay_kernel(
                                              ticks + d.off, links + d.off, a, st, &wi);
                 acc.absorb(sR, wi);
             }
-            acc.store(u, st.F, lane, cnt_out, F_out, S_out, big, big_count);
+            acc.store(u, st.F, lane, cnt_out, F_out, S_out, big, huge, big_count);
         }
     }
 }
@@ -725,14 +734,24 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
 // Bytes of rb_big_kernel's dynamic LDS: B 8-byte entries (the forward's
 // histogram, the adjoint's cotangent) or the Jacobian's [B][kRbJacCols]
 // rows, then three pmf buffers (X, Y, Z) of cap floats.
-__host__ __device__ inline size_t rb_big_lds(int B, int cap, int mode) {
+__host__ __device__ inline size_t rb_big_lds(int B, int cap, int mode, bool wide) {
     const size_t head = size_t(mode == kRbJac ? B * kRbJacCols : B);
-    return sizeof(double) * head + sizeof(float) * 3u * size_t(cap);
+    return sizeof(double) * head +
+           (mode != kRbFwd && wide ? sizeof(double) : sizeof(float)) * 3u * size_t(cap);
 }
 
 // The units the walk listed (Mt >= kRbMemoCap), one wave per unit, lanes
-// over acceptor totals; cap = the largest Mt + 1 of the run's bursts.
-template <int kMode>
+// over acceptor totals; cap = the largest Mt + 1 of the units of the list.
+// The forward convolves in float, in the oracle's order (bit-exact).  The
+// derivative passes take the leave-one-out pmfs from P by a recurrence
+// whose factor has magnitude |alpha| = p / q (or q / p): it carries P's
+// rounding errors, amplified up to 1 / (1 - |alpha|) -- 3x at p = 0.4, the
+// length of the chain at p = 1/2, where a P rounded to float at every
+// convolution step had put 1e-3 of an efficiency gradient off at
+// Mt ~ 1,000.  kWide (sets with a state of 0.4 < p < 0.6): those passes
+// convolve in double (from the same float binomial tables), at twice the
+// LDS.
+template <int kMode, bool kWide>
 __global__ __launch_bounds__(kWave) void rb_big_kernel(
     const RbModel* __restrict__ rbm, const double* __restrict__ lf,
     const float* __restrict__ pmf_tab, const RunArgs a, const uint2* __restrict__ cnt_in,
@@ -754,7 +773,8 @@ __global__ __launch_bounds__(kWave) void rb_big_kernel(
     const int head = kMode == kRbJac ? B * kRbJacCols : B;
     unsigned long long* s_h = reinterpret_cast<unsigned long long*>(s_dyn);
     double* s_d = s_dyn;
-    float* bufs = reinterpret_cast<float*>(s_dyn + head);
+    using T = typename std::conditional<kMode != kRbFwd && kWide, double, float>::type;
+    T* bufs = reinterpret_cast<T*>(s_dyn + head);
     for (int i = int(lane); i < head; i += kWave) {
         if (kMode == kRbGrad) s_d[i] = g[i];
         else if (kMode == kRbJac) s_d[i] = 0.0;
@@ -769,14 +789,17 @@ __global__ __launch_bounds__(kWave) void rb_big_kernel(
         const float F = F_in[u];
         int bX = 0, bY = 1, bZ = 2;
         int xl = 1;
-        if (lane == 0) bufs[0] = 1.0f;
+        if (lane == 0) bufs[0] = T(1);
 #pragma unroll
         for (int s = 0; s < kRbMaxStates; ++s) {
             const int m = s < n ? int(rb_count(c, s)) : 0;
-            float* Y = bufs + bY * cap;
-            for (int k = int(lane); m > 0 && k <= m; k += kWave)
-                Y[k] = m < kRbPmfTable ? pmf_tab[rb_pmf_index(s, m, k)]
-                                       : float(rb_binom_pmf(m, k, lf, sR.p[s], sR.lp[s], sR.lq[s]));
+            T* Y = bufs + bY * cap;
+            for (int k = int(lane); m > 0 && k <= m; k += kWave) {
+                const double y = m < kRbPmfTable
+                                     ? double(pmf_tab[rb_pmf_index(s, m, k)])
+                                     : rb_binom_pmf(m, k, lf, sR.p[s], sR.lp[s], sR.lq[s]);
+                Y[k] = kMode == kRbFwd ? T(float(y)) : T(y);
+            }
             __syncthreads();
             if (m > 0) {
                 if (xl == 1) {  // X = [1]: the product is Y itself, bit for bit
@@ -785,8 +808,8 @@ __global__ __launch_bounds__(kWave) void rb_big_kernel(
                     bY = t;
                     xl = m + 1;
                 } else {
-                    const float* X = bufs + bX * cap;
-                    float* Z = bufs + bZ * cap;
+                    const T* X = bufs + bX * cap;
+                    T* Z = bufs + bZ * cap;
                     for (int q = int(lane); q < xl + m; q += kWave)
                         Z[q] = rb_conv_at(Y, m + 1, X, xl, q);
                     const int t = bX;
@@ -797,7 +820,7 @@ __global__ __launch_bounds__(kWave) void rb_big_kernel(
             }
             __syncthreads();
         }
-        const float* P = bufs + bX * cap;
+        const T* P = bufs + bX * cap;
         if (kMode == kRbFwd) {
             for (int q = int(lane); q < xl; q += kWave) {
                 const int bin = rb_bin(q, F, B);
@@ -815,7 +838,7 @@ __global__ __launch_bounds__(kWave) void rb_big_kernel(
                         ? double(S_in[size_t(u) * size_t(K) + size_t(k)]) -
                               Sbar[size_t(bl) * size_t(K) + size_t(k)]
                         : 0.0;
-        float* dG = bufs + bY * cap;  // adjoint: G(bin(c + 1)) - G(bin(c))
+        T* dG = bufs + bY * cap;  // adjoint: G(bin(c + 1)) - G(bin(c)), rounded to float
         double hp = 0.0;
         for (int q = int(lane); q < xl; q += kWave) {
             const int b0 = rb_bin(q, F, B);
@@ -833,7 +856,7 @@ __global__ __launch_bounds__(kWave) void rb_big_kernel(
             hp += G0 * double(P[q]);
             if (q + 1 < xl) {
                 const int b1 = rb_bin(q + 1, F, B);
-                dG[q] = float((b1 >= 0 ? s_d[b1] : 0.0) - G0);
+                dG[q] = T(float((b1 >= 0 ? s_d[b1] : 0.0) - G0));
             }
         }
         __syncthreads();
@@ -1032,37 +1055,40 @@ hipError_t rb_unit_occupancy(int mode, int n, int dyn_lds, int* per_cu) {
     return rb_unit_occupancy_m<kRbFwd>(n, l, per_cu);
 }
 
-hipError_t rb_occupancy(int which, int dyn_lds, int* per_cu) {
+hipError_t rb_big_occupancy(int mode, bool wide, int dyn_lds, int* per_cu) {
     const size_t l = size_t(dyn_lds);
-    switch (which) {
-        case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbFwd>, kWave, l);
-        case 6: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbGrad>, kWave, l);
-        default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbJac>, kWave, l);
-    }
+    if (mode == kRbGrad)
+        return wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbGrad, true>, kWave, l)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbGrad, false>, kWave, l);
+    if (mode == kRbJac)
+        return wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbJac, true>, kWave, l)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbJac, false>, kWave, l);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rb_big_kernel<kRbFwd, false>, kWave, l);
 }
 
 template <bool kScores, int kN>
 static void launch_rb_walk_n(uint32_t grid, uint32_t replay_grid, const BurstDesc* bursts,
                              const uint32_t* ticks, const uint2* links, const ModelBlob* blob,
                              const RbModel* rbm, const RunArgs& a, uint2* cnt, float* F, float* S,
-                             uint32_t* work, uint32_t* big, uint32_t* big_count, uint32_t* replay,
-                             uint32_t* replay_count, hipStream_t stream) {
+                             uint32_t* work, uint32_t* big, uint32_t* huge, uint32_t* big_count,
+                             uint32_t* replay, uint32_t* replay_count, hipStream_t stream) {
     hipLaunchKernelGGL((rb_walk_kernel<kScores, kN>), dim3(grid), dim3(kWave), 0, stream, bursts,
-                       ticks, links, blob, rbm, a, cnt, F, S, work, big, big_count, replay,
+                       ticks, links, blob, rbm, a, cnt, F, S, work, big, huge, big_count, replay,
                        replay_count);
     hipLaunchKernelGGL((rb_replay_kernel<kScores, kN>), dim3(replay_grid), dim3(kWave), 0, stream,
-                       bursts, ticks, links, blob, rbm, a, cnt, F, S, big, big_count, replay,
+                       bursts, ticks, links, blob, rbm, a, cnt, F, S, big, huge, big_count, replay,
                        replay_count);
 }
 
 hipError_t launch_rb_walk(bool scores, int n, uint32_t grid, uint32_t replay_grid,
                           const BurstDesc* bursts, const uint32_t* ticks, const uint2* links,
                           const ModelBlob* blob, const RbModel* rbm, const RunArgs& a, uint2* cnt,
-                          float* F, float* S, uint32_t* work, uint32_t* big, uint32_t* big_count,
-                          uint32_t* replay, uint32_t* replay_count, hipStream_t stream) {
+                          float* F, float* S, uint32_t* work, uint32_t* big, uint32_t* huge,
+                          uint32_t* big_count, uint32_t* replay, uint32_t* replay_count,
+                          hipStream_t stream) {
 #define PDA_RB_WALK(SC, N)                                                                    \
     launch_rb_walk_n<SC, N>(grid, replay_grid, bursts, ticks, links, blob, rbm, a, cnt, F, S,  \
-                            work, big, big_count, replay, replay_count, stream)
+                            work, big, huge, big_count, replay, replay_count, stream)
     if (n < 2 || n > kRbMaxStates) return hipErrorInvalidValue;
     if (scores) {
         if (n == 2) PDA_RB_WALK(true, 2);
@@ -1079,7 +1105,7 @@ hipError_t launch_rb_walk(bool scores, int n, uint32_t grid, uint32_t replay_gri
 
 int rb_unit_threads() { return kWave * kRbUnitWaves; }
 size_t rb_unit_lds_bytes(int B, int mode) { return rb_unit_lds(B, mode); }
-size_t rb_big_lds_bytes(int B, int cap, int mode) { return rb_big_lds(B, cap, mode); }
+size_t rb_big_lds_bytes(int B, int cap, int mode, bool wide) { return rb_big_lds(B, cap, mode, wide); }
 size_t rb_memo_lds_bytes(int n) { return rb_memo_lds(n); }
 
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,
@@ -1136,21 +1162,35 @@ hipError_t launch_rb_unit(int mode, int n, uint32_t grid, const RbModel* rbm, co
     return hipGetLastError();
 }
 
-hipError_t launch_rb_big(int mode, uint32_t grid, int cap, const RbModel* rbm, const double* lf,
-                         const float* pmf_tab, const RunArgs& a, const uint2* cnt, const float* F,
-                         const float* S, const double* Sbar, const double* g,
-                         const uint32_t* big, const uint32_t* big_count,
+template <int kMode, bool kWide>
+static void launch_rb_big_m(uint32_t grid, int cap, const RbModel* rbm, const double* lf,
+                            const float* pmf_tab, const RunArgs& a, const uint2* cnt,
+                            const float* F, const float* S, const double* Sbar, const double* g,
+                            const uint32_t* big, const uint32_t* big_count,
+                            unsigned long long* hist_acc, double* grad, hipStream_t stream) {
+    hipLaunchKernelGGL((rb_big_kernel<kMode, kWide>), dim3(grid), dim3(kWave),
+                       rb_big_lds(a.B, cap, kMode, kWide), stream, rbm, lf, pmf_tab, a, cnt, F, S,
+                       Sbar, g, big, big_count, hist_acc, grad, cap);
+}
+
+hipError_t launch_rb_big(int mode, bool wide, uint32_t grid, int cap, const RbModel* rbm,
+                         const double* lf, const float* pmf_tab, const RunArgs& a,
+                         const uint2* cnt, const float* F, const float* S, const double* Sbar,
+                         const double* g, const uint32_t* big, const uint32_t* big_count,
                          unsigned long long* hist_acc, double* grad, hipStream_t stream) {
-    const size_t lds = rb_big_lds(a.B, cap, mode);
-    if (mode == kRbGrad)
-        hipLaunchKernelGGL(rb_big_kernel<kRbGrad>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
-                           pmf_tab, a, cnt, F, S, Sbar, g, big, big_count, hist_acc, grad, cap);
-    else if (mode == kRbJac)
-        hipLaunchKernelGGL(rb_big_kernel<kRbJac>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
-                           pmf_tab, a, cnt, F, S, Sbar, g, big, big_count, hist_acc, grad, cap);
-    else
-        hipLaunchKernelGGL(rb_big_kernel<kRbFwd>, dim3(grid), dim3(kWave), lds, stream, rbm, lf,
-                           pmf_tab, a, cnt, F, S, Sbar, g, big, big_count, hist_acc, grad, cap);
+#define PDA_RB_BIG(M, W)                                                                      \
+    launch_rb_big_m<M, W>(grid, cap, rbm, lf, pmf_tab, a, cnt, F, S, Sbar, g, big, big_count, \
+                          hist_acc, grad, stream)
+    if (mode == kRbGrad) {
+        if (wide) PDA_RB_BIG(kRbGrad, true);
+        else PDA_RB_BIG(kRbGrad, false);
+    } else if (mode == kRbJac) {
+        if (wide) PDA_RB_BIG(kRbJac, true);
+        else PDA_RB_BIG(kRbJac, false);
+    } else {
+        PDA_RB_BIG(kRbFwd, false);
+    }
+#undef PDA_RB_BIG
     return hipGetLastError();
 }
 

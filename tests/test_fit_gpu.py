@@ -18,11 +18,18 @@ def problem(gpu_mc):
 
 
 def test_lm_reaches_the_truth_chi2(problem):
+    """From the best of 16 starting sets, restarting from the next best when
+    a descent converges short of the target, as the fit study and the bench
+    run it: a single descent can settle in a local minimum of this chi2
+    (its common-random-number form is a step function at small scales), and
+    which one depends on the last bits of the Jacobian."""
     prob, truth, pop0 = problem
     c_truth = prob.chi2(truth)
-    start = pop0[int(np.argmin(prob.chi2_population(pop0)))]
+    order = np.argsort(prob.chi2_population(pop0), kind="stable")
+    start = pop0[order[0]]
     assert prob.chi2(start) > 2.0 * c_truth
-    h = fit.LMFitter(R_grad=8).run(prob, start, max_seconds=30.0, target=1.01 * c_truth)
+    h = fit.LMFitter(R_grad=8).run(prob, start, max_seconds=30.0, target=1.01 * c_truth,
+                                   restarts=[pop0[i] for i in order[1:]])
     assert h.chi2[-1] <= 1.01 * c_truth, (h.chi2[-5:], c_truth)
     assert np.all(np.diff(h.chi2) <= 0)
 

@@ -79,8 +79,10 @@ def test_rb_forward_bit_exact(gpu_mc, oracle, n, bg, rng_):
 
 @pytest.mark.parametrize("base,geom", [(200, 60), (400, 150)])
 def test_rb_forward_bit_exact_long_bursts(gpu_mc, oracle, base, geom):
-    """Long bursts: units with acceptor totals of 48 and more (listed by the
-    walk, one wave per unit) mixed with memo units, and nearly all long."""
+    """Long bursts: a window counts its first colour run only, so even bursts
+    of 400 photons keep most acceptor totals below 48; each of these data sets
+    lists one unit for rb_big_kernel (test_rb_units_of_every_size covers all
+    three kernels' units in numbers)."""
     d = synth.generate(120, synth.make_args([0.2, 0.55, 0.85], [900, 2500, 700, 3000, 1500, 5000]),
                        3, base_photons=base, mean_geom=geom, seed=61)
     _load(gpu_mc, d)
@@ -250,6 +252,40 @@ def test_rb_jacobian_is_the_adjoint_of_every_cotangent(gpu_mc, oracle, long_burs
         _, go = oracle.rb_grad(od, p, R, seed, e, set_id=2)
         sc = np.abs(go).max()
         assert np.all(np.abs(J[b, :9] - go) <= 1e-3 * np.maximum(np.abs(go), 1e-3 * sc)), (b, J[b], go)
+
+
+def test_rb_units_of_every_size(gpu_mc, oracle):
+    """Efficiencies near 0 and 1 make a window's colour run span most of its
+    photons, so acceptor totals reach the burst sizes: units from the memo
+    (Mt < 48), from the big list (48 <= Mt < 256, small pmf buffers) and from
+    the huge list (Mt >= 256, buffers for the run's largest total) in one run.
+    Forward bit-exact against the oracle; adjoint against its FP64 gradient;
+    Jacobian rows equal to the adjoint of their cotangents."""
+    E = [0.002, 0.5, 0.998]
+    args = synth.make_args(E, [300, 800, 400, 900, 500, 700])
+    d = synth.generate(60, args, 3, base_photons=200, mean_geom=200, seed=71)
+    obs = _load(gpu_mc, d)
+    R, seed = 4, 5
+    od = oracle.Data.from_bursts(d)
+    p = oracle.make_params(3, np.asarray(args, np.float32), *PH)
+    mt = np.array([int(np.sum(oracle.rb_unit(od, p, b, r, seed, 0)[0]))
+                   for b in range(d.nbursts) for r in range(R)])
+    assert (mt < 48).sum() > 20 and ((mt >= 48) & (mt < 256)).sum() > 20 and (mt >= 256).sum() > 20
+    M, fixed, _, _ = _rb(gpu_mc, 3, args, R, seed, set_id=0)
+    want = oracle.rb_run(od, p, R, seed, 0)
+    assert np.array_equal(fixed, want), np.nonzero(fixed != want)
+    g = _chi2_cotangent(M, _obs_hist(obs), R, 3)
+    grad = gpu_mc.rb_adjoint(g, 3)
+    _, go = oracle.rb_grad(od, p, R, seed, g, set_id=0)
+    scale = np.abs(go).max()
+    assert np.all(np.abs(grad[:9] - go) <= 1e-3 * np.maximum(np.abs(go), 1e-3 * scale)), (grad, go)
+    J = gpu_mc.rb_jacobian(3, len(M))
+    rng = np.random.default_rng(9)
+    for _ in range(2):
+        c = rng.normal(size=50)
+        adj = gpu_mc.rb_adjoint(c, 3)
+        sc = np.abs(adj).max()
+        assert np.all(np.abs(c @ J - adj) <= 1e-5 * np.maximum(np.abs(adj), 1e-2 * sc)), (c @ J, adj)
 
 
 def test_rb_efficiency_gradient_matches_central_differences(gpu_mc, oracle):
