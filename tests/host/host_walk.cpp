@@ -68,7 +68,12 @@ static uint64_t splitmix(uint64_t& x) {
 
 // The capped-Poisson draw as a plain inversion loop (the form of
 // background_count's single chunk, src/mc.cu:153-166), for comparison.
+// trace: the last k the loop reached and whether some pk was subnormal.
+static int32_t g_loop_k = 0;
+static bool g_loop_subnormal = false;
 static float poisson_loop(float lam, float cap, float u_first, float u_cap) {
+    g_loop_k = 0;
+    g_loop_subnormal = false;
     if (lam > 64.0f || !(cap < 0x1p30f)) return -1.0f;
     int32_t k = 0;
     bool over = false;
@@ -82,31 +87,53 @@ static float poisson_loop(float lam, float cap, float u_first, float u_cap) {
                 break;
             }
             pk = (pk * lam) / float(k);
+            if (pk != 0.0f && pk < 0x1p-126f) g_loop_subnormal = true;
             cdf = cdf + pk;
         }
     }
+    g_loop_k = k;
     if (!over && float(k) <= cap) return float(k);
     return floorf(cap * u_cap);
 }
 
 // background_single against poisson_loop: out[0] draws compared, out[1]
-// results that differ (bitwise).  Means from [0, 64] on several scales, caps
-// of 0, 1/2, 1, 3/2, 2, 5/2, 3, 10 and random, uniforms on the 2^-24 grid at
-// random and at every cumulative sum of the first 8 steps and its neighbours.
-extern "C" int hw_poisson_check(int n_lam, uint64_t seed, int64_t* out) {
+// results that differ (bitwise), out[2] draws whose loop went past k = 255,
+// out[3] draws whose loop made a subnormal pk.  Means from [0, 64] on
+// several scales, caps of 0, 1/2, 1, 3/2, 2, 5/2, 3, 10 and random, uniforms
+// on the 2^-24 grid at random and at every cumulative sum of the first 8
+// steps and its neighbours; then, for n_deep means spread over (0, 64], caps
+// of 300 and 1000, the uniforms 1, 1 - 2^-24 and 1 - 2^-23 (past a cdf that
+// stalls below them the loop runs to the cap) and every cumulative sum of the
+// whole loop, until pk underflows, with its neighbours.
+extern "C" int hw_poisson_check(int n_lam, int n_deep, uint64_t seed, int64_t* out) {
     uint64_t st = seed;
     auto unif = [&]() { return float(double(splitmix(st) >> 11) * 0x1p-53); };
     const float caps[] = {0.0f, 0.5f, 1.0f, 1.5f, 2.0f, 2.5f, 3.0f, 10.0f};
-    out[0] = out[1] = 0;
+    out[0] = out[1] = out[2] = out[3] = 0;
     auto cmp = [&](float lam, float cap, float u1, float u2) {
         const float a = background_single(lam, cap, u1, u2);
         const float b = poisson_loop(lam, cap, u1, u2);
         ++out[0];
+        out[2] += g_loop_k > 255;
+        out[3] += g_loop_subnormal;
         uint32_t ba, bb;
         std::memcpy(&ba, &a, 4);
         std::memcpy(&bb, &b, 4);
         if (ba != bb) ++out[1];
     };
+    for (int i = 0; i < n_deep; ++i) {
+        const float lam = 64.0f * float(i + 1) / float(n_deep);
+        const float u2 = uniform24(uint32_t(splitmix(st)));
+        for (float cap : {300.0f, 1000.0f})
+            for (float u : {1.0f, 1.0f - 0x1p-24f, 1.0f - 0x1p-23f}) cmp(lam, cap, u, u2);
+        float pk = fexp(-lam), cdf = pk;
+        for (int k = 1; k < 400 && pk > 0.0f; ++k) {
+            for (float u : {cdf, std::nextafter(cdf, 0.0f), std::nextafter(cdf, 2.0f)})
+                if (u > 0.0f && u <= 1.0f) cmp(lam, 300.0f, u, u2);
+            pk = (pk * lam) / float(k);
+            cdf = cdf + pk;
+        }
+    }
     for (int i = 0; i < n_lam; ++i) {
         const int scale = i % 4;
         float lam = scale == 0 ? unif() * 3.0f

@@ -122,9 +122,14 @@ def test_device_rows_equal_host_rows(hw, name):
 
 def test_poisson_peeled_steps_equal_the_loop(hw):
     """background_single (first two inversion steps without a branch) equals
-    the plain inversion loop bitwise, at random uniforms and at every
-    cumulative sum of the first steps and its float neighbours."""
-    out = np.zeros(2, np.int64)
-    assert hw.hw_poisson_check(200_000, ctypes.c_uint64(3), out.ctypes.data_as(_P(ctypes.c_int64))) == 0
-    assert out[0] > 3_000_000
-    assert out[1] == 0
+    the plain inversion loop bitwise: at random uniforms, at every cumulative
+    sum of the first steps and its float neighbours, and on long loops (past
+    k = 255 and into subnormal pk: uniforms next to 1, caps of 300 and
+    1000)."""
+    out = np.zeros(4, np.int64)
+    assert hw.hw_poisson_check(200_000, 2_000, ctypes.c_uint64(3), out.ctypes.data_as(_P(ctypes.c_int64))) == 0
+    compared, differ, past_255, subnormal = out
+    assert compared > 3_000_000
+    assert differ == 0
+    assert past_255 > 100
+    assert subnormal > 100
