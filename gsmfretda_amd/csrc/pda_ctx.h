@@ -304,7 +304,7 @@ hipError_t launch_rb_walk(bool scores, int n, uint32_t grid, uint32_t replay_gri
                           hipStream_t stream);
 // dynamic LDS bytes (mode: 0 forward, 1 adjoint, 2 Jacobian)
 int rb_unit_threads();
-size_t rb_unit_lds_bytes(int B, int mode);
+size_t rb_unit_lds_bytes(int B, int mode, int n);
 size_t rb_big_lds_bytes(int B, int cap, int mode, bool wide);
 size_t rb_memo_lds_bytes(int n);
 hipError_t launch_rb_table(const RbModel* rbm, const double* lf, float* pmf_tab, int n,

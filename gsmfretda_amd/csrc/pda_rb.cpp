@@ -261,7 +261,7 @@ int pass_shape(pda_ctx* ctx, int mode, int n, bool wide, int B, int cap, uint32_
                PassShape& ps) {
     static const char* names[] = {"forward", "adjoint", "Jacobian"};
     constexpr size_t kStatic = sizeof(RbModel) + 64;  // the kernels' static LDS
-    const size_t lds = rb_unit_lds_bytes(B, mode);
+    const size_t lds = rb_unit_lds_bytes(B, mode, n);
     if (lds + kStatic > ctx->lds_per_block)
         return fail(PDA_E_DATA, "%d bins exceed the LDS of the RB %s pass", B, names[mode]);
     int per = 0;

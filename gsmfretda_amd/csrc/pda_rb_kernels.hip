@@ -58,11 +58,10 @@ constexpr int kRbMemoTab = kRbMemoCap * (kRbMemoCap + 1) / 2;
 // each over a contiguous share of the units: one queue per XCD).
 constexpr int kRbUnitWaves = 4;
 constexpr uint32_t kRbUnitCounter = kMaxQueues * kCounterStride;
-// Jacobian rows in global memory: kRbMaxStates efficiency columns, then
-// kRbMaxPairs rates; in LDS one more (odd rows of doubles spread the rows'
-// entries over the banks).
+// Jacobian rows in global memory, kRbJacCols doubles each: the set's n
+// efficiency columns, then its n (n - 1) rates (rb_big_kernel's LDS rows
+// hold the rates from column kRbMaxStates).
 constexpr int kRbJacCols = kRbMaxStates + kRbMaxPairs;
-constexpr int kRbJacRow = kRbJacCols + 1;
 
 constexpr int kRbFwd = 0, kRbGrad = 1, kRbJac = 2;
 
@@ -487,19 +486,24 @@ __device__ __forceinline__ int wave_owner(int pos, int j, int* pown) {
 }
 
 // Per-wave LDS of rb_unit_kernel (doubles), derivative passes: S - Sbar per
-// unit and rate.
-__host__ __device__ inline int rb_unit_wave_doubles(int mode) {
-    return mode == kRbFwd ? 0 : kWave * kRbMaxPairs;
+// unit and rate pair of the set's n states.
+__host__ __device__ inline int rb_unit_wave_doubles(int mode, int n) {
+    return mode == kRbFwd ? 0 : kWave * n * (n - 1);
 }
+// The Jacobian's LDS row per bin: n efficiency columns, then the n (n - 1)
+// rates, padded to an odd number of doubles (odd rows spread the rows'
+// entries over the banks).
+__host__ __device__ constexpr int rb_unit_jac_row(int n) { return (n + n * (n - 1)) | 1; }
 // Dynamic LDS of rb_unit_kernel: the 8-byte head (the forward's fixed-point
-// histogram, the adjoint's cotangent, the Jacobian's [B][kRbJacRow] rows),
-// rounded to 16 bytes, then the waves' regions.
-__host__ __device__ inline int rb_unit_head(int B, int mode) {
-    return ((mode == kRbJac ? B * kRbJacRow : B) + 1) & ~1;
+// histogram, the adjoint's cotangent, the Jacobian's [B][rb_unit_jac_row]
+// rows), rounded to 16 bytes, then the waves' regions.  Sized to the set's
+// states: the 3-state Jacobian fits seven four-wave workgroups per CU.
+__host__ __device__ inline int rb_unit_head(int B, int mode, int n) {
+    return ((mode == kRbJac ? B * rb_unit_jac_row(n) : B) + 1) & ~1;
 }
-__host__ __device__ inline size_t rb_unit_lds(int B, int mode) {
-    return sizeof(double) * (size_t(rb_unit_head(B, mode)) +
-                             size_t(kRbUnitWaves) * size_t(rb_unit_wave_doubles(mode)));
+__host__ __device__ inline size_t rb_unit_lds(int B, int mode, int n) {
+    return sizeof(double) * (size_t(rb_unit_head(B, mode, n)) +
+                             size_t(kRbUnitWaves) * size_t(rb_unit_wave_doubles(mode, n)));
 }
 
 // One (unit, outcome) element of a window: its unit's lane, outcome, and
@@ -541,7 +545,7 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
         uint32_t* dst = reinterpret_cast<uint32_t*>(&sR);
         for (uint32_t i = uint32_t(tid); i < sizeof(RbModel) / 4; i += blockDim.x) dst[i] = src[i];
     }
-    const int head = rb_unit_head(B, kMode);
+    const int head = rb_unit_head(B, kMode, kN);
     unsigned long long* s_h = reinterpret_cast<unsigned long long*>(s_unit);
     double* s_d = s_unit;
     for (int i = tid; i < head; i += int(blockDim.x)) {
@@ -551,7 +555,8 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
     }
     __syncthreads();
     constexpr int n = kN, K = kN * (kN - 1);  // states and rate pairs of the set
-    double* w_x = s_unit + head + wave * rb_unit_wave_doubles(kMode);
+    constexpr int kRow = rb_unit_jac_row(kN);
+    double* w_x = s_unit + head + wave * rb_unit_wave_doubles(kMode, kN);
 
     const uint32_t nq = gridDim.x < kAuxQueues ? gridDim.x : kAuxQueues;
     const uint32_t queue = blockIdx.x % nq;
@@ -587,7 +592,7 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
             float F = 0.0f;
             // derivative passes: the unit's score less its burst's mean, in
             // the wave's LDS
-            double* w_sc = w_x + lane * kRbMaxPairs;
+            double* w_sc = w_x + lane * K;
             if (memo) {
                 const uint32_t rk = rb_rank(mm, n, kRbMemoCap - 1);
                 offP = rank_offP[rk];
@@ -661,15 +666,15 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
                     // the sum of G(bin(a)) P(a) over the unit's outcomes)
                     if (e.act && b0 >= 0) {
                         const double hv = G0 * double(e.P);
-                        const double* scu = w_x + e.own * kRbMaxPairs;
+                        const double* scu = w_x + e.own * K;
 #pragma unroll
                         for (int k = 0; k < K; ++k)
                             gk[k] += hv * scu[k];
                     }
                 } else {
                     if (b0 >= 0) {
-                        double* row = s_d + b0 * kRbJacRow + kRbMaxStates;
-                        const double* scu = w_x + e.own * kRbMaxPairs;
+                        double* row = s_d + b0 * kRow + kN;
+                        const double* scu = w_x + e.own * K;
 #pragma unroll
                         for (int k = 0; k < K; ++k)
                             atomicAdd(&row[k], double(e.P) * scu[k]);
@@ -680,8 +685,8 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
                             const uint32_t m = rb_count(ci, s);
                             if (m > 0u) {
                                 const double w = double(m) * double(e.pd[s]);
-                                if (b1 >= 0) atomicAdd(&s_d[b1 * kRbJacRow + s], w);
-                                if (b0 >= 0) atomicAdd(&s_d[b0 * kRbJacRow + s], -w);
+                                if (b1 >= 0) atomicAdd(&s_d[b1 * kRow + s], w);
+                                if (b0 >= 0) atomicAdd(&s_d[b0 * kRow + s], -w);
                             }
                         }
                     }
@@ -724,8 +729,7 @@ __global__ __launch_bounds__(kWave * kRbUnitWaves) __attribute__((amdgpu_waves_p
         const int w = n + K;
         for (int i = tid; i < B * w; i += int(blockDim.x)) {
             const int b = i / w, col = i - b * w;
-            const double v = col < n ? s_d[b * kRbJacRow + col]
-                                     : f * s_d[b * kRbJacRow + kRbMaxStates + (col - n)];
+            const double v = col < n ? s_d[b * kRow + col] : f * s_d[b * kRow + col];
             if (v != 0.0) atomicAdd(&grad[size_t(b) * kRbJacCols + size_t(col)], v);
         }
     }
@@ -1104,7 +1108,7 @@ hipError_t launch_rb_walk(bool scores, int n, uint32_t grid, uint32_t replay_gri
 }
 
 int rb_unit_threads() { return kWave * kRbUnitWaves; }
-size_t rb_unit_lds_bytes(int B, int mode) { return rb_unit_lds(B, mode); }
+size_t rb_unit_lds_bytes(int B, int mode, int n) { return rb_unit_lds(B, mode, n); }
 size_t rb_big_lds_bytes(int B, int cap, int mode, bool wide) { return rb_big_lds(B, cap, mode, wide); }
 size_t rb_memo_lds_bytes(int n) { return rb_memo_lds(n); }
 
@@ -1142,7 +1146,7 @@ hipError_t launch_rb_unit(int mode, int n, uint32_t grid, const RbModel* rbm, co
                           const float* poolP, unsigned long long* hist_acc, double* grad,
                           uint32_t* work, hipStream_t stream) {
     if (n < 2 || n > kRbMaxStates) return hipErrorInvalidValue;
-    const size_t lds = rb_unit_lds(a.B, mode);
+    const size_t lds = rb_unit_lds(a.B, mode, n);
 #define PDA_RB_UNIT(M, N)                                                                      \
     launch_rb_unit_n<M, N>(grid, lds, rbm, a, cnt, F, S, Sbar, g, rank_offP, rank_offD, poolP, \
                            hist_acc, grad, work, stream)
