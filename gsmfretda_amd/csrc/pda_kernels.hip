@@ -127,6 +127,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     WalkState<kFastForm> st{};
     UnitKey key{a.seed, a.set, 0u, 0u};
     BurstDesc d{0u, 0u, 0u, 0.0f};
+    const UnitDiv divR = make_unit_div(a.R);
     bool drained = false;
     while (true) {
         bool need = !busy;
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
             const uint32_t avail = pool_end - pool_next;
             if (need && rank < avail) {
                 u = kReplay ? replay[pool_next + rank] : pool_next + rank;
-                const uint32_t bl = u / a.R;
+                const uint32_t bl = unit_div(u, divR);
                 const uint32_t b = a.begin + bl;
                 d = bursts[b];
                 key.r = u - bl * a.R;
@@ -383,6 +384,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     WalkState<kFastForm> st{};
     UnitKey key{seed, 0u, 0u, 0u};
     BurstDesc d{0u, 0u, 0u, 0.0f};
+    const UnitDiv divR = make_unit_div(a.R);
     bool drained = false;
     while (true) {
         bool need = !busy;
@@ -424,7 +426,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
                 u = pool_next + rank;
                 my_slot = cur;
                 my_set = pool_set;
-                const uint32_t bl = u / a.R;
+                const uint32_t bl = unit_div(u, divR);
                 const uint32_t b = a.begin + bl;
                 d = bursts[b];
                 key.set = set_base + my_set * set_stride;

@@ -188,6 +188,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(7))) void
     WalkState<true> st{};
     UnitKey key{a.seed, a.set, 0u, 0u};
     BurstDesc d{0u, 0u, 0u, 0.0f};
+    const UnitDiv divR = make_unit_div(a.R);
     bool drained = false;
     while (true) {
         bool need = !busy;
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(7))) void
             const uint32_t avail = pool_end - pool_next;
             if (need && rank < avail) {
                 u = pool_next + rank;
-                const uint32_t bl = u / a.R;
+                const uint32_t bl = unit_div(u, divR);
                 const uint32_t b = a.begin + bl;
                 d = bursts[b];
                 key.r = u - bl * a.R;

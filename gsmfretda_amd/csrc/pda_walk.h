@@ -182,6 +182,25 @@ PDA_HD int64_t binomial_fast(int n, int s, const Model& M, const BinomGuide& g, 
     return binomial_guided(n, s, M, g, hi, lo);
 }
 
+// Division of a unit index by the resample count without a divide
+// instruction sequence: for d >= 1, l = ceil(log2 d) and m = ceil(2^(31+l) / d)
+// (< 2^32), floor(u m / 2^(31+l)) = floor(u / d) for every u < 2^31 (m d =
+// 2^(31+l) + e with 0 <= e < d, so u m / 2^(31+l) exceeds u / d by less than
+// u / 2^(31+l) * e / d < 2^-l <= 1 / d, which cannot reach the next integer).
+// Unit indices stay below 2^31 (every run checks it).  Built once per wave.
+struct UnitDiv {
+    uint32_t m, sh;
+};
+PDA_HD UnitDiv make_unit_div(uint32_t d) {
+    uint32_t l = 0;
+    while (l < 32u && (uint64_t(1) << l) < uint64_t(d)) ++l;
+    const uint64_t m = ((uint64_t(1) << (31u + l)) + uint64_t(d) - 1u) / uint64_t(d);
+    return UnitDiv{uint32_t(m), 31u + l};
+}
+PDA_HD uint32_t unit_div(uint32_t u, const UnitDiv& q) {
+    return uint32_t((uint64_t(u) * uint64_t(q.m)) >> q.sh);
+}
+
 template <class T>
 struct NoDeduce {
     using type = T;

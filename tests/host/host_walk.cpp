@@ -156,6 +156,39 @@ extern "C" int hw_poisson_check(int n_lam, int n_deep, uint64_t seed, int64_t* o
 }
 
 
+// unit_div against the integer division: out[0] quotients compared, out[1]
+// that differ.  Every divisor 1 .. 4096 and n_rand random ones (up to
+// 2^32 - 1), each at the numerators 0, 1, d - 1, d, d + 1, multiples of d and
+// their neighbours near 2^31, 2^31 - 1, and n_u random ones below 2^31.
+extern "C" int hw_unit_div_check(int n_rand, int n_u, uint64_t seed, int64_t* out) {
+    uint64_t st = seed;
+    out[0] = out[1] = 0;
+    auto check = [&](uint32_t d) {
+        const UnitDiv q = make_unit_div(d);
+        auto one = [&](uint64_t u) {
+            if (u >= (uint64_t(1) << 31)) return;
+            ++out[0];
+            if (unit_div(uint32_t(u), q) != uint32_t(u) / d) ++out[1];
+        };
+        for (uint64_t u : {uint64_t(0), uint64_t(1), uint64_t(d) - 1, uint64_t(d), uint64_t(d) + 1,
+                           (uint64_t(1) << 31) - 1})
+            one(u);
+        const uint64_t top = ((uint64_t(1) << 31) - 1) / d * d;
+        for (uint64_t k = 0; k < 4 && top >= k * d; ++k)
+            for (int64_t e = -1; e <= 1; ++e) one(top - k * d + uint64_t(e));
+        for (int i = 0; i < n_u; ++i) one(splitmix(st) & 0x7FFFFFFFull);
+    };
+    for (uint32_t d = 1; d <= 4096; ++d) check(d);
+    for (int i = 0; i < n_rand; ++i) {
+        const uint32_t d = uint32_t(splitmix(st) >> (32 + splitmix(st) % 32));
+        if (d) check(d);
+    }
+    check(0xFFFFFFFFu);
+    check(0x80000000u);
+    check(0x80000001u);
+    return 0;
+}
+
 // The guided draw against the inversion loop of binomial_table on one
 // model: for every state and count 1..127, at every threshold pair +- 3 grid
 // steps and at n_rand random grid indices.  out: [0] draws compared, [1]

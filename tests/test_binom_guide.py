@@ -133,3 +133,14 @@ def test_poisson_peeled_steps_equal_the_loop(hw):
     assert differ == 0
     assert past_255 > 100
     assert subnormal > 100
+
+
+def test_unit_division_by_multiply_and_shift(hw):
+    """unit_div (a multiply by ceil(2^(31+l) / R) and a shift, pda_walk.h),
+    which the walks use for the burst of a unit index, equals u / R for every
+    R up to 4096 and random R up to 2^32 - 1, at the edges (0, R - 1, R, the
+    multiples of R next to 2^31, 2^31 - 1) and at random u below 2^31."""
+    out = np.zeros(2, np.int64)
+    assert hw.hw_unit_div_check(20_000, 200, ctypes.c_uint64(9), out.ctypes.data_as(_P(ctypes.c_int64))) == 0
+    assert out[0] > 4_000_000
+    assert out[1] == 0
