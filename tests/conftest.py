@@ -31,7 +31,8 @@ def build_hostwalk():
     csrc = os.path.join(ROOT, "gsmfretda_amd", "csrc")
     srcs = [os.path.join(ROOT, "tests", "host", "host_walk.cpp"), os.path.join(csrc, "pda_layout.cpp")]
     newest = max(os.path.getmtime(p) for p in srcs + [
-        os.path.join(csrc, f) for f in ("pda_walk.h", "pda_math.h", "pda_types.h", "pda_layout.h")])
+        os.path.join(csrc, f) for f in ("pda_walk.h", "pda_math.h", "pda_types.h", "pda_layout.h",
+                                     "pda_rb.h")])
     if os.path.exists(HOSTWALK) and os.path.getmtime(HOSTWALK) >= newest:
         return HOSTWALK
     os.makedirs(os.path.dirname(HOSTWALK), exist_ok=True)
