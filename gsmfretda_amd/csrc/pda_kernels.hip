@@ -62,11 +62,15 @@ __device__ __forceinline__ void tally(int32_t* s_hist, const RunArgs& a, const W
 // atomics of a launch spread over 8 x B addresses).  No completion ticket
 // and no fence: the finish kernel, launched behind on the same stream, reads
 // the rows.
-// The fast form is held to 72 VGPRs (7 waves per SIMD, no spills to
-// scratch): its window loop waits on dependent loads, and one more resident
-// wave per SIMD hides more of that latency than the register it gives up
-// costs (-2 % walk time on the 3-state workload).
-constexpr int kFastWavesPerEU = 7;
+// The fast forms are held to a register count that keeps many waves
+// resident: their window loop waits on dependent loads, and each more
+// resident wave per SIMD hides more of that latency than the registers it
+// gives up cost.  The single-set walk fits 64 VGPRs (8 waves per SIMD, no
+// spill to scratch; -2 % walk time against 7 waves on the 3-state
+// workload, the LDS of 32 workgroups exactly 160 KB); the population walk
+// keeps 72 (7 waves), where 64 would spill.
+constexpr int kFastWavesPerEU = 8;
+constexpr int kPopFastWavesPerEU = 7;
 
 template <int kPass>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
@@ -356,7 +360,7 @@ __device__ __forceinline__ void flush_hist(int32_t* h, int32_t* __restrict__ his
 // else by the inversion loop from each set's BinomTable.
 template <int kPass, bool kGuided>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
-    kPass == kPassFast ? kFastWavesPerEU : 1))) void pop_walk_kernel(
+    kPass == kPassFast ? kPopFastWavesPerEU : 1))) void pop_walk_kernel(
     const BurstDesc* __restrict__ bursts, const uint32_t* __restrict__ ticks,
     const uint2* __restrict__ links, const PopControl* __restrict__ ctl,
     const ModelBlob* __restrict__ blobs, const BinomGuide guide, const RunArgs a,
